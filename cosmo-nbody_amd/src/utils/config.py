@@ -1,0 +1,128 @@
+"""Simulation configuration.
+
+Module-level constants carry the reference's defaults with the same names
+(src/utils/config.py:9-35), so ``import src.utils.config as consts`` and
+``SimulationEngine(consts, state)`` work as in the reference.  ``Config`` is
+a dataclass with the same attribute names plus the knobs this build adds;
+``preset()`` returns the named workloads:
+
+  reference_defaults  the reference's own constants (theta 0.7, eps 1e-3)
+  small_32            N = 32^3 Zel'dovich ICs, theta 0.5, eps 0.02/32, 5 steps
+  repo_128            N = 128^3, theta 0.5, eps 0.02/128, 20 steps, 1024^2 splat
+  clustered_128       N = 128^3 evolved 300 steps, theta swept (precision study)
+  learned_128         N = 128^3 with the learned per-group opening angle
+  large_256           N = 256^3, 50 steps, 2048^2 splat, learned theta
+
+Time stepping follows the reference schedule: da = (A_END - A_START) /
+N_STEPS with N_STEPS = 2500 (src/core/engine.py:46); RUN_STEPS is how many of
+those steps a workload runs.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+# --- SIMULATION PARAMETERS (reference src/utils/config.py:9-14) ---
+N_GRID = 128
+N_BODIES = N_GRID ** 3
+BOX_SIZE = 1.0
+SOFTENING = 0.001
+N_STEPS = 2500
+A_START, A_END = 0.02, 1.0
+
+# --- RENDER SETTINGS (:16-21) ---
+ROTATION_SPEED = 0.001
+CAM_DIST_MULT = 1.8
+FOV = 60.0
+RES = 1024
+OUTPUT_DIR = "render_output"
+
+# --- LAUNCH CONFIG (:23-27); TPB/BPG only size the [off, off+B*T) range ---
+TPB = 256
+BPG = (N_BODIES + TPB - 1) // TPB
+BATCH_SIZE = int(5_000)
+CACHE_SIZE = 256
+
+# --- PHYSICS CONSTANTS (:29-35) ---
+THETA = 0.7
+G = 1.0
+OMEGA_M = 1.0
+OMEGA_L = 0.692
+H0 = 67.8
+STACK_SIZE_CHILD = 128
+
+# --- additions of this build ---
+SEED = 42                  # Zel'dovich phase seed (reference cosmology.py:25)
+MORTON_MASKS = "full21"    # "full21" (21 bits/axis) or "ref10" (morton.py:33-36 as written)
+RUN_STEPS = N_STEPS
+RENDER_FRAC_BITS = 20      # fixed-point fraction bits of the density bins
+COMPENSATED = False        # Neumaier-compensated far-field accumulation
+LEARNED_THETA = False
+
+
+@dataclasses.dataclass
+class Config:
+    N_GRID: int = N_GRID
+    BOX_SIZE: float = BOX_SIZE
+    SOFTENING: float = SOFTENING
+    N_STEPS: int = N_STEPS
+    A_START: float = A_START
+    A_END: float = A_END
+    ROTATION_SPEED: float = ROTATION_SPEED
+    CAM_DIST_MULT: float = CAM_DIST_MULT
+    FOV: float = FOV
+    RES: int = RES
+    OUTPUT_DIR: str = OUTPUT_DIR
+    TPB: int = TPB
+    BATCH_SIZE: int = BATCH_SIZE
+    CACHE_SIZE: int = CACHE_SIZE
+    THETA: float = THETA
+    G: float = G
+    OMEGA_M: float = OMEGA_M
+    OMEGA_L: float = OMEGA_L
+    H0: float = H0
+    STACK_SIZE_CHILD: int = STACK_SIZE_CHILD
+    SEED: int = SEED
+    MORTON_MASKS: str = MORTON_MASKS
+    RUN_STEPS: int = RUN_STEPS
+    RENDER_FRAC_BITS: int = RENDER_FRAC_BITS
+    COMPENSATED: bool = COMPENSATED
+    LEARNED_THETA: bool = LEARNED_THETA
+    NAME: str = "reference_defaults"
+
+    @property
+    def N_BODIES(self) -> int:
+        return self.N_GRID ** 3
+
+    @property
+    def BPG(self) -> int:
+        return (self.N_BODIES + self.TPB - 1) // self.TPB
+
+    @property
+    def DA(self) -> float:
+        return (self.A_END - self.A_START) / self.N_STEPS
+
+    def replace(self, **kw) -> "Config":
+        return dataclasses.replace(self, **kw)
+
+
+def _plain(n_grid: int, **kw) -> Config:
+    base = dict(N_GRID=n_grid, THETA=0.5, SOFTENING=0.02 / n_grid)
+    base.update(kw)
+    return Config(**base)
+
+
+PRESETS = {
+    "reference_defaults": lambda: Config(NAME="reference_defaults"),
+    "small_32": lambda: _plain(32, RUN_STEPS=5, NAME="small_32"),
+    "repo_128": lambda: _plain(128, RUN_STEPS=20, RES=1024, NAME="repo_128"),
+    "clustered_128": lambda: _plain(128, RUN_STEPS=300, NAME="clustered_128"),
+    "learned_128": lambda: _plain(128, RUN_STEPS=20, LEARNED_THETA=True, NAME="learned_128"),
+    "large_256": lambda: _plain(256, RUN_STEPS=50, RES=2048, LEARNED_THETA=True,
+                                NAME="large_256"),
+}
+
+
+def preset(name: str, **overrides) -> Config:
+    if name not in PRESETS:
+        raise KeyError(f"unknown preset {name!r}; known: {sorted(PRESETS)}")
+    return PRESETS[name]().replace(**overrides)

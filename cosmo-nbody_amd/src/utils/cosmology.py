@@ -1,0 +1,65 @@
+"""Zel'dovich initial conditions and the Hubble rate.
+
+``generate_zeldovich_ics`` restates src/utils/cosmology.py:12-73 on the host
+with numpy: P(k) = k^-2.5, uniform phases in [0, 2pi), displacement
+d = Re ifftn(i k / (k^2 + 1e-10) * sqrt(P) e^{i phi}), rescaled to
+std(dx) = 0.3 / n_grid, added to the ij-ordered lattice and wrapped into the
+unit box; velocities d * H(a_start) * Om(a_start)^0.55 * a_start / 20.  The
+phases come from numpy's PCG64 seeded with config.SEED, so the realisation
+differs from CuPy's generator; the statistics (std, zero mean, v/d ratio)
+are those of the reference.  Positions and velocities are returned as FP32.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def get_hubble_param(a, config):
+    """src/utils/cosmology.py:76-85: H0 * sqrt(Om a^-3 + OL), FP64."""
+    return config.H0 * np.sqrt(config.OMEGA_M * a ** -3 + config.OMEGA_L)
+
+
+def velocity_coupling(config) -> float:
+    """src/utils/cosmology.py:64-67."""
+    a0 = config.A_START
+    h_start = config.H0 * np.sqrt(config.OMEGA_M * a0 ** -3 + config.OMEGA_L)
+    om_a = config.OMEGA_M * a0 ** -3 / (config.OMEGA_M * a0 ** -3 + config.OMEGA_L)
+    return float(h_start * om_a ** 0.55 * a0 / 20)
+
+
+def displacement_field(n_grid: int, seed: int):
+    """Unscaled Zel'dovich displacement (cosmology.py:27-46), FP64 [3, n^3]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    k = np.fft.fftfreq(n_grid) * n_grid
+    kx, ky, kz = np.meshgrid(k, k, k, indexing="ij")
+    k2 = kx ** 2 + ky ** 2 + kz ** 2
+    k_mag = np.sqrt(k2)
+    k_mag[0, 0, 0] = 1e-10
+    P = k_mag ** -2.5
+    P[0, 0, 0] = 0
+    phase = rng.uniform(0, 2 * np.pi, (n_grid, n_grid, n_grid))
+    delta_k = np.sqrt(P) * (np.cos(phase) + 1j * np.sin(phase))
+    del phase, P
+    k_mag2 = k_mag ** 2 + 1e-10
+    out = np.empty((3, n_grid ** 3), np.float64)
+    for c, kc in enumerate((kx, ky, kz)):
+        term = 1j * kc / k_mag2
+        term[0, 0, 0] = 0
+        out[c] = np.fft.ifftn(term * delta_k).real.reshape(-1)
+    return out
+
+
+def generate_zeldovich_ics(config, seed=None):
+    """Returns ([pos_x, pos_y, pos_z], [vel_x, vel_y, vel_z]) as FP32 numpy."""
+    n_grid = config.N_GRID
+    seed = config.SEED if seed is None else seed
+    d = displacement_field(n_grid, seed)
+    scale = (1.0 / n_grid) * 0.3 / np.std(d[0])
+    d *= scale
+    lin = np.linspace(0, 1, n_grid, endpoint=False)
+    gx, gy, gz = np.meshgrid(lin, lin, lin, indexing="ij")
+    grid = (gx.reshape(-1), gy.reshape(-1), gz.reshape(-1))
+    coupling = velocity_coupling(config)
+    pos = [((g + d[c]) % 1.0).astype(np.float32) for c, g in enumerate(grid)]
+    vel = [(d[c] * coupling).astype(np.float32) for c in range(3)]
+    return pos, vel

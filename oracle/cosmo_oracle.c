@@ -1,0 +1,631 @@
+/*
+ * cosmo_oracle.c -- CPU restatement of Cosmo-NBody's per-step gravity path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP kernels in
+ * cosmo-nbody_amd/csrc.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it.  Built with -ffp-contract=off so that every
+ * floating-point operation below is exactly the one written: FMAs appear
+ * only where fmaf() is called.
+ *
+ * Pinning (tests/test_oracle_*.py):
+ *   compute_codes     closed form, de-interleave round trip, REF10 low-bit
+ *                     invariant, known answers                      pinned
+ *   argsort_stable    numpy.argsort(kind="stable")                  pinned
+ *   build_tree        brute-force top-down radix split (unique codes),
+ *                     tree invariants; equal-code runs follow the
+ *                     reference tie-break as restated      pinned (unique)
+ *   compute_multipoles root mass = sum m, root CoM = mean position,
+ *                     AABB = min/max, recursive FP64 recomputation   pinned
+ *   normalize/find_root/top_tree_cache  closed form / brute-force BFS pinned
+ *   compute_forces    theta=0 -> direct sum (both acceptance tests),
+ *                     two-body closed form, FP64 vs FP32 agreement,
+ *                     convergence to the direct sum (AABB test), stack-
+ *                     overflow drop on a deep chain                 pinned
+ *   direct_sum        two-body / three-body closed form             pinned
+ *   kick_drift        closed form per element                       pinned
+ *   render_density    projection closed form, centre pixel, totals  pinned
+ */
+#include "cosmo_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---------------------------------------------------------------- Morton */
+
+/* morton.py:33-36, the masks as written (keeps the low 10 bits). */
+static inline uint64_t spread_ref10(uint64_t v) {
+  v = (v | (v << 16)) & 0x030000FFull;
+  v = (v | (v << 8)) & 0x0300F00Full;
+  v = (v | (v << 4)) & 0x030C30C3ull;
+  v = (v | (v << 2)) & 0x09249249ull;
+  return v;
+}
+
+/* 21-bit spread (notebooks/final.ipynb:428-434; README.md:44). */
+static inline uint64_t spread_full21(uint64_t v) {
+  v &= 0x1fffffull;
+  v = (v | (v << 32)) & 0x1f00000000ffffull;
+  v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+  v = (v | (v << 8)) & 0x100f00f00f00f00full;
+  v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
+  return v;
+}
+
+/* morton.py:57-60: uint64(x * 2^21).  x*2^21 is exact in FP32.  Values that
+ * are not > 0 (negatives, NaN) give 0; values >= 2^40 saturate to 2^40. */
+static inline uint64_t fixed21(float x) {
+  float s = x * 2097152.0f;
+  if (!(s > 0.0f)) return 0;
+  if (s >= 1099511627776.0f) return 1099511627776ull;
+  return (uint64_t)s;
+}
+
+void orc_compute_codes(const float *px, const float *py, const float *pz,
+                       uint64_t *codes, uint32_t *indices, int64_t n,
+                       int mask_set) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    uint64_t ix = fixed21(px[i]), iy = fixed21(py[i]), iz = fixed21(pz[i]);
+    uint64_t c;
+    if (mask_set == ORC_MASK_REF10)
+      c = spread_ref10(ix) | (spread_ref10(iy) << 1) | (spread_ref10(iz) << 2);
+    else
+      c = spread_full21(ix) | (spread_full21(iy) << 1) |
+          (spread_full21(iz) << 2);
+    codes[i] = c;
+    if (indices) indices[i] = (uint32_t)i; /* morton.py:91 */
+  }
+}
+
+/* ------------------------------------------------------------------ sort */
+
+typedef struct {
+  uint64_t key;
+  uint32_t idx;
+} kv_t;
+
+static int kv_cmp(const void *a, const void *b) {
+  const kv_t *x = (const kv_t *)a, *y = (const kv_t *)b;
+  if (x->key != y->key) return x->key < y->key ? -1 : 1;
+  return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+void orc_argsort_stable(const uint64_t *codes, uint32_t *perm, int64_t n) {
+  kv_t *a = (kv_t *)malloc(sizeof(kv_t) * (size_t)(n > 0 ? n : 1));
+  for (int64_t i = 0; i < n; ++i) {
+    a[i].key = codes[i];
+    a[i].idx = (uint32_t)i;
+  }
+  qsort(a, (size_t)n, sizeof(kv_t), kv_cmp);
+  for (int64_t i = 0; i < n; ++i) perm[i] = a[i].idx;
+  free(a);
+}
+
+/* ------------------------------------------------------------- tree build */
+
+void orc_reset_tree_buffers(int32_t *child, int32_t *parent, int32_t *counters,
+                            int32_t *root_idx, int64_t n_nodes) {
+  root_idx[0] = 0; /* tree_kernels.py:255-256 */
+  for (int64_t i = 0; i < n_nodes; ++i) {
+    child[2 * i] = -1;
+    child[2 * i + 1] = -1;
+    parent[i] = -1;
+    counters[i] = 0;
+  }
+}
+
+/* tree_kernels.py:17-34 */
+static inline int64_t clz64(uint64_t v) { return v ? __builtin_clzll(v) : 64; }
+
+/* tree_kernels.py:36-58, including the equal-code tie-break as written:
+ * 64 + (63 - clz64(i ^ j)). */
+static inline int64_t delta_fn(const uint64_t *codes, int64_t i, int64_t j,
+                               int64_t n) {
+  if (j < 0 || j >= n) return -1;
+  if (codes[i] == codes[j]) return 64 + (63 - clz64((uint64_t)(i ^ j)));
+  return clz64(codes[i] ^ codes[j]);
+}
+
+/* tree_kernels.py:60-129.  Children are computed in parallel; parent links
+ * are then written in increasing node order so that a malformed tree (a
+ * child claimed twice) still gives a deterministic parents array. */
+void orc_build_tree(const uint64_t *codes, int32_t *children, int32_t *parents,
+                    int64_t n) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n - 1; ++i) {
+    const int64_t idx = i + n;
+    const int64_t d_prev = delta_fn(codes, i, i - 1, n);
+    const int64_t d_next = delta_fn(codes, i, i + 1, n);
+    const int64_t dir = (d_next > d_prev) ? 1 : -1;
+    const int64_t min_delta = (d_next > d_prev) ? d_prev : d_next;
+    int64_t l_max = 2;
+    while (delta_fn(codes, i, i + l_max * dir, n) > min_delta) l_max *= 2;
+    int64_t l = 0;
+    for (int64_t t = l_max / 2; t >= 1; t /= 2)
+      if (delta_fn(codes, i, i + (l + t) * dir, n) > min_delta) l += t;
+    const int64_t j = i + l * dir;
+    const int64_t delta_node = delta_fn(codes, i, j, n);
+    int64_t s = 0;
+    for (int64_t t = l; t > 0;) {
+      const int64_t step = (t + 1) / 2;
+      if (delta_fn(codes, i, i + (s + step) * dir, n) > delta_node) s += step;
+      t -= step;
+    }
+    int64_t gamma = i + s * dir;
+    if (dir == -1) gamma -= 1;
+    const int64_t left = gamma, right = gamma + 1;
+    const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+    children[2 * idx] = (int32_t)(lo == left ? left : left + n);
+    children[2 * idx + 1] = (int32_t)(hi == right ? right : right + n);
+  }
+  for (int64_t i = 0; i < n - 1; ++i) {
+    const int64_t idx = i + n;
+    int32_t c0 = children[2 * idx], c1 = children[2 * idx + 1];
+    if (c0 >= 0 && c0 < 2 * n - 1) parents[c0] = (int32_t)idx;
+    if (c1 >= 0 && c1 < 2 * n - 1) parents[c1] = (int32_t)idx;
+  }
+}
+
+/* ----------------------------------------------------------- upward pass */
+
+static inline float minf_ref(float a, float b) { return (b < a) ? b : a; }
+static inline float maxf_ref(float a, float b) { return (b > a) ? b : a; }
+
+/* tree_kernels.py:152-191.  Executed as a serial climb: leaf i climbs while
+ * it is the second arrival at its parent.  Every parent is formed exactly
+ * once, after both children, as left + right, so the result equals that of
+ * any parallel arrival order. */
+void orc_compute_multipoles(const float *px, const float *py, const float *pz,
+                            const float *mass, const int32_t *children,
+                            const int32_t *parents, double *n_mass,
+                            double *n_com, float *n_min, float *n_max,
+                            int32_t *counts, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) {
+    const double m = (double)mass[i];
+    n_mass[i] = m;
+    n_com[3 * i + 0] = (double)px[i] * m;
+    n_com[3 * i + 1] = (double)py[i] * m;
+    n_com[3 * i + 2] = (double)pz[i] * m;
+    n_min[3 * i + 0] = n_max[3 * i + 0] = px[i];
+    n_min[3 * i + 1] = n_max[3 * i + 1] = py[i];
+    n_min[3 * i + 2] = n_max[3 * i + 2] = pz[i];
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t curr = i;
+    for (;;) {
+      const int32_t p = parents[curr];
+      if (p == -1) break;
+      if (counts[p]++ == 0) break;
+      const int32_t l = children[2 * p], r = children[2 * p + 1];
+      n_mass[p] = n_mass[l] + n_mass[r];
+      for (int k = 0; k < 3; ++k) {
+        n_com[3 * p + k] = n_com[3 * l + k] + n_com[3 * r + k];
+        n_min[3 * p + k] = minf_ref(n_min[3 * l + k], n_min[3 * r + k]);
+        n_max[3 * p + k] = maxf_ref(n_max[3 * l + k], n_max[3 * r + k]);
+      }
+      curr = p;
+    }
+  }
+}
+
+void orc_normalize_com(const double *n_mass, double *n_com, int64_t n_nodes) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n_nodes; ++i) {
+    if (n_mass[i] > 0) {
+      const double inv = 1.0 / n_mass[i];
+      n_com[3 * i + 0] *= inv;
+      n_com[3 * i + 1] *= inv;
+      n_com[3 * i + 2] *= inv;
+    }
+  }
+}
+
+void orc_find_root(const int32_t *parents, int32_t *root_idx,
+                   int64_t n_nodes) {
+  for (int64_t i = 0; i < n_nodes; ++i)
+    if (parents[i] == -1 && i > root_idx[0]) root_idx[0] = (int32_t)i;
+}
+
+void orc_build_top_tree_cache(const int32_t *child, const int32_t *root_idx,
+                              int32_t *top_nodes, int32_t *node_to_cache,
+                              int64_t cache_size) {
+  const int32_t root = root_idx[0];
+  if (root == -1) return;
+  top_nodes[0] = root;
+  node_to_cache[root] = 0;
+  int64_t read_idx = 0, write_idx = 1;
+  while (read_idx < write_idx && write_idx < cache_size) {
+    const int32_t curr = top_nodes[read_idx];
+    const int32_t c1 = child[2 * curr], c2 = child[2 * curr + 1];
+    if (c1 != -1 && write_idx < cache_size) {
+      top_nodes[write_idx] = c1;
+      node_to_cache[c1] = (int32_t)write_idx;
+      write_idx++;
+    }
+    if (c2 != -1 && write_idx < cache_size) {
+      top_nodes[write_idx] = c2;
+      node_to_cache[c2] = (int32_t)write_idx;
+      write_idx++;
+    }
+    read_idx++;
+  }
+  while (write_idx < cache_size) top_nodes[write_idx++] = -1;
+}
+
+int orc_validate_tree(const int32_t *children, const int32_t *parents,
+                      int64_t n) {
+  const int64_t M = 2 * n - 1;
+  int flags = 0;
+  int64_t n_root = 0, root = -1;
+  for (int64_t i = 0; i < M; ++i)
+    if (parents[i] == -1) {
+      n_root++;
+      root = i;
+    }
+  if (n_root != 1) flags |= 1;
+  if (n >= 2 && !(n_root == 1 && root == n)) flags |= 2;
+  for (int64_t k = n; k < M; ++k) {
+    for (int c = 0; c < 2; ++c) {
+      const int32_t ch = children[2 * k + c];
+      if (ch < 0 || ch >= M || parents[ch] != k) flags |= 4;
+    }
+    if (children[2 * k] == children[2 * k + 1]) flags |= 4;
+  }
+  for (int64_t k = 0; k < n; ++k)
+    if (children[2 * k] != -1 || children[2 * k + 1] != -1) flags |= 4;
+  /* reachability: DFS from the root with a visit bound */
+  if (n_root >= 1) {
+    int64_t start = (n >= 2) ? n : 0;
+    if (n_root == 1) start = root;
+    uint8_t *seen = (uint8_t *)calloc((size_t)M, 1);
+    int64_t *stk = (int64_t *)malloc(sizeof(int64_t) * (size_t)(M + 1));
+    int64_t top = 0, leaves = 0, visits = 0;
+    stk[top++] = start;
+    while (top > 0 && visits <= M) {
+      const int64_t v = stk[--top];
+      visits++;
+      if (v < 0 || v >= M || seen[v]) {
+        flags |= 8;
+        break;
+      }
+      seen[v] = 1;
+      if (v < n) {
+        leaves++;
+        continue;
+      }
+      if (top + 2 > M + 1) {
+        flags |= 8;
+        break;
+      }
+      stk[top++] = children[2 * v];
+      stk[top++] = children[2 * v + 1];
+    }
+    if (leaves != n) flags |= 8;
+    free(seen);
+    free(stk);
+  } else {
+    flags |= 8;
+  }
+  return flags;
+}
+
+/* --------------------------------------------------------------- forces */
+
+/* Acceptance tests.
+ * ORC_MAC_REF (physics_kernels.py:128-133): sx = x-extent of the tight AABB,
+ *   replaced by box when sx > box/2; accept iff sx^2 < theta^2 d^2, with d
+ *   the minimum-image distance to the CoM.
+ * ORC_MAC_AABB (no reference counterpart): s = largest extent of the tight
+ *   AABB, dl = |CoM - AABB centre|; accept iff (s/theta + dl)^2 < d^2 and
+ *   max_k |d_k| + s < box/2, so that no particle of an accepted node sits
+ *   across the minimum-image boundary from its CoM.
+ * Leaves are always accepted; interactions with d^2 <= 1e-7 are dropped. */
+
+/* physics_kernels.py:79-153, one particle, FP64 arithmetic on FP32 data
+ * (positions, node CoM/mass/AABB), as Numba promotes them. */
+static void bh_one_f64(const float *px, const float *py, const float *pz,
+                       const int32_t *children, const float *n_mass,
+                       const float *n_com, const float *n_min,
+                       const float *n_max, int mac, double theta, double G,
+                       double soft, double box, int32_t root, int64_t i,
+                       double out[3], int64_t *n_int, int64_t *n_vis) {
+  const double x = px[i], y = py[i], z = pz[i];
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  int32_t stack[64];
+  int top = 0;
+  int64_t ni = 0, nv = 0;
+  stack[top++] = root;
+  const double theta_sq = theta * theta;
+  const double inv_theta = 1.0 / theta;
+  while (top > 0) {
+    const int32_t node = stack[--top];
+    const double nm = n_mass[node];
+    const int32_t c1 = children[2 * node], c2 = children[2 * node + 1];
+    if (nm <= 0) continue;
+    double dx = (double)n_com[3 * node + 0] - x;
+    double dy = (double)n_com[3 * node + 1] - y;
+    double dz = (double)n_com[3 * node + 2] - z;
+    dx -= nearbyint(dx);
+    dy -= nearbyint(dy);
+    dz -= nearbyint(dz);
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    nv++;
+    int accept;
+    if (mac == ORC_MAC_REF) {
+      double sx = (double)(n_max[3 * node] - n_min[3 * node]); /* FP32 op */
+      sx = (sx > box * 0.5) ? box : sx;
+      accept = (sx * sx) < (theta_sq * d2);
+    } else {
+      double s = 0.0, dl2 = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        const double lo = n_min[3 * node + k], hi = n_max[3 * node + k];
+        const double off = (double)n_com[3 * node + k] - 0.5 * (hi + lo);
+        s = (hi - lo) > s ? (hi - lo) : s;
+        dl2 += off * off;
+      }
+      const double r_open = s * inv_theta + sqrt(dl2);
+      double dm = fabs(dx);
+      dm = fabs(dy) > dm ? fabs(dy) : dm;
+      dm = fabs(dz) > dm ? fabs(dz) : dm;
+      accept = (r_open * r_open < d2) && (dm + s < 0.5 * box);
+    }
+    if (c1 == -1 || accept) {
+      if (d2 > 1e-7) {
+        const double d_inv = 1.0 / sqrt(d2 + soft * soft);
+        const double f = G * nm * (d_inv * d_inv * d_inv);
+        ax += f * dx;
+        ay += f * dy;
+        az += f * dz;
+        ni++;
+      }
+    } else if (top + 1 < 64) {
+      if (c1 != -1) stack[top++] = c2;
+      if (c2 != -1) stack[top++] = c1;
+    }
+  }
+  out[0] = ax;
+  out[1] = ay;
+  out[2] = az;
+  if (n_int) *n_int = ni;
+  if (n_vis) *n_vis = nv;
+}
+
+/* Neumaier's compensated summation (Neumaier 1974): s + c carries the sum
+ * of all added terms with the rounding error of each addition kept in c. */
+static inline void neumaier_add(float *s, float *c, float x) {
+  const float t = *s + x;
+  if (fabsf(*s) >= fabsf(x))
+    *c += (*s - t) + x;
+  else
+    *c += (x - t) + *s;
+  *s = t;
+}
+
+/* The FP32 arithmetic of the HIP traversal.  Per node (computed once when
+ * the node records are packed): REF sx2 = sx*sx; AABB s = max extent and
+ * dl = sqrt(fma(oz,oz,fma(oy,oy,ox*ox))) with o = CoM - 0.5*(hi+lo).
+ * Per visit: d = CoM - p minus rint; d2 = fma(dz,dz,fma(dy,dy,dx*dx));
+ * REF: sx2 < (theta*theta)*d2;  AABB: r = fma(s, 1/theta, dl), r*r < d2 and
+ * max|d_k| + s < box/2; force 1/sqrt(d2+eps2) (IEEE here, hardware rsq on
+ * the GPU), f = (G*m)*((inv*inv)*inv), a = fma(f, d, a). */
+static void bh_one_f32(const float *px, const float *py, const float *pz,
+                       const int32_t *children, const float *n_mass,
+                       const float *n_com, const float *n_min,
+                       const float *n_max, int mac, float thf, float Gf,
+                       float softf, float boxf, int32_t root, int64_t i,
+                       int compensated, double out[3], int64_t *n_int,
+                       int64_t *n_vis) {
+  const float x = px[i], y = py[i], z = pz[i];
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  float cx = 0.0f, cy = 0.0f, cz = 0.0f; /* Neumaier compensation terms */
+  int32_t stack[64];
+  int top = 0;
+  int64_t ni = 0, nv = 0;
+  stack[top++] = root;
+  const float th2 = thf * thf;
+  const float inv_theta = 1.0f / thf;
+  const float eps2 = softf * softf;
+  const float half = boxf * 0.5f;
+  while (top > 0) {
+    const int32_t node = stack[--top];
+    const float nm = n_mass[node];
+    const int32_t c1 = children[2 * node], c2 = children[2 * node + 1];
+    if (nm <= 0.0f) continue;
+    float dx = n_com[3 * node + 0] - x;
+    float dy = n_com[3 * node + 1] - y;
+    float dz = n_com[3 * node + 2] - z;
+    dx = dx - rintf(dx);
+    dy = dy - rintf(dy);
+    dz = dz - rintf(dz);
+    const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    nv++;
+    int accept;
+    if (mac == ORC_MAC_REF) {
+      float sx = n_max[3 * node] - n_min[3 * node];
+      sx = (sx > half) ? boxf : sx;
+      const float sx2 = sx * sx;
+      accept = sx2 < th2 * d2;
+    } else {
+      float s = 0.0f, o[3];
+      for (int k = 0; k < 3; ++k) {
+        const float lo = n_min[3 * node + k], hi = n_max[3 * node + k];
+        const float e = hi - lo;
+        s = e > s ? e : s;
+        o[k] = n_com[3 * node + k] - 0.5f * (hi + lo);
+      }
+      const float dl = sqrtf(fmaf(o[2], o[2], fmaf(o[1], o[1], o[0] * o[0])));
+      const float r = fmaf(s, inv_theta, dl);
+      float dm = fabsf(dx);
+      dm = fabsf(dy) > dm ? fabsf(dy) : dm;
+      dm = fabsf(dz) > dm ? fabsf(dz) : dm;
+      accept = (r * r < d2) && (dm + s < half);
+    }
+    if (c1 == -1 || accept) {
+      if (d2 > 1e-7f) {
+        const float r2 = d2 + eps2;
+        const float inv = 1.0f / sqrtf(r2);
+        const float f = (Gf * nm) * ((inv * inv) * inv);
+        if (compensated) {
+          neumaier_add(&ax, &cx, f * dx);
+          neumaier_add(&ay, &cy, f * dy);
+          neumaier_add(&az, &cz, f * dz);
+        } else {
+          ax = fmaf(f, dx, ax);
+          ay = fmaf(f, dy, ay);
+          az = fmaf(f, dz, az);
+        }
+        ni++;
+      }
+    } else if (top + 1 < 64) {
+      if (c1 != -1) stack[top++] = c2;
+      if (c2 != -1) stack[top++] = c1;
+    }
+  }
+  out[0] = ax + cx;
+  out[1] = ay + cy;
+  out[2] = az + cz;
+  if (n_int) *n_int = ni;
+  if (n_vis) *n_vis = nv;
+}
+
+void orc_compute_forces(const float *px, const float *py, const float *pz,
+                        const int32_t *children, const float *n_mass,
+                        const float *n_com, const float *n_min,
+                        const float *n_max, float *force, double theta,
+                        double G, double soft, double box, int32_t root,
+                        int64_t off, int64_t count, int64_t n, int precision,
+                        int mac, const float *theta_g, int64_t *n_inter,
+                        int64_t *n_visit, double *force64) {
+  int64_t hi = off + count;
+  if (hi > n) hi = n;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = off; i < hi; ++i) {
+    double a[3];
+    int64_t ni = 0, nv = 0;
+    if (precision == 64) {
+      const double th = theta_g ? (double)theta_g[i] : theta;
+      bh_one_f64(px, py, pz, children, n_mass, n_com, n_min, n_max, mac, th, G,
+                 soft, box, root, i, a, &ni, &nv);
+    } else {
+      const float th = theta_g ? theta_g[i] : (float)theta;
+      bh_one_f32(px, py, pz, children, n_mass, n_com, n_min, n_max, mac, th,
+                 (float)G, (float)soft, (float)box, root, i, precision == 31,
+                 a, &ni, &nv);
+    }
+    force[3 * i + 0] = (float)a[0];
+    force[3 * i + 1] = (float)a[1];
+    force[3 * i + 2] = (float)a[2];
+    if (force64) {
+      force64[3 * i + 0] = a[0];
+      force64[3 * i + 1] = a[1];
+      force64[3 * i + 2] = a[2];
+    }
+    if (n_inter) n_inter[i] = ni;
+    if (n_visit) n_visit[i] = nv;
+  }
+}
+
+void orc_direct_sum(const float *px, const float *py, const float *pz,
+                    const float *mass, int64_t n, const int64_t *targets,
+                    int64_t n_targets, double G, double soft, double *acc) {
+  const double eps2 = soft * soft;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t t = 0; t < n_targets; ++t) {
+    const int64_t i = targets[t];
+    const double x = px[i], y = py[i], z = pz[i];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+      double dx = (double)px[j] - x, dy = (double)py[j] - y,
+             dz = (double)pz[j] - z;
+      dx -= nearbyint(dx);
+      dy -= nearbyint(dy);
+      dz -= nearbyint(dz);
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 > 1e-7) {
+        const double d_inv = 1.0 / sqrt(d2 + eps2);
+        const double f = G * (double)mass[j] * (d_inv * d_inv * d_inv);
+        ax += f * dx;
+        ay += f * dy;
+        az += f * dz;
+      }
+    }
+    acc[3 * t + 0] = ax;
+    acc[3 * t + 1] = ay;
+    acc[3 * t + 2] = az;
+  }
+}
+
+/* --------------------------------------------------------------- update */
+
+void orc_integrate_coeffs(double a, double H, double dt, float *c_drag,
+                          float *c_kick, float *c_drift) {
+  *c_drag = (float)(1.0 - H * dt);
+  *c_kick = (float)(dt / a);
+  *c_drift = (float)(dt / a);
+}
+
+void orc_kick_drift(float *px, float *py, float *pz, float *vx, float *vy,
+                    float *vz, const float *force, float c_drag, float c_kick,
+                    float c_drift, int64_t n) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    const float ux = vx[i] * c_drag + force[3 * i + 0] * c_kick;
+    const float uy = vy[i] * c_drag + force[3 * i + 1] * c_kick;
+    const float uz = vz[i] * c_drag + force[3 * i + 2] * c_kick;
+    vx[i] = ux;
+    vy[i] = uy;
+    vz[i] = uz;
+    float x = px[i] + ux * c_drift;
+    float y = py[i] + uy * c_drift;
+    float z = pz[i] + uz * c_drift;
+    px[i] = x - floorf(x);
+    py[i] = y - floorf(y);
+    pz[i] = z - floorf(z);
+  }
+}
+
+/* --------------------------------------------------------------- render */
+
+void orc_render_density(const float *px, const float *py, const float *pz,
+                        uint64_t *grid, const float *mvp, int32_t res,
+                        int64_t n, int frac_bits) {
+  const float resf = (float)res;
+  const float scale = ldexpf(1.0f, frac_bits);
+  for (int64_t i = 0; i < n; ++i) {
+    const float x = px[i], y = py[i], z = pz[i];
+    const float xc = ((mvp[0] * x + mvp[1] * y) + mvp[2] * z) + mvp[3];
+    const float yc = ((mvp[4] * x + mvp[5] * y) + mvp[6] * z) + mvp[7];
+    const float wc = ((mvp[12] * x + mvp[13] * y) + mvp[14] * z) + mvp[15];
+    if (wc > 0.01f) {
+      const float inv = 1.0f / wc;
+      const float sx = ((xc * inv + 1.0f) * 0.5f) * resf;
+      const float sy = ((1.0f - yc * inv) * 0.5f) * resf;
+      if (sx >= 0.0f && sx < resf && sy >= 0.0f && sy < resf) {
+        const float val = 2.5f * inv;
+        const uint64_t q = (uint64_t)rintf(val * scale);
+        grid[(int64_t)(int32_t)sy * res + (int32_t)sx] += q;
+      }
+    }
+  }
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void orc_set_num_threads(int t) {
+#ifdef _OPENMP
+  if (t > 0) omp_set_num_threads(t);
+#else
+  (void)t;
+#endif
+}
