@@ -1,0 +1,155 @@
+"""Throughput of the per-step gravity path on one MI355X.
+
+Workload (default): N = 128^3 Zel'dovich ICs (seed 42), theta = 0.5,
+eps = 0.02/128, reference time schedule; one step = Morton encode + radix
+sort + permute, tree build, FP64 upward pass + node packing, Barnes-Hut
+traversal (reference acceptance test, FP32), kick-drift update, and the
+1024^2 fixed-point density splat.  W untimed steps, then K timed steps
+between device synchronisations.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "cosmo-nbody_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 (vector, packed) / dense MFMA FP32 figure
+FLOP_PER_INTERACTION = 20     # standard particle-node interaction count
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--preset", default="repo_128")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def cpu_baseline(pos, vel, c, threads):
+    """One full step of the CPU oracle (C/OpenMP restatement of the reference
+    kernels, FP64 force arithmetic) on the host cores."""
+    import numpy as np
+
+    from oracle import oracle as orc
+    from src.utils.cosmology import get_hubble_param
+    if threads > 0:
+        orc.set_num_threads(threads)
+    used = orc.num_threads()
+    n = len(pos[0])
+    t0 = time.perf_counter()
+    codes, _ = orc.compute_codes(*pos)
+    perm = orc.argsort_stable(codes)
+    p = [x[perm] for x in pos]
+    v = [x[perm] for x in vel]
+    mass = np.full(n, 1.0 / n, np.float32)
+    children, parents = orc.build_tree(codes[perm])
+    nm, nc, lo, hi = orc.compute_multipoles(*p, mass, children, parents)
+    com = orc.normalize_com(nm, nc)
+    force = orc.compute_forces(*p, children, nm, com, lo, hi, c.THETA, c.G, c.SOFTENING,
+                               c.BOX_SIZE, n)
+    H = get_hubble_param(c.A_START, c)
+    orc.integrate(*p, *v, force, c.A_START, H, c.DA / (c.A_START * H) * 0.8)
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "particles/s", "cores": used, "kind": "port",
+            "sample": f"one full step (codes, sort, tree, FP64 upward pass, FP64 BH forces, "
+                      f"update) at N={n}, {dt:.2f} s on {used} host threads"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(this build runs on a single MI355X)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from src.core.engine import SimulationEngine
+    from src.core.state import SimulationState
+    from src.kernels.physics_kernels import integrate
+    from src.utils import config as cfg
+    from src.utils.cosmology import generate_zeldovich_ics
+
+    c = cfg.preset(args.preset)
+    pos, vel = generate_zeldovich_ics(c)
+    n = len(pos[0])
+    torch.cuda.set_device(0)
+    st = SimulationState(n, pos, vel)
+    eng = SimulationEngine(c, st)
+    eng.stats = torch.zeros(2, dtype=torch.int64, device="cuda")
+
+    def one_step(ev=None):
+        H, dt = eng.timestep()
+        if ev: ev[0].record()
+        st.sort_buffer_by_morton(c)
+        if ev: ev[1].record()
+        eng.tree_manager.build_packed(st.pos, st.mass, st.codes, c.BOX_SIZE, eng.mac)
+        if ev: ev[2].record()
+        eng.compute_forces()
+        if ev: ev[3].record()
+        integrate(*st.pos, *st.vel, st.force, eng.a, H, dt)
+        if ev: ev[4].record()
+        eng.camera.capture(eng.steps_done, *st.pos)
+        if ev: ev[5].record()
+        eng.a += eng.da
+        eng.steps_done += 1
+
+    for _ in range(args.warmup):
+        one_step()
+    torch.cuda.synchronize()
+    eng.stats.zero_()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        one_step(evs[k])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    names = ["sort", "tree", "forces", "update", "splat"]
+    stage = {nm: float(np.mean([evs[k][j].elapsed_time(evs[k][j + 1])
+                                for k in range(args.steps)])) for j, nm in enumerate(names)}
+    inter, visits = (int(x) for x in eng.stats.cpu().tolist())
+    ms = elapsed * 1e3 / args.steps
+    trav_s = stage["forces"] * 1e-3
+    flops = FLOP_PER_INTERACTION * inter / args.steps
+    achieved = flops / trav_s / 1e12
+    out = {
+        "metric": "ms/step + particles/s at N=128^3 (full step)",
+        "value": n * args.steps / elapsed,
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"{c.NAME}: N={n} Zel'dovich ICs seed {c.SEED}, theta={c.THETA}, "
+                               f"eps={c.SOFTENING:.4g}, MAC={c.MAC}, {c.RES}^2 splat, "
+                               f"reference schedule da={c.DA:.3g}",
+                   "n_bodies": n, "theta": c.THETA, "res": c.RES},
+        "stages_ms": stage,
+        "interactions_per_particle": inter / args.steps / n,
+        "visits_per_particle": visits / args.steps / n,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "k_traverse (FP32 VALU; 20 flop per accepted interaction)"},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(pos, vel, c, args.cpu_threads)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

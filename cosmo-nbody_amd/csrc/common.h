@@ -1,0 +1,74 @@
+// Shared helpers of the gfx950 kernels: status handling, launch geometry,
+// wave64 primitives and the context layout.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cosmo_abi.h"
+
+namespace cosmo {
+
+constexpr int kWave = 64;
+
+// Records the HIP error of the most recent failing call (thread-local) so
+// that cosmo_status_string() can name it.
+void set_hip_error(hipError_t e);
+
+#define COSMO_CHECK_LAUNCH()                        \
+  do {                                              \
+    hipError_t e_ = hipGetLastError();              \
+    if (e_ != hipSuccess) {                         \
+      ::cosmo::set_hip_error(e_);                   \
+      return COSMO_E_HIP;                           \
+    }                                               \
+  } while (0)
+
+#define COSMO_CHECK_HIP(call)                       \
+  do {                                              \
+    hipError_t e_ = (call);                         \
+    if (e_ != hipSuccess) {                         \
+      ::cosmo::set_hip_error(e_);                   \
+      return COSMO_E_HIP;                           \
+    }                                               \
+  } while (0)
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline unsigned blocks_for(int64_t n, int threads) {
+  return static_cast<unsigned>((n + threads - 1) / threads);
+}
+
+// Largest particle count any entry point accepts: node ids (2n-1) and the
+// tree-build search range (up to 4n) stay inside int32.
+constexpr int64_t kMaxBodies = int64_t(1) << 29;
+
+// Radix sort geometry: 256 threads, 16 keys each, 8-bit digits.
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Mask of lanes below this one.
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const int l = lane_id();
+  return (l == 0) ? 0ull : (~0ull >> (64 - l));
+}
+
+}  // namespace cosmo
+
+struct cosmo_ctx {
+  int device;
+  int64_t n_max;
+  int64_t max_tiles;
+  uint64_t *keys_alt;   // [n_max]
+  uint32_t *vals_a;     // [n_max]
+  uint32_t *vals_b;     // [n_max]
+  uint32_t *hist;       // [kRadix * max_tiles]
+  uint32_t *digit_tot;  // [kRadix]
+  cosmo_node *nodes;    // [2 n_max - 1]
+  int32_t *scratch_i32; // [4 n_max + 4] validator counters
+};

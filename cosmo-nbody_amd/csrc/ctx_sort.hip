@@ -1,0 +1,359 @@
+// Context, status strings, Morton encoding and the (code, index) radix sort.
+//
+// Reference: src/kernels/morton.py (compute_codes) and src/core/state.py:20-40
+// (cp.argsort of the codes followed by eight gathers).  Here the sort is a
+// stable LSD radix sort with 8-bit digits whose passes each read a tile of
+// 4096 keys per workgroup: a digit count, a per-digit scan over tiles, and a
+// scatter whose within-tile ranks come from wave64 match masks (8 ballots),
+// so equal digits keep their input order and the result is the permutation
+// of a sort on (code, index).
+#include <stdio.h>
+#include <string.h>
+
+#include "common.h"
+
+namespace cosmo {
+
+static thread_local hipError_t g_last_hip = hipSuccess;
+void set_hip_error(hipError_t e) { g_last_hip = e; }
+
+// ------------------------------------------------------------------ Morton
+
+// morton.py:33-36 as written: keeps the low 10 bits of v.
+__device__ __forceinline__ uint64_t spread_ref10(uint64_t v) {
+  v = (v | (v << 16)) & 0x030000FFull;
+  v = (v | (v << 8)) & 0x0300F00Full;
+  v = (v | (v << 4)) & 0x030C30C3ull;
+  v = (v | (v << 2)) & 0x09249249ull;
+  return v;
+}
+
+// 21 bits per axis: bit k of v moves to bit 3k.
+__device__ __forceinline__ uint64_t spread_full21(uint64_t v) {
+  v &= 0x1fffffull;
+  v = (v | (v << 32)) & 0x1f00000000ffffull;
+  v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+  v = (v | (v << 8)) & 0x100f00f00f00f00full;
+  v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
+  return v;
+}
+
+// morton.py:57-60 uint64(x * 2^21); x*2^21 is exact in FP32.  Not-positive
+// values (and NaN) give 0, values >= 2^40 saturate.
+__device__ __forceinline__ uint64_t fixed21(float x) {
+  const float s = x * 2097152.0f;
+  if (!(s > 0.0f)) return 0;
+  if (s >= 1099511627776.0f) return 1099511627776ull;
+  return static_cast<uint64_t>(s);
+}
+
+template <int MASKS>
+__global__ __launch_bounds__(256) void k_compute_codes(
+    const float *__restrict__ px, const float *__restrict__ py,
+    const float *__restrict__ pz, uint64_t *__restrict__ codes,
+    uint32_t *__restrict__ indices, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t ix = fixed21(px[i]), iy = fixed21(py[i]), iz = fixed21(pz[i]);
+  uint64_t c;
+  if (MASKS == COSMO_MASK_REF10)
+    c = spread_ref10(ix) | (spread_ref10(iy) << 1) | (spread_ref10(iz) << 2);
+  else
+    c = spread_full21(ix) | (spread_full21(iy) << 1) | (spread_full21(iz) << 2);
+  codes[i] = c;
+  if (indices) indices[i] = static_cast<uint32_t>(i);
+}
+
+// ------------------------------------------------------------- radix sort
+
+// Lanes of this wave holding the same 8-bit digit (among `valid` lanes).
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
+  uint64_t m = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < kRadixBits; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bb = __ballot(bit);
+    m &= bit ? bb : ~bb;
+  }
+  return m;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v,
+                                                             uint32_t *total) {
+  __shared__ uint32_t wave_sums[kSortThreads / kWave];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t t = __shfl_up(x, off, kWave);
+    if (lane >= off) x += t;
+  }
+  if (lane == kWave - 1) wave_sums[wave] = x;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < kSortThreads / kWave; ++w) {
+    const uint32_t s = wave_sums[w];
+    before += (w < wave) ? s : 0u;
+    all += s;
+  }
+  __syncthreads();
+  if (total) *total = all;
+  return before + x - v;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_count(
+    const uint64_t *__restrict__ keys, int64_t n, int shift,
+    uint32_t *__restrict__ hist, int ntiles) {
+  __shared__ uint32_t cnt[kRadix];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  cnt[tid] = 0;
+  __syncthreads();
+  const int64_t base = int64_t(blockIdx.x) * kSortTile + wave * (kSortItems * kWave);
+#pragma unroll 4
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t idx = base + r * kWave + lane;
+    const bool valid = idx < n;
+    const uint32_t d = valid ? uint32_t(keys[idx] >> shift) & (kRadix - 1) : 0u;
+    const uint64_t m = match_digit(d, valid);
+    if (valid && lane == __ffsll((unsigned long long)m) - 1)
+      atomicAdd(&cnt[d], (uint32_t)__popcll(m));
+  }
+  __syncthreads();
+  hist[int64_t(tid) * ntiles + blockIdx.x] = cnt[tid];
+}
+
+// One block per digit: exclusive scan of that digit's per-tile counts.
+__global__ __launch_bounds__(kSortThreads) void k_radix_rowscan(
+    uint32_t *__restrict__ hist, int ntiles, uint32_t *__restrict__ digit_tot) {
+  uint32_t *row = hist + int64_t(blockIdx.x) * ntiles;
+  uint32_t running = 0;
+  for (int start = 0; start < ntiles; start += kSortThreads) {
+    const int j = start + threadIdx.x;
+    const uint32_t v = (j < ntiles) ? row[j] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan_256(v, &total);
+    if (j < ntiles) row[j] = running + ex;
+    running += total;
+  }
+  if (threadIdx.x == 0) digit_tot[blockIdx.x] = running;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
+    const uint64_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
+    uint64_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out, int64_t n,
+    int shift, const uint32_t *__restrict__ hist,
+    const uint32_t *__restrict__ digit_tot, int ntiles) {
+  constexpr int kWaves = kSortThreads / kWave;
+  __shared__ uint32_t wcnt[kWaves][kRadix];
+  __shared__ uint32_t tile_base[kRadix];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
+  const uint32_t dstart = block_exclusive_scan_256(digit_tot[tid], nullptr);
+  tile_base[tid] = dstart + hist[int64_t(tid) * ntiles + blockIdx.x];
+  __syncthreads();
+
+  const int64_t base = int64_t(blockIdx.x) * kSortTile + wave * (kSortItems * kWave);
+  const uint64_t lt = lanemask_lt();
+  uint64_t k[kSortItems];
+  uint32_t v[kSortItems], rank[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t idx = base + r * kWave + lane;
+    const bool valid = idx < n;
+    k[r] = valid ? keys_in[idx] : ~0ull;
+    v[r] = valid ? (vals_in ? vals_in[idx] : static_cast<uint32_t>(idx)) : 0u;
+    const uint32_t d = uint32_t(k[r] >> shift) & (kRadix - 1);
+    const uint64_t m = match_digit(d, valid);
+    const uint32_t prior = wcnt[wave][d];
+    rank[r] = prior + (uint32_t)__popcll(m & lt);
+    if (valid && lane == __ffsll((unsigned long long)m) - 1)
+      wcnt[wave][d] = prior + (uint32_t)__popcll(m);
+  }
+  __syncthreads();
+  {
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      const uint32_t t = wcnt[w][tid];
+      wcnt[w][tid] = run;
+      run += t;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t idx = base + r * kWave + lane;
+    if (idx < n) {
+      const uint32_t d = uint32_t(k[r] >> shift) & (kRadix - 1);
+      const uint32_t pos = tile_base[d] + wcnt[wave][d] + rank[r];
+      keys_out[pos] = k[r];
+      vals_out[pos] = v[r];
+    }
+  }
+}
+
+struct PermuteArgs {
+  const float *src[8];
+  float *dst[8];
+};
+
+template <int NA>
+__global__ __launch_bounds__(256) void k_permute(const uint32_t *__restrict__ perm,
+                                                 PermuteArgs a, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t j = perm[i];
+#pragma unroll
+  for (int k = 0; k < NA; ++k) a.dst[k][i] = a.src[k][j];
+}
+
+}  // namespace cosmo
+
+using namespace cosmo;
+
+extern "C" {
+
+const char *cosmo_status_string(int status) {
+  switch (status) {
+    case COSMO_OK:
+      return "ok";
+    case COSMO_E_INVALID:
+      return "invalid argument";
+    case COSMO_E_HIP: {
+      static thread_local char buf[160];
+      snprintf(buf, sizeof buf, "HIP error: %s", hipGetErrorString(g_last_hip));
+      return buf;
+    }
+    case COSMO_E_TOO_LARGE:
+      return "size exceeds the context capacity";
+    case COSMO_E_MALFORMED_TREE:
+      return "malformed tree";
+    default:
+      return "unknown status";
+  }
+}
+
+int cosmo_ctx_create(int64_t n_max, cosmo_ctx **out) {
+  if (!out || n_max < 1 || n_max > kMaxBodies) return COSMO_E_INVALID;
+  *out = nullptr;
+  cosmo_ctx *c = new cosmo_ctx();
+  memset(c, 0, sizeof *c);
+  COSMO_CHECK_HIP(hipGetDevice(&c->device));
+  c->n_max = n_max;
+  c->max_tiles = (n_max + kSortTile - 1) / kSortTile;
+  const int64_t M = 2 * n_max - 1;
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = hipMalloc(&c->keys_alt, sizeof(uint64_t) * n_max);
+  if (e == hipSuccess) e = hipMalloc(&c->vals_a, sizeof(uint32_t) * n_max);
+  if (e == hipSuccess) e = hipMalloc(&c->vals_b, sizeof(uint32_t) * n_max);
+  if (e == hipSuccess) e = hipMalloc(&c->hist, sizeof(uint32_t) * kRadix * c->max_tiles);
+  if (e == hipSuccess) e = hipMalloc(&c->digit_tot, sizeof(uint32_t) * kRadix);
+  if (e == hipSuccess) e = hipMalloc(&c->nodes, sizeof(cosmo_node) * M);
+  if (e == hipSuccess) e = hipMalloc(&c->scratch_i32, sizeof(int32_t) * (4 * n_max + 4));
+  if (e != hipSuccess) {
+    set_hip_error(e);
+    cosmo_ctx_destroy(c);
+    return COSMO_E_HIP;
+  }
+  *out = c;
+  return COSMO_OK;
+}
+
+int cosmo_ctx_destroy(cosmo_ctx *c) {
+  if (!c) return COSMO_OK;
+  (void)hipFree(c->keys_alt);
+  (void)hipFree(c->vals_a);
+  (void)hipFree(c->vals_b);
+  (void)hipFree(c->hist);
+  (void)hipFree(c->digit_tot);
+  (void)hipFree(c->nodes);
+  (void)hipFree(c->scratch_i32);
+  delete c;
+  return COSMO_OK;
+}
+
+cosmo_node *cosmo_ctx_nodes(cosmo_ctx *c) { return c ? c->nodes : nullptr; }
+
+int cosmo_compute_codes(const float *px, const float *py, const float *pz,
+                        uint64_t *codes, uint32_t *indices, int64_t n,
+                        int mask_set, void *stream) {
+  if (n < 0 || n > kMaxBodies) return COSMO_E_INVALID;
+  if (n == 0) return COSMO_OK;
+  if (!px || !py || !pz || !codes) return COSMO_E_INVALID;
+  const unsigned g = blocks_for(n, 256);
+  if (mask_set == COSMO_MASK_REF10)
+    k_compute_codes<COSMO_MASK_REF10><<<g, 256, 0, as_stream(stream)>>>(px, py, pz, codes, indices, n);
+  else if (mask_set == COSMO_MASK_FULL21)
+    k_compute_codes<COSMO_MASK_FULL21><<<g, 256, 0, as_stream(stream)>>>(px, py, pz, codes, indices, n);
+  else
+    return COSMO_E_INVALID;
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+int cosmo_sort_by_code(cosmo_ctx *ctx, const uint64_t *codes_in, uint64_t *codes_out,
+                       uint32_t *perm, int64_t n, int key_bits, void *stream) {
+  if (!ctx || n < 0 || key_bits < 1 || key_bits > 64) return COSMO_E_INVALID;
+  if (n > ctx->n_max) return COSMO_E_TOO_LARGE;
+  if (n == 0) return COSMO_OK;
+  if (!codes_in || !codes_out || !perm) return COSMO_E_INVALID;
+  hipStream_t s = as_stream(stream);
+  const int ntiles = static_cast<int>((n + kSortTile - 1) / kSortTile);
+  int passes = (key_bits + kRadixBits - 1) / kRadixBits;
+  // Pass p writes to (codes_out, perm) when (passes-1-p) is even, so the
+  // last pass lands there.  With codes_in == codes_out the first pass must
+  // not write codes_out: keep the pass count even (an extra pass over a
+  // digit that is zero everywhere is a stable copy).
+  if (codes_in == codes_out && (passes & 1) && passes < 8) passes++;
+  const uint64_t *kin = codes_in;
+  const uint32_t *vin = nullptr;
+  for (int p = 0; p < passes; ++p) {
+    const bool to_out = ((passes - 1 - p) & 1) == 0;
+    uint64_t *kout = to_out ? codes_out : ctx->keys_alt;
+    uint32_t *vout = to_out ? perm : ctx->vals_a;
+    const int shift = p * kRadixBits;
+    k_radix_count<<<ntiles, kSortThreads, 0, s>>>(kin, n, shift, ctx->hist, ntiles);
+    k_radix_rowscan<<<kRadix, kSortThreads, 0, s>>>(ctx->hist, ntiles, ctx->digit_tot);
+    k_radix_scatter<<<ntiles, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, shift, ctx->hist,
+                                                    ctx->digit_tot, ntiles);
+    COSMO_CHECK_LAUNCH();
+    kin = kout;
+    vin = vout;
+  }
+  return COSMO_OK;
+}
+
+int cosmo_permute(const uint32_t *perm, int n_arrays, const float *const *src,
+                  float *const *dst, int64_t n, void *stream) {
+  if (n < 0 || n_arrays < 0 || n_arrays > 8 || (n_arrays && (!src || !dst)))
+    return COSMO_E_INVALID;
+  if (n == 0 || n_arrays == 0) return COSMO_OK;
+  if (!perm) return COSMO_E_INVALID;
+  PermuteArgs a;
+  memset(&a, 0, sizeof a);
+  for (int k = 0; k < n_arrays; ++k) {
+    if (!src[k] || !dst[k]) return COSMO_E_INVALID;
+    a.src[k] = src[k];
+    a.dst[k] = dst[k];
+  }
+  const unsigned g = blocks_for(n, 256);
+  hipStream_t s = as_stream(stream);
+  switch (n_arrays) {
+    case 1: k_permute<1><<<g, 256, 0, s>>>(perm, a, n); break;
+    case 2: k_permute<2><<<g, 256, 0, s>>>(perm, a, n); break;
+    case 3: k_permute<3><<<g, 256, 0, s>>>(perm, a, n); break;
+    case 4: k_permute<4><<<g, 256, 0, s>>>(perm, a, n); break;
+    case 5: k_permute<5><<<g, 256, 0, s>>>(perm, a, n); break;
+    case 6: k_permute<6><<<g, 256, 0, s>>>(perm, a, n); break;
+    case 7: k_permute<7><<<g, 256, 0, s>>>(perm, a, n); break;
+    default: k_permute<8><<<g, 256, 0, s>>>(perm, a, n); break;
+  }
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,323 @@
+// Barnes-Hut traversal and the FP64 direct sum.
+//
+// Reference: src/kernels/physics_kernels.py:14-153 (compute_forces), one
+// thread per particle with a private 64-entry stack.  Here one wavefront
+// walks the tree for 64 Morton-consecutive particles with ONE shared stack:
+// each entry carries the 64-bit mask of the lanes that must visit it.  On a
+// pop every masked lane runs its own acceptance test, accepted lanes
+// interact, and the lanes that open the node become the mask of the pushed
+// children (c2 then c1, so c1 is visited first, physics_kernels.py:144-149).
+// Each lane therefore visits exactly its own nodes in its own depth-first
+// order, so its interaction set and FP32 summation order are those of the
+// per-particle walk.  A per-lane depth counter equals the depth of the
+// reference's private stack and reproduces its overflow drop
+// (`stack_top + 1 < 64`, :143).  Masks of stacked entries are nested (an
+// entry's lanes opened every node below it), so the shared stack never
+// holds more than 64 entries and lives in three VGPRs indexed by lane
+// (v_readlane with a wave-uniform slot; a push selects one lane).  Node records are
+// wave-uniform loads (scalar cache).
+#include "common.h"
+
+namespace cosmo {
+
+struct TravParams {
+  float th2;        // REF: theta^2
+  float inv_theta;  // AABB: 1/theta
+  float G;
+  float eps2;
+  float half_box;
+};
+
+__device__ __forceinline__ void neumaier_add(float &s, float &c, float x) {
+  const float t = s + x;
+  if (fabsf(s) >= fabsf(x))
+    c += (s - t) + x;
+  else
+    c += (x - t) + s;
+  s = t;
+}
+
+template <int MAC, bool COMP, bool STATS, bool GROUP_THETA>
+__global__ __launch_bounds__(256) void k_traverse(
+    const float *__restrict__ px, const float *__restrict__ py,
+    const float *__restrict__ pz, const cosmo_node *__restrict__ nodes,
+    const int32_t *__restrict__ root_idx, float *__restrict__ force, TravParams p,
+    const float *__restrict__ theta_group, int64_t lo, int64_t hi,
+    unsigned long long *__restrict__ stats) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t group = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const int64_t base = lo + group * kWave;
+  if (base >= hi) return;  // wave-uniform
+  const int64_t i = base + lane;
+  const bool active = i < hi;
+  float x = 0.0f, y = 0.0f, z = 0.0f;
+  if (active) {
+    x = px[i];
+    y = py[i];
+    z = pz[i];
+  }
+  float th2 = p.th2, inv_theta = p.inv_theta;
+  if (GROUP_THETA) {
+    const float t = theta_group[group];
+    th2 = t * t;
+    inv_theta = 1.0f / t;
+  }
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+  uint32_t n_int = 0, n_vis = 0;
+  int depth = active ? 1 : 0;
+
+  const uint64_t amask = __ballot(active);
+  const int root = root_idx[0];
+  int stk_node = (lane == 0) ? root : 0;
+  uint32_t stk_lo = (lane == 0) ? static_cast<uint32_t>(amask) : 0u;
+  uint32_t stk_hi = (lane == 0) ? static_cast<uint32_t>(amask >> 32) : 0u;
+  int top = 1;
+
+  while (top > 0) {
+    --top;
+    const int node = __builtin_amdgcn_readlane(stk_node, top);
+    const uint32_t mlo = __builtin_amdgcn_readlane(stk_lo, top);
+    const uint32_t mhi = __builtin_amdgcn_readlane(stk_hi, top);
+    const bool in = ((lane < 32 ? mlo >> lane : mhi >> (lane - 32)) & 1u) != 0;
+    depth -= in ? 1 : 0;
+    const cosmo_node rec = nodes[node];
+    if (rec.m <= 0.0f) continue;
+    bool open = false;
+    if (in) {
+      float dx = rec.cx - x, dy = rec.cy - y, dz = rec.cz - z;
+      dx = dx - rintf(dx);
+      dy = dy - rintf(dy);
+      dz = dz - rintf(dz);
+      const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+      bool accept;
+      if (MAC == COSMO_MAC_REF) {
+        accept = rec.a < th2 * d2;
+      } else {
+        const float r = __builtin_fmaf(rec.a, inv_theta, rec.b);
+        float dm = fabsf(dx);
+        dm = fabsf(dy) > dm ? fabsf(dy) : dm;
+        dm = fabsf(dz) > dm ? fabsf(dz) : dm;
+        accept = (r * r < d2) && (dm + rec.a < p.half_box);
+      }
+      if (rec.c1 == -1 || accept) {
+        if (d2 > 1e-7f) {
+          const float inv = __builtin_amdgcn_rsqf(d2 + p.eps2);
+          const float f = (p.G * rec.m) * ((inv * inv) * inv);
+          if (COMP) {
+            neumaier_add(ax, cx, f * dx);
+            neumaier_add(ay, cy, f * dy);
+            neumaier_add(az, cz, f * dz);
+          } else {
+            ax = __builtin_fmaf(f, dx, ax);
+            ay = __builtin_fmaf(f, dy, ay);
+            az = __builtin_fmaf(f, dz, az);
+          }
+          if (STATS) n_int++;
+        }
+      } else {
+        open = depth + 1 < 64;
+      }
+      if (STATS) n_vis++;
+    }
+    const uint64_t om = __ballot(open);
+    if (om) {
+      const uint32_t olo = static_cast<uint32_t>(om), ohi = static_cast<uint32_t>(om >> 32);
+      if (open) depth += (rec.c1 != -1 ? 1 : 0) + (rec.c2 != -1 ? 1 : 0);
+      // push = write lane `top` of the three stack registers
+      if (rec.c1 != -1) {
+        const bool here = (lane == top);
+        stk_node = here ? rec.c2 : stk_node;
+        stk_lo = here ? olo : stk_lo;
+        stk_hi = here ? ohi : stk_hi;
+        ++top;
+      }
+      if (rec.c2 != -1) {
+        const bool here = (lane == top);
+        stk_node = here ? rec.c1 : stk_node;
+        stk_lo = here ? olo : stk_lo;
+        stk_hi = here ? ohi : stk_hi;
+        ++top;
+      }
+    }
+  }
+  if (active) {
+    force[3 * i + 0] = ax + cx;
+    force[3 * i + 1] = ay + cy;
+    force[3 * i + 2] = az + cz;
+  }
+  if (STATS) {
+    uint32_t a = n_int, b = n_vis;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      a += __shfl_xor(a, off, kWave);
+      b += __shfl_xor(b, off, kWave);
+    }
+    if (lane == 0) {
+      atomicAdd(&stats[0], static_cast<unsigned long long>(a));
+      atomicAdd(&stats[1], static_cast<unsigned long long>(b));
+    }
+  }
+}
+
+// FP64 direct sum: a block of 1024 threads takes 64 targets (one per lane);
+// its 16 waves split the sources and their partial sums are combined in a
+// fixed order, so results do not depend on scheduling.
+constexpr int kDirectWaves = 16;
+
+__global__ __launch_bounds__(1024) void k_direct_sum(
+    const float *__restrict__ px, const float *__restrict__ py,
+    const float *__restrict__ pz, const float *__restrict__ mass, int64_t n,
+    const int64_t *__restrict__ targets, int64_t nt, double G, double eps2,
+    double *__restrict__ acc) {
+  __shared__ double part[kDirectWaves][kWave][3];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int64_t t = int64_t(blockIdx.x) * kWave + lane;
+  const bool valid = t < nt;
+  const int64_t ti = valid ? targets[t] : 0;
+  const double x = px[ti], y = py[ti], z = pz[ti];
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  const int64_t chunk = (n + kDirectWaves - 1) / kDirectWaves;
+  const int64_t j0 = wave * chunk;
+  const int64_t j1 = (j0 + chunk < n) ? j0 + chunk : n;
+  for (int64_t j = j0; j < j1; ++j) {
+    double dx = static_cast<double>(px[j]) - x;
+    double dy = static_cast<double>(py[j]) - y;
+    double dz = static_cast<double>(pz[j]) - z;
+    dx -= rint(dx);
+    dy -= rint(dy);
+    dz -= rint(dz);
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 > 1e-7) {
+      const double d_inv = 1.0 / sqrt(d2 + eps2);
+      const double f = G * static_cast<double>(mass[j]) * (d_inv * d_inv * d_inv);
+      ax += f * dx;
+      ay += f * dy;
+      az += f * dz;
+    }
+  }
+  part[wave][lane][0] = ax;
+  part[wave][lane][1] = ay;
+  part[wave][lane][2] = az;
+  __syncthreads();
+  if (wave == 0 && valid) {
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int w = 0; w < kDirectWaves; ++w) {
+      sx += part[w][lane][0];
+      sy += part[w][lane][1];
+      sz += part[w][lane][2];
+    }
+    acc[3 * t + 0] = sx;
+    acc[3 * t + 1] = sy;
+    acc[3 * t + 2] = sz;
+  }
+}
+
+template <int MAC, bool COMP, bool STATS, bool GT>
+static void launch_traverse(unsigned grid, hipStream_t s, const float *px, const float *py,
+                            const float *pz, const cosmo_node *nodes, const int32_t *root_idx,
+                            float *force, TravParams p, const float *tg, int64_t lo, int64_t hi,
+                            uint64_t *stats) {
+  k_traverse<MAC, COMP, STATS, GT><<<grid, 256, 0, s>>>(
+      px, py, pz, nodes, root_idx, force, p, tg, lo, hi,
+      reinterpret_cast<unsigned long long *>(stats));
+}
+
+template <int MAC, bool COMP>
+static void dispatch_traverse2(bool stats_on, bool gt, unsigned grid, hipStream_t s,
+                               const float *px, const float *py, const float *pz,
+                               const cosmo_node *nodes, const int32_t *root_idx, float *force,
+                               TravParams p, const float *tg, int64_t lo, int64_t hi,
+                               uint64_t *stats) {
+  if (stats_on) {
+    if (gt)
+      launch_traverse<MAC, COMP, true, true>(grid, s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
+    else
+      launch_traverse<MAC, COMP, true, false>(grid, s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
+  } else {
+    if (gt)
+      launch_traverse<MAC, COMP, false, true>(grid, s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
+    else
+      launch_traverse<MAC, COMP, false, false>(grid, s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
+  }
+}
+
+}  // namespace cosmo
+
+using namespace cosmo;
+
+extern "C" {
+
+int cosmo_traverse(const float *px, const float *py, const float *pz, const cosmo_node *nodes,
+                   const int32_t *root_idx, float *force, double theta,
+                   const float *theta_group, double G, double soft, double box, int64_t off,
+                   int64_t count, int64_t n_bodies, int mac, int flags, uint64_t *stats,
+                   void *stream) {
+  if (n_bodies < 0 || n_bodies > kMaxBodies || off < 0 || count < 0) return COSMO_E_INVALID;
+  if (mac != COSMO_MAC_REF && mac != COSMO_MAC_AABB) return COSMO_E_INVALID;
+  const bool stats_on = (flags & COSMO_TRAV_STATS) != 0;
+  if (stats_on && !stats) return COSMO_E_INVALID;
+  const int64_t lo = off;
+  const int64_t hi = (off + count < n_bodies) ? off + count : n_bodies;
+  if (hi <= lo) return COSMO_OK;
+  if (!px || !py || !pz || !nodes || !root_idx || !force) return COSMO_E_INVALID;
+  TravParams p;
+  const float thf = static_cast<float>(theta);
+  p.th2 = thf * thf;
+  p.inv_theta = 1.0f / thf;
+  p.G = static_cast<float>(G);
+  const float sf = static_cast<float>(soft);
+  p.eps2 = sf * sf;
+  p.half_box = static_cast<float>(box) * 0.5f;
+  const int64_t groups = (hi - lo + kWave - 1) / kWave;
+  const unsigned grid = blocks_for(groups * kWave, 256);
+  hipStream_t s = as_stream(stream);
+  const bool comp = (flags & COSMO_TRAV_COMPENSATED) != 0;
+  const bool gt = theta_group != nullptr;
+  if (mac == COSMO_MAC_REF) {
+    if (comp)
+      dispatch_traverse2<COSMO_MAC_REF, true>(stats_on, gt, grid, s, px, py, pz, nodes, root_idx, force, p, theta_group, lo, hi, stats);
+    else
+      dispatch_traverse2<COSMO_MAC_REF, false>(stats_on, gt, grid, s, px, py, pz, nodes, root_idx, force, p, theta_group, lo, hi, stats);
+  } else {
+    if (comp)
+      dispatch_traverse2<COSMO_MAC_AABB, true>(stats_on, gt, grid, s, px, py, pz, nodes, root_idx, force, p, theta_group, lo, hi, stats);
+    else
+      dispatch_traverse2<COSMO_MAC_AABB, false>(stats_on, gt, grid, s, px, py, pz, nodes, root_idx, force, p, theta_group, lo, hi, stats);
+  }
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+int cosmo_compute_forces(cosmo_ctx *ctx, const float *px, const float *py, const float *pz,
+                         const int32_t *children, const double *n_mass, const double *n_com,
+                         const float *n_min, const float *n_max, float *force, double theta,
+                         double G, double soft, double box, const int32_t *root_idx,
+                         const int32_t *top_nodes, const int32_t *node_to_cache, int64_t off,
+                         int64_t count, int64_t n_bodies, void *stream) {
+  (void)top_nodes;
+  (void)node_to_cache;
+  if (!ctx) return COSMO_E_INVALID;
+  if (n_bodies > ctx->n_max) return COSMO_E_TOO_LARGE;
+  if (n_bodies < 1) return COSMO_OK;
+  const int rc = cosmo_pack_nodes(children, n_mass, n_com, n_min, n_max, 2 * n_bodies - 1, box,
+                                  COSMO_MAC_REF, ctx->nodes, stream);
+  if (rc != COSMO_OK) return rc;
+  return cosmo_traverse(px, py, pz, ctx->nodes, root_idx, force, theta, nullptr, G, soft, box,
+                        off, count, n_bodies, COSMO_MAC_REF, 0, nullptr, stream);
+}
+
+int cosmo_direct_sum(const float *px, const float *py, const float *pz, const float *mass,
+                     int64_t n, const int64_t *targets, int64_t n_targets, double G,
+                     double soft, double *acc, void *stream) {
+  if (n < 0 || n_targets < 0) return COSMO_E_INVALID;
+  if (n_targets == 0) return COSMO_OK;
+  if (!px || !py || !pz || !mass || !targets || !acc || n == 0) return COSMO_E_INVALID;
+  const unsigned grid = static_cast<unsigned>((n_targets + kWave - 1) / kWave);
+  k_direct_sum<<<grid, kDirectWaves * kWave, 0, as_stream(stream)>>>(
+      px, py, pz, mass, n, targets, n_targets, G, soft * soft, acc);
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,448 @@
+// Radix-tree build (Karras 2012), FP64 upward pass, root/top-cache helpers,
+// node packing and the tree validator.
+//
+// Reference: src/kernels/tree_kernels.py.  The build keeps the reference's
+// delta function including its equal-code tie-break (:55-56), so `children`
+// is bit-identical to the reference algorithm's for any input.  The upward
+// pass climbs from each leaf with an arrival counter per node (:164-191):
+// the second arrival forms the parent as left + right in FP64.  Arrival
+// counters are agent-scope acquire/release atomics, which make a child's
+// stores visible to the sibling's thread across the eight XCD L2 caches.
+#include "common.h"
+
+namespace cosmo {
+
+__device__ __forceinline__ int clz64(uint64_t v) {
+  return v ? __builtin_clzll(v) : 64;
+}
+
+// tree_kernels.py:36-58.
+__device__ __forceinline__ int delta_fn(const uint64_t *__restrict__ codes,
+                                        uint64_t ci, int i, int j, int n) {
+  if (j < 0 || j >= n) return -1;
+  const uint64_t cj = codes[j];
+  if (ci == cj) return 64 + (63 - clz64(static_cast<uint64_t>(i ^ j)));
+  return clz64(ci ^ cj);
+}
+
+__global__ __launch_bounds__(256) void k_reset_tree(int32_t *__restrict__ child,
+                                                    int32_t *__restrict__ parent,
+                                                    int32_t *__restrict__ counters,
+                                                    int32_t *__restrict__ root_idx,
+                                                    int64_t m) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i == 0) root_idx[0] = 0;
+  if (i < m) {
+    reinterpret_cast<int2 *>(child)[i] = make_int2(-1, -1);
+    parent[i] = -1;
+    counters[i] = 0;
+  }
+}
+
+// tree_kernels.py:60-129, one thread per internal node i in [0, n-2].
+__global__ __launch_bounds__(256) void k_build_tree(const uint64_t *__restrict__ codes,
+                                                    int32_t *__restrict__ children,
+                                                    int32_t *__restrict__ parents, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const uint64_t ci = codes[i];
+  const int idx = i + n;
+  const int d_prev = delta_fn(codes, ci, i, i - 1, n);
+  const int d_next = delta_fn(codes, ci, i, i + 1, n);
+  const int dir = (d_next > d_prev) ? 1 : -1;
+  const int min_delta = (d_next > d_prev) ? d_prev : d_next;
+  int l_max = 2;
+  while (delta_fn(codes, ci, i, i + l_max * dir, n) > min_delta) l_max *= 2;
+  int l = 0;
+  for (int t = l_max / 2; t >= 1; t /= 2)
+    if (delta_fn(codes, ci, i, i + (l + t) * dir, n) > min_delta) l += t;
+  const int j = i + l * dir;
+  const int delta_node = delta_fn(codes, ci, i, j, n);
+  int s = 0;
+  for (int t = l; t > 0;) {
+    const int step = (t + 1) / 2;
+    if (delta_fn(codes, ci, i, i + (s + step) * dir, n) > delta_node) s += step;
+    t -= step;
+  }
+  int gamma = i + s * dir;
+  if (dir == -1) gamma -= 1;
+  const int left = gamma, right = gamma + 1;
+  const int lo = min(i, j), hi = max(i, j);
+  const int c0 = (lo == left) ? left : left + n;
+  const int c1 = (hi == right) ? right : right + n;
+  reinterpret_cast<int2 *>(children)[idx] = make_int2(c0, c1);
+  parents[c0] = idx;
+  parents[c1] = idx;
+}
+
+__device__ __forceinline__ float min_ref(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float max_ref(float a, float b) { return (b > a) ? b : a; }
+
+// tree_kernels.py:152-191 with FP64 mass and mass-weighted position sums.
+// The climbing thread keeps its own node's values in registers and loads
+// only the sibling's.  IEEE addition is commutative, so "mine + sibling"
+// equals the reference's "left + right" bit for bit; min/max keep the
+// left/right order explicitly.
+__global__ __launch_bounds__(256) void k_multipoles(
+    const float *__restrict__ px, const float *__restrict__ py,
+    const float *__restrict__ pz, const float *__restrict__ mass,
+    const int32_t *__restrict__ children, const int32_t *__restrict__ parents,
+    double *n_mass, double *n_com, float *n_min, float *n_max, int32_t *counts,
+    int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float x = px[i], y = py[i], z = pz[i];
+  double m = static_cast<double>(mass[i]);
+  double c0 = static_cast<double>(x) * m, c1 = static_cast<double>(y) * m,
+         c2 = static_cast<double>(z) * m;
+  float lo0 = x, lo1 = y, lo2 = z, hi0 = x, hi1 = y, hi2 = z;
+  n_mass[i] = m;
+  n_com[3 * int64_t(i) + 0] = c0;
+  n_com[3 * int64_t(i) + 1] = c1;
+  n_com[3 * int64_t(i) + 2] = c2;
+  n_min[3 * int64_t(i) + 0] = lo0;
+  n_min[3 * int64_t(i) + 1] = lo1;
+  n_min[3 * int64_t(i) + 2] = lo2;
+  n_max[3 * int64_t(i) + 0] = hi0;
+  n_max[3 * int64_t(i) + 1] = hi1;
+  n_max[3 * int64_t(i) + 2] = hi2;
+  int curr = i;
+  for (;;) {
+    const int p = parents[curr];
+    if (p == -1) break;
+    const int old = __hip_atomic_fetch_add(&counts[p], 1, __ATOMIC_ACQ_REL,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) break;
+    const int2 ch = reinterpret_cast<const int2 *>(children)[p];
+    const bool mine_left = (ch.x == curr);
+    const int64_t sib = mine_left ? ch.y : ch.x;
+    const double sm = n_mass[sib];
+    const double s0 = n_com[3 * sib + 0], s1 = n_com[3 * sib + 1], s2 = n_com[3 * sib + 2];
+    const float sl0 = n_min[3 * sib + 0], sl1 = n_min[3 * sib + 1], sl2 = n_min[3 * sib + 2];
+    const float sh0 = n_max[3 * sib + 0], sh1 = n_max[3 * sib + 1], sh2 = n_max[3 * sib + 2];
+    m = m + sm;
+    c0 = c0 + s0;
+    c1 = c1 + s1;
+    c2 = c2 + s2;
+    if (mine_left) {
+      lo0 = min_ref(lo0, sl0); lo1 = min_ref(lo1, sl1); lo2 = min_ref(lo2, sl2);
+      hi0 = max_ref(hi0, sh0); hi1 = max_ref(hi1, sh1); hi2 = max_ref(hi2, sh2);
+    } else {
+      lo0 = min_ref(sl0, lo0); lo1 = min_ref(sl1, lo1); lo2 = min_ref(sl2, lo2);
+      hi0 = max_ref(sh0, hi0); hi1 = max_ref(sh1, hi1); hi2 = max_ref(sh2, hi2);
+    }
+    const int64_t q = p;
+    n_mass[q] = m;
+    n_com[3 * q + 0] = c0;
+    n_com[3 * q + 1] = c1;
+    n_com[3 * q + 2] = c2;
+    n_min[3 * q + 0] = lo0;
+    n_min[3 * q + 1] = lo1;
+    n_min[3 * q + 2] = lo2;
+    n_max[3 * q + 0] = hi0;
+    n_max[3 * q + 1] = hi1;
+    n_max[3 * q + 2] = hi2;
+    curr = p;
+  }
+}
+
+// tree_kernels.py:193-214.
+__global__ __launch_bounds__(256) void k_normalize_com(const double *__restrict__ n_mass,
+                                                       double *__restrict__ n_com,
+                                                       int64_t m) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const double nm = n_mass[i];
+  if (nm > 0) {
+    const double inv = 1.0 / nm;
+    n_com[3 * i + 0] *= inv;
+    n_com[3 * i + 1] *= inv;
+    n_com[3 * i + 2] *= inv;
+  }
+}
+
+// tree_kernels.py:216-234: the largest parentless index wins (deterministic).
+__global__ __launch_bounds__(256) void k_find_root(const int32_t *__restrict__ parents,
+                                                   int32_t *__restrict__ root_idx, int64_t m) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < m && parents[i] == -1) atomicMax(root_idx, static_cast<int32_t>(i));
+}
+
+// tree_kernels.py:264-314, restated serially on one lane (256 slots).
+__global__ void k_top_tree_cache(const int32_t *__restrict__ child,
+                                 const int32_t *__restrict__ root_idx,
+                                 int32_t *__restrict__ top_nodes,
+                                 int32_t *__restrict__ node_to_cache, int cache_size) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int root = root_idx[0];
+  if (root == -1) return;
+  top_nodes[0] = root;
+  node_to_cache[root] = 0;
+  int read_idx = 0, write_idx = 1;
+  while (read_idx < write_idx && write_idx < cache_size) {
+    const int curr = top_nodes[read_idx];
+    const int c1 = child[2 * curr], c2 = child[2 * curr + 1];
+    if (c1 != -1 && write_idx < cache_size) {
+      top_nodes[write_idx] = c1;
+      node_to_cache[c1] = write_idx;
+      write_idx++;
+    }
+    if (c2 != -1 && write_idx < cache_size) {
+      top_nodes[write_idx] = c2;
+      node_to_cache[c2] = write_idx;
+      write_idx++;
+    }
+    read_idx++;
+  }
+  while (write_idx < cache_size) top_nodes[write_idx++] = -1;
+}
+
+// Packs node k into a 32-byte FP32 record; NORMALIZE also divides the FP64
+// CoM sums by the mass first (normalize_com fused) and writes them back.
+template <int MAC, bool NORMALIZE>
+__global__ __launch_bounds__(256) void k_pack_nodes(
+    const int32_t *__restrict__ children, const double *__restrict__ n_mass,
+    double *__restrict__ n_com, const float *__restrict__ n_min,
+    const float *__restrict__ n_max, int64_t m, float boxf, cosmo_node *__restrict__ out) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const double nm = n_mass[k];
+  double c0 = n_com[3 * k + 0], c1 = n_com[3 * k + 1], c2 = n_com[3 * k + 2];
+  if (NORMALIZE && nm > 0) {
+    const double inv = 1.0 / nm;
+    c0 *= inv;
+    c1 *= inv;
+    c2 *= inv;
+    n_com[3 * k + 0] = c0;
+    n_com[3 * k + 1] = c1;
+    n_com[3 * k + 2] = c2;
+  }
+  const int2 ch = reinterpret_cast<const int2 *>(children)[k];
+  const float lo0 = n_min[3 * k + 0], lo1 = n_min[3 * k + 1], lo2 = n_min[3 * k + 2];
+  const float hi0 = n_max[3 * k + 0], hi1 = n_max[3 * k + 1], hi2 = n_max[3 * k + 2];
+  const float fx = static_cast<float>(c0), fy = static_cast<float>(c1), fz = static_cast<float>(c2);
+  float a, b;
+  if (MAC == COSMO_MAC_REF) {
+    // physics_kernels.py:129-130
+    float sx = hi0 - lo0;
+    sx = (sx > boxf * 0.5f) ? boxf : sx;
+    a = sx * sx;
+    b = 0.0f;
+  } else {
+    const float e0 = hi0 - lo0, e1 = hi1 - lo1, e2 = hi2 - lo2;
+    float s = 0.0f;
+    s = e0 > s ? e0 : s;
+    s = e1 > s ? e1 : s;
+    s = e2 > s ? e2 : s;
+    const float o0 = fx - 0.5f * (hi0 + lo0);
+    const float o1 = fy - 0.5f * (hi1 + lo1);
+    const float o2 = fz - 0.5f * (hi2 + lo2);
+    a = s;
+    b = sqrtf(__builtin_fmaf(o2, o2, __builtin_fmaf(o1, o1, o0 * o0)));
+  }
+  float4 *dst = reinterpret_cast<float4 *>(out + k);
+  dst[0] = make_float4(fx, fy, fz, static_cast<float>(nm));
+  dst[1] = make_float4(a, b, __int_as_float(ch.x), __int_as_float(ch.y));
+}
+
+// ------------------------------------------------------------ validator
+
+// scratch layout: [0] flag bits, [1] parentless count, [2] largest parentless
+// id, [3..3+M) arrival counters, [3+M..3+2M) leaves below each node.
+__global__ __launch_bounds__(256) void k_validate_clear(int32_t *scratch, int64_t len) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < len) scratch[i] = (i == 2) ? -1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_validate_links(const int32_t *__restrict__ children,
+                                                        const int32_t *__restrict__ parents,
+                                                        int n, int32_t *scratch) {
+  const int64_t M = 2 * int64_t(n) - 1;
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= M) return;
+  int bad = 0;
+  const int2 ch = reinterpret_cast<const int2 *>(children)[k];
+  if (k < n) {
+    if (ch.x != -1 || ch.y != -1) bad = 4;
+  } else {
+    if (ch.x < 0 || ch.x >= M || ch.y < 0 || ch.y >= M || ch.x == ch.y) bad = 4;
+    else if (parents[ch.x] != k || parents[ch.y] != k) bad = 4;
+  }
+  if (bad) atomicOr(&scratch[0], bad);
+  if (parents[k] == -1) {
+    atomicAdd(&scratch[1], 1);
+    atomicMax(&scratch[2], static_cast<int32_t>(k));
+  }
+}
+
+// Counts leaves below every node with the same arrival-counter climb as the
+// upward pass; each node completes at most once, so the climb terminates
+// on any parents array.
+__global__ __launch_bounds__(256) void k_validate_climb(const int32_t *__restrict__ children,
+                                                        const int32_t *__restrict__ parents,
+                                                        int n, int32_t *scratch) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t M = 2 * int64_t(n) - 1;
+  int32_t *arrive = scratch + 3;
+  int32_t *leaves = scratch + 3 + M;
+  int cnt = 1;
+  leaves[i] = 1;
+  int curr = i;
+  for (;;) {
+    const int p = parents[curr];
+    if (p < 0 || p >= M) break;
+    const int old = __hip_atomic_fetch_add(&arrive[p], 1, __ATOMIC_ACQ_REL,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) break;
+    const int2 ch = reinterpret_cast<const int2 *>(children)[p];
+    const int sib = (ch.x == curr) ? ch.y : ch.x;
+    if (sib < 0 || sib >= M) break;
+    cnt += leaves[sib];
+    leaves[p] = cnt;
+    curr = p;
+  }
+}
+
+__global__ void k_validate_finish(int n, const int32_t *scratch, int32_t *flags_out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int64_t M = 2 * int64_t(n) - 1;
+  int f = scratch[0];
+  const int n_root = scratch[1], root = scratch[2];
+  if (n_root != 1) f |= 1;
+  if (n >= 2 && !(n_root == 1 && root == n)) f |= 2;
+  if (n_root < 1 || root < 0 || root >= M || scratch[3 + M + root] != n) f |= 8;
+  flags_out[0] = f;
+}
+
+}  // namespace cosmo
+
+using namespace cosmo;
+
+extern "C" {
+
+int cosmo_reset_tree_buffers(int32_t *child, int32_t *parent, int32_t *counters,
+                             int32_t *root_idx, int64_t n_nodes, void *stream) {
+  if (n_nodes < 0 || n_nodes > 2 * kMaxBodies) return COSMO_E_INVALID;
+  if (!child || !parent || !counters || !root_idx) return COSMO_E_INVALID;
+  const unsigned g = blocks_for(n_nodes > 0 ? n_nodes : 1, 256);
+  k_reset_tree<<<g, 256, 0, as_stream(stream)>>>(child, parent, counters, root_idx, n_nodes);
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+int cosmo_build_tree(const uint64_t *codes, int32_t *children, int32_t *parents, int64_t n,
+                     void *stream) {
+  if (n < 0 || n > kMaxBodies) return COSMO_E_INVALID;
+  if (n < 2) return COSMO_OK;
+  if (!codes || !children || !parents) return COSMO_E_INVALID;
+  k_build_tree<<<blocks_for(n - 1, 256), 256, 0, as_stream(stream)>>>(codes, children, parents,
+                                                                       static_cast<int>(n));
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+int cosmo_compute_multipoles(const float *px, const float *py, const float *pz,
+                             const float *mass, const int32_t *children,
+                             const int32_t *parents, double *n_mass, double *n_com,
+                             float *n_min, float *n_max, int32_t *counts, int64_t n,
+                             void *stream) {
+  if (n < 0 || n > kMaxBodies) return COSMO_E_INVALID;
+  if (n == 0) return COSMO_OK;
+  if (!px || !py || !pz || !mass || !children || !parents || !n_mass || !n_com || !n_min ||
+      !n_max || !counts)
+    return COSMO_E_INVALID;
+  k_multipoles<<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(
+      px, py, pz, mass, children, parents, n_mass, n_com, n_min, n_max, counts,
+      static_cast<int>(n));
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+int cosmo_normalize_com(const double *n_mass, double *n_com, int64_t n_nodes, void *stream) {
+  if (n_nodes < 0 || n_nodes > 2 * kMaxBodies) return COSMO_E_INVALID;
+  if (n_nodes == 0) return COSMO_OK;
+  if (!n_mass || !n_com) return COSMO_E_INVALID;
+  k_normalize_com<<<blocks_for(n_nodes, 256), 256, 0, as_stream(stream)>>>(n_mass, n_com,
+                                                                           n_nodes);
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+int cosmo_find_root(const int32_t *parents, int32_t *root_idx, int64_t n_nodes, void *stream) {
+  if (n_nodes < 0 || n_nodes > 2 * kMaxBodies) return COSMO_E_INVALID;
+  if (n_nodes == 0) return COSMO_OK;
+  if (!parents || !root_idx) return COSMO_E_INVALID;
+  k_find_root<<<blocks_for(n_nodes, 256), 256, 0, as_stream(stream)>>>(parents, root_idx,
+                                                                       n_nodes);
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+int cosmo_build_top_tree_cache(const int32_t *child, const int32_t *root_idx,
+                               int32_t *top_nodes, int32_t *node_to_cache,
+                               int64_t cache_size, void *stream) {
+  if (cache_size < 0 || cache_size > (1 << 20)) return COSMO_E_INVALID;
+  if (cache_size == 0) return COSMO_OK;
+  if (!child || !root_idx || !top_nodes || !node_to_cache) return COSMO_E_INVALID;
+  k_top_tree_cache<<<1, 64, 0, as_stream(stream)>>>(child, root_idx, top_nodes, node_to_cache,
+                                                    static_cast<int>(cache_size));
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+static int pack_impl(const int32_t *children, const double *n_mass, double *n_com,
+                     const float *n_min, const float *n_max, int64_t m, double box, int mac,
+                     bool normalize, cosmo_node *nodes, void *stream) {
+  if (m < 0 || m > 2 * kMaxBodies) return COSMO_E_INVALID;
+  if (m == 0) return COSMO_OK;
+  if (!children || !n_mass || !n_com || !n_min || !n_max || !nodes) return COSMO_E_INVALID;
+  const unsigned g = blocks_for(m, 256);
+  hipStream_t s = as_stream(stream);
+  const float boxf = static_cast<float>(box);
+  if (mac == COSMO_MAC_REF && normalize)
+    k_pack_nodes<COSMO_MAC_REF, true><<<g, 256, 0, s>>>(children, n_mass, n_com, n_min, n_max, m, boxf, nodes);
+  else if (mac == COSMO_MAC_REF)
+    k_pack_nodes<COSMO_MAC_REF, false><<<g, 256, 0, s>>>(children, n_mass, n_com, n_min, n_max, m, boxf, nodes);
+  else if (mac == COSMO_MAC_AABB && normalize)
+    k_pack_nodes<COSMO_MAC_AABB, true><<<g, 256, 0, s>>>(children, n_mass, n_com, n_min, n_max, m, boxf, nodes);
+  else if (mac == COSMO_MAC_AABB)
+    k_pack_nodes<COSMO_MAC_AABB, false><<<g, 256, 0, s>>>(children, n_mass, n_com, n_min, n_max, m, boxf, nodes);
+  else
+    return COSMO_E_INVALID;
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+int cosmo_pack_nodes(const int32_t *children, const double *n_mass, const double *n_com,
+                     const float *n_min, const float *n_max, int64_t n_nodes, double box,
+                     int mac, cosmo_node *nodes, void *stream) {
+  return pack_impl(children, n_mass, const_cast<double *>(n_com), n_min, n_max, n_nodes, box,
+                   mac, false, nodes, stream);
+}
+
+int cosmo_normalize_pack(const int32_t *children, const double *n_mass, double *n_com,
+                         const float *n_min, const float *n_max, int64_t n_nodes, double box,
+                         int mac, cosmo_node *nodes, void *stream) {
+  return pack_impl(children, n_mass, n_com, n_min, n_max, n_nodes, box, mac, true, nodes,
+                   stream);
+}
+
+int cosmo_validate_tree(cosmo_ctx *ctx, const int32_t *children, const int32_t *parents,
+                        int64_t n, int32_t *flags_out, void *stream) {
+  if (!ctx || n < 1 || !children || !parents || !flags_out) return COSMO_E_INVALID;
+  if (n > ctx->n_max) return COSMO_E_TOO_LARGE;
+  const int64_t M = 2 * n - 1;
+  const int64_t len = 3 + 2 * M;
+  hipStream_t s = as_stream(stream);
+  k_validate_clear<<<blocks_for(len, 256), 256, 0, s>>>(ctx->scratch_i32, len);
+  k_validate_links<<<blocks_for(M, 256), 256, 0, s>>>(children, parents, static_cast<int>(n),
+                                                      ctx->scratch_i32);
+  k_validate_climb<<<blocks_for(n, 256), 256, 0, s>>>(children, parents, static_cast<int>(n),
+                                                      ctx->scratch_i32);
+  k_validate_finish<<<1, 64, 0, s>>>(static_cast<int>(n), ctx->scratch_i32, flags_out);
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+"""Force and update kernels (src/kernels/physics_kernels.py).
+
+``compute_forces[blocks, threads](pos_x, pos_y, pos_z, children, n_mass,
+n_com, n_min, n_max, force, theta, G, soft, box, root, top_nodes,
+node_to_cache, off)`` computes Barnes-Hut accelerations for particles
+[off, off + blocks*threads) with the reference's acceptance test and walk
+(physics_kernels.py:14-153) on the gfx950 wavefront-shared-stack traversal.
+``integrate[blocks, threads](pos_x, pos_y, pos_z, vel_x, vel_y, vel_z, force,
+a, H, dt)`` is the kick-drift update (:155-203) in FP32.
+
+``traverse`` (packed records, either acceptance test, per-group theta,
+compensated sums, interaction counts) and ``direct_sum`` (FP64 yardstick)
+have no reference counterpart.
+"""
+from __future__ import annotations
+
+import torch
+
+from ._hip import (MAC_AABB, MAC_REF, TRAV_COMPENSATED, TRAV_STATS, check, kernel, load, ptr,
+                   shared_context, stream_handle)
+
+__all__ = ["compute_forces", "integrate", "kick_drift", "traverse", "direct_sum", "MAC_REF",
+           "MAC_AABB"]
+
+_F32 = torch.float32
+
+
+def _xyz(pos_x, pos_y, pos_z, n):
+    return [ptr(t, _F32, nm, numel=n) for t, nm in ((pos_x, "pos_x"), (pos_y, "pos_y"),
+                                                     (pos_z, "pos_z"))]
+
+
+@kernel
+def compute_forces(pos_x, pos_y, pos_z, children, n_mass, n_com, n_min, n_max, force, theta, G,
+                   soft, box, root, top_nodes, node_to_cache, off, _launch=None):
+    n = pos_x.shape[0]
+    off = int(off)
+    count = (n - off) if _launch is None else _launch[0] * _launch[1]
+    m = 2 * n - 1
+    if n_mass.dtype != torch.float64:  # reference-layout FP32 node arrays
+        n_mass = n_mass.to(torch.float64)
+        n_com = n_com.to(torch.float64)
+    ctx = shared_context(n)
+    check(load().cosmo_compute_forces(
+        ctx.handle, *_xyz(pos_x, pos_y, pos_z, n),
+        ptr(children, torch.int32, "children", numel=2 * m),
+        ptr(n_mass, torch.float64, "n_mass", numel=m),
+        ptr(n_com, torch.float64, "n_com", numel=3 * m),
+        ptr(n_min, _F32, "n_min", numel=3 * m), ptr(n_max, _F32, "n_max", numel=3 * m),
+        ptr(force, _F32, "force", numel=3 * n), float(theta), float(G), float(soft),
+        float(box), ptr(root, torch.int32, "root"), None, None, off, count, n,
+        stream_handle()), "compute_forces")
+
+
+def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0, count=None,
+             mac=MAC_REF, theta_group=None, compensated=False, stats=None):
+    """Traversal over packed node records; stats: int64[2] tensor accumulating
+    (interactions, node visits) when given."""
+    n = pos_x.shape[0]
+    count = n - off if count is None else count
+    flags = (TRAV_COMPENSATED if compensated else 0) | (TRAV_STATS if stats is not None else 0)
+    groups = (min(n, off + count) - off + 63) // 64
+    tg = None if theta_group is None else ptr(theta_group, _F32, "theta_group",
+                                              numel=max(groups, 0))
+    st = None if stats is None else ptr(stats, (torch.int64, torch.uint64), "stats", numel=2)
+    check(load().cosmo_traverse(
+        *_xyz(pos_x, pos_y, pos_z, n), ptr(nodes, _F32, "nodes", numel=8 * (2 * n - 1)),
+        ptr(root, torch.int32, "root"), ptr(force, _F32, "force", numel=3 * n), float(theta), tg,
+        float(G), float(soft), float(box), int(off), int(count), n, int(mac), flags, st,
+        stream_handle()), "traverse")
+
+
+@kernel
+def integrate(pos_x, pos_y, pos_z, vel_x, vel_y, vel_z, force, a, H, dt, _launch=None):
+    n = pos_x.shape[0] if _launch is None else min(pos_x.shape[0], _launch[0] * _launch[1])
+    v = [ptr(t, _F32, nm, numel=n) for t, nm in ((vel_x, "vel_x"), (vel_y, "vel_y"),
+                                                  (vel_z, "vel_z"))]
+    check(load().cosmo_integrate(*_xyz(pos_x, pos_y, pos_z, n), *v,
+                                 ptr(force, _F32, "force", numel=3 * n), float(a), float(H),
+                                 float(dt), n, stream_handle()), "integrate")
+
+
+def kick_drift(pos, vel, force, c_drag, c_kick, c_drift):
+    n = pos[0].shape[0]
+    v = [ptr(t, _F32, "vel", numel=n) for t in vel]
+    check(load().cosmo_kick_drift(*_xyz(*pos, n), *v, ptr(force, _F32, "force", numel=3 * n),
+                                  float(c_drag), float(c_kick), float(c_drift), n,
+                                  stream_handle()), "kick_drift")
+
+
+def direct_sum(pos_x, pos_y, pos_z, mass, targets, G, soft, out=None):
+    """FP64 direct-sum accelerations float64[t, 3] for the target indices."""
+    n = pos_x.shape[0]
+    targets = targets.to(device=pos_x.device, dtype=torch.int64).contiguous()
+    nt = targets.shape[0]
+    out = torch.empty((nt, 3), dtype=torch.float64, device=pos_x.device) if out is None else out
+    check(load().cosmo_direct_sum(*_xyz(pos_x, pos_y, pos_z, n), ptr(mass, _F32, "mass", numel=n),
+                                  n, ptr(targets, torch.int64, "targets"), nt, float(G),
+                                  float(soft), ptr(out, torch.float64, "out", numel=3 * nt),
+                                  stream_handle()), "direct_sum")
+    return out

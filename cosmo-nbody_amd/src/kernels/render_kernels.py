@@ -1,0 +1,42 @@
+"""Density splat (src/kernels/render_kernels.py).
+
+``render_density[blocks, threads](pos_x, pos_y, pos_z, grid, mvp, res)``
+projects every particle with the MVP matrix and adds 2.5/w to its pixel
+(render_kernels.py:35-47).  The grid holds fixed-point uint64 bins (int64
+tensor of shape (res, res)): each visible particle adds
+rint(2.5/w * 2^frac_bits), so the grid is independent of arrival order.
+``to_float`` converts it back to the reference's float density.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from ._hip import RENDER_FRAC_BITS, U64, check, kernel, load, ptr, stream_handle
+
+__all__ = ["render_density", "to_float", "RENDER_FRAC_BITS"]
+
+
+def _mvp_host(mvp):
+    if isinstance(mvp, torch.Tensor):
+        mvp = mvp.detach().to("cpu")
+        mvp = mvp.numpy()
+    m = np.ascontiguousarray(np.asarray(mvp, dtype=np.float32).reshape(16))
+    return m
+
+
+@kernel
+def render_density(pos_x, pos_y, pos_z, grid, mvp, res, frac_bits=RENDER_FRAC_BITS,
+                   _launch=None):
+    n = pos_x.shape[0] if _launch is None else min(pos_x.shape[0], _launch[0] * _launch[1])
+    res = int(res)
+    m = _mvp_host(mvp)
+    check(load().cosmo_render_density(
+        ptr(pos_x, torch.float32, "pos_x", numel=n), ptr(pos_y, torch.float32, "pos_y", numel=n),
+        ptr(pos_z, torch.float32, "pos_z", numel=n), ptr(grid, U64, "grid", numel=res * res),
+        m.ctypes.data, res, n, int(frac_bits), stream_handle()), "render_density")
+
+
+def to_float(grid, frac_bits=RENDER_FRAC_BITS):
+    """Fixed-point bins -> float64 density."""
+    return grid.to(torch.float64) * (2.0 ** -frac_bits)

@@ -1,0 +1,141 @@
+"""Tree construction kernels (src/kernels/tree_kernels.py).
+
+Drop-ins with the reference's names and arguments, each running one gfx950
+kernel through the C ABI:
+
+  reset_tree_buffers(child, parent, counters, root_idx)            :236-262
+  build_tree(codes, children, parents)                               :60-129
+  compute_multipoles(pos_x, pos_y, pos_z, mass, children, parents,
+                     n_mass, n_com, n_min, n_max, counts)           :131-191
+  normalize_com(n_mass, n_com, n_nodes)                             :193-214
+  find_root(parents, root_idx)                                      :216-234
+  build_top_tree_cache(child, root_idx, top_nodes, node_to_cache,
+                       cache_size)                                  :264-314
+
+Node mass and CoM tensors are float64 (the upward pass accumulates in FP64;
+the reference stores float32).  build_tree and compute_multipoles need a
+launch that covers every node, as all of the reference's callers use.
+
+Additions without a reference counterpart: pack_nodes / normalize_pack (the
+32-byte FP32 records the traversal reads) and validate_tree.
+"""
+from __future__ import annotations
+
+import torch
+
+from ._hip import (MAC_AABB, MAC_REF, NODE_BYTES, U32, U64, check, kernel, launch_extent, load,
+                   ptr, shared_context, stream_handle)
+
+__all__ = ["reset_tree_buffers", "build_tree", "compute_multipoles", "normalize_com",
+           "find_root", "build_top_tree_cache", "pack_nodes", "normalize_pack",
+           "validate_tree", "node_buffer", "MAC_REF", "MAC_AABB"]
+
+
+def _full(launch, need, name):
+    if launch is not None and launch[0] * launch[1] < need:
+        raise ValueError(f"{name}: the launch must cover all {need} work items "
+                         f"(got {launch[0]}x{launch[1]})")
+
+
+@kernel
+def reset_tree_buffers(child, parent, counters, root_idx, _launch=None):
+    m = launch_extent(_launch, child.shape[0])
+    check(load().cosmo_reset_tree_buffers(
+        ptr(child, torch.int32, "child"), ptr(parent, torch.int32, "parent", numel=m),
+        ptr(counters, torch.int32, "counters", numel=m), ptr(root_idx, torch.int32, "root_idx"),
+        m, stream_handle()), "reset_tree_buffers")
+
+
+@kernel
+def build_tree(codes, children, parents, _launch=None):
+    n = codes.shape[0]
+    _full(_launch, n - 1, "build_tree")
+    m = 2 * n - 1
+    check(load().cosmo_build_tree(
+        ptr(codes, U64, "codes"), ptr(children, torch.int32, "children", numel=2 * m),
+        ptr(parents, torch.int32, "parents", numel=m), n, stream_handle()), "build_tree")
+
+
+@kernel
+def compute_multipoles(pos_x, pos_y, pos_z, mass, children, parents, n_mass, n_com, n_min,
+                       n_max, counts, _launch=None):
+    n = pos_x.shape[0]
+    _full(_launch, n, "compute_multipoles")
+    m = 2 * n - 1
+    f32 = torch.float32
+    check(load().cosmo_compute_multipoles(
+        ptr(pos_x, f32, "pos_x", numel=n), ptr(pos_y, f32, "pos_y", numel=n),
+        ptr(pos_z, f32, "pos_z", numel=n), ptr(mass, f32, "mass", numel=n),
+        ptr(children, torch.int32, "children", numel=2 * m),
+        ptr(parents, torch.int32, "parents", numel=m),
+        ptr(n_mass, torch.float64, "n_mass", numel=m),
+        ptr(n_com, torch.float64, "n_com", numel=3 * m),
+        ptr(n_min, f32, "n_min", numel=3 * m), ptr(n_max, f32, "n_max", numel=3 * m),
+        ptr(counts, torch.int32, "counts", numel=m), n, stream_handle()), "compute_multipoles")
+
+
+@kernel
+def normalize_com(n_mass, n_com, n_nodes, _launch=None):
+    m = launch_extent(_launch, int(n_nodes))
+    check(load().cosmo_normalize_com(
+        ptr(n_mass, torch.float64, "n_mass", numel=m),
+        ptr(n_com, torch.float64, "n_com", numel=3 * m), m, stream_handle()), "normalize_com")
+
+
+@kernel
+def find_root(parents, root_idx, _launch=None):
+    m = launch_extent(_launch, parents.shape[0])
+    check(load().cosmo_find_root(ptr(parents, torch.int32, "parents", numel=m),
+                                 ptr(root_idx, torch.int32, "root_idx"), m, stream_handle()),
+          "find_root")
+
+
+@kernel
+def build_top_tree_cache(child, root_idx, top_nodes, node_to_cache, cache_size, _launch=None):
+    cs = int(cache_size)
+    check(load().cosmo_build_top_tree_cache(
+        ptr(child, torch.int32, "child"), ptr(root_idx, torch.int32, "root_idx"),
+        ptr(top_nodes, torch.int32, "top_nodes", numel=cs),
+        ptr(node_to_cache, torch.int32, "node_to_cache"), cs, stream_handle()),
+        "build_top_tree_cache")
+
+
+def node_buffer(n_nodes: int, device=None) -> torch.Tensor:
+    """Uninitialised storage for n_nodes packed 32-byte node records."""
+    return torch.empty((n_nodes, NODE_BYTES // 4), dtype=torch.float32, device=device or "cuda")
+
+
+def _pack(fn, children, n_mass, n_com, n_min, n_max, box, mac, nodes, what):
+    m = children.shape[0]
+    check(fn(ptr(children, torch.int32, "children", numel=2 * m),
+             ptr(n_mass, torch.float64, "n_mass", numel=m),
+             ptr(n_com, torch.float64, "n_com", numel=3 * m),
+             ptr(n_min, torch.float32, "n_min", numel=3 * m),
+             ptr(n_max, torch.float32, "n_max", numel=3 * m), m, float(box), int(mac),
+             ptr(nodes, torch.float32, "nodes", numel=8 * m), stream_handle()), what)
+    return nodes
+
+
+def pack_nodes(children, n_mass, n_com, n_min, n_max, box, mac=MAC_REF, nodes=None):
+    """Packs normalised node arrays into FP32 records for acceptance test mac."""
+    nodes = node_buffer(children.shape[0], children.device) if nodes is None else nodes
+    return _pack(load().cosmo_pack_nodes, children, n_mass, n_com, n_min, n_max, box, mac,
+                 nodes, "pack_nodes")
+
+
+def normalize_pack(children, n_mass, n_com, n_min, n_max, box, mac=MAC_REF, nodes=None):
+    """normalize_com fused with pack_nodes (n_com is normalised in place)."""
+    nodes = node_buffer(children.shape[0], children.device) if nodes is None else nodes
+    return _pack(load().cosmo_normalize_pack, children, n_mass, n_com, n_min, n_max, box, mac,
+                 nodes, "normalize_pack")
+
+
+def validate_tree(children, parents, n: int, ctx=None) -> int:
+    """Defect bits of the tree (0 = well formed); synchronises the stream."""
+    ctx = ctx or shared_context(n)
+    flags = torch.zeros(1, dtype=torch.int32, device=children.device)
+    check(load().cosmo_validate_tree(ctx.handle, ptr(children, torch.int32, "children"),
+                                     ptr(parents, torch.int32, "parents"), int(n),
+                                     ptr(flags, torch.int32, "flags"), stream_handle()),
+          "validate_tree")
+    return int(flags.item())

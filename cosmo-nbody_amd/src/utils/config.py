@@ -56,6 +56,7 @@ MORTON_MASKS = "full21"    # "full21" (21 bits/axis) or "ref10" (morton.py:33-36
 RUN_STEPS = N_STEPS
 RENDER_FRAC_BITS = 20      # fixed-point fraction bits of the density bins
 COMPENSATED = False        # Neumaier-compensated far-field accumulation
+MAC = "ref"                # acceptance test: "ref" (physics_kernels.py:128-133) or "aabb"
 LEARNED_THETA = False
 
 
@@ -86,6 +87,7 @@ class Config:
     RUN_STEPS: int = RUN_STEPS
     RENDER_FRAC_BITS: int = RENDER_FRAC_BITS
     COMPENSATED: bool = COMPENSATED
+    MAC: str = MAC
     LEARNED_THETA: bool = LEARNED_THETA
     NAME: str = "reference_defaults"
 

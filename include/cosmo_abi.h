@@ -1,0 +1,219 @@
+/*
+ * cosmo_abi.h -- C ABI of libcosmo_hip.so, the gfx950 (MI355X) kernels of
+ * the per-step gravity path of a cosmological Barnes-Hut N-body code.
+ *
+ * Plain C: device pointers are ordinary pointers obtained from any HIP
+ * allocator (torch tensors' data_ptr(), hipMalloc), sizes are int64_t and
+ * `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ * Every call only enqueues work on `stream` and returns a status:
+ * COSMO_OK, or an error whose text cosmo_status_string() gives.  Nothing is
+ * allocated per call except by cosmo_ctx_create().
+ *
+ * Each entry point names the reference (Cosmo-NBody) function it replaces.
+ * The reference launches Numba kernels as kern[blocks, threads](args) on
+ * CuPy arrays; the Python drop-ins in cosmo-nbody_amd/src/kernels keep those
+ * names and call the functions below through ctypes.
+ *
+ * Layouts are the reference's: positions/velocities SoA float[n];
+ * force float[n][3]; children int32[M][2]; node CoM/min/max [M][3];
+ * M = 2n - 1 nodes, leaves 0..n-1, internal node i at n+i, root n.
+ * Node mass and CoM are FP64 here (the reference stores FP32): the upward
+ * pass accumulates in FP64 and the traversal rounds them to FP32.
+ */
+#ifndef COSMO_ABI_H
+#define COSMO_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------- statuses */
+#define COSMO_OK 0
+#define COSMO_E_INVALID 1        /* bad argument (null pointer, size)      */
+#define COSMO_E_HIP 2            /* HIP runtime error                      */
+#define COSMO_E_TOO_LARGE 3      /* n exceeds the context's capacity       */
+#define COSMO_E_MALFORMED_TREE 4 /* tree validator found a defect          */
+
+/* Morton mask sets: FULL21 = 21 bits per axis (63-bit code);
+ * REF10 = the masks of the reference's spread_bits as written, which keep
+ * each axis' low 10 bits (src/kernels/morton.py:33-36). */
+#define COSMO_MASK_FULL21 0
+#define COSMO_MASK_REF10 1
+
+/* Acceptance tests of the traversal: REF = the reference's
+ * (physics_kernels.py:128-133: x-extent of the AABB vs theta * distance to
+ * the CoM); AABB = largest extent + CoM offset + minimum-image guard. */
+#define COSMO_MAC_REF 0
+#define COSMO_MAC_AABB 1
+
+/* traversal flags */
+#define COSMO_TRAV_COMPENSATED 1 /* Neumaier-compensated accumulation     */
+#define COSMO_TRAV_STATS 2       /* count interactions / node visits      */
+
+/* Fixed-point fraction bits of the density bins used by the drop-ins. */
+#define COSMO_RENDER_FRAC_BITS 20
+
+/* Packed FP32 node record read by the traversal (32 bytes).
+ * REF:  a = sx*sx (x-extent, clamped to box when > box/2), b = 0.
+ * AABB: a = largest AABB extent s, b = |CoM - AABB centre|.
+ * c1 == -1 marks a leaf. */
+typedef struct cosmo_node {
+  float cx, cy, cz, m;
+  float a, b;
+  int32_t c1, c2;
+} cosmo_node;
+
+typedef struct cosmo_ctx cosmo_ctx;
+
+const char *cosmo_status_string(int status);
+
+/* Device-side scratch for the sort, the packed node records and the
+ * validator, sized for up to n_max particles on the current device. */
+int cosmo_ctx_create(int64_t n_max, cosmo_ctx **out);
+int cosmo_ctx_destroy(cosmo_ctx *ctx);
+/* Node records inside the context (2*n_max-1 entries). */
+cosmo_node *cosmo_ctx_nodes(cosmo_ctx *ctx);
+
+/* src/kernels/morton.py:65-91 compute_codes(pos_x, pos_y, pos_z, codes,
+ * indices, n_bodies).  indices may be NULL. */
+int cosmo_compute_codes(const float *pos_x, const float *pos_y,
+                        const float *pos_z, uint64_t *codes, uint32_t *indices,
+                        int64_t n_bodies, int mask_set, void *stream);
+
+/* src/core/state.py:35 sort_idx = cp.argsort(codes): stable LSD radix sort
+ * on the key (code, index).  codes_out = sorted codes, perm[k] = index of the
+ * k-th smallest.  key_bits (1..64) = how many low bits of the code vary
+ * (63 for FULL21, 30 for REF10).  codes_in may equal codes_out. */
+int cosmo_sort_by_code(cosmo_ctx *ctx, const uint64_t *codes_in,
+                       uint64_t *codes_out, uint32_t *perm, int64_t n,
+                       int key_bits, void *stream);
+
+/* src/core/state.py:37-40 the gathers x[sort_idx]: dst[a][k] = src[a][perm[k]]
+ * for n_arrays (<= 8) float arrays; src/dst are host arrays of device
+ * pointers; dst must not alias src. */
+int cosmo_permute(const uint32_t *perm, int n_arrays, const float *const *src,
+                  float *const *dst, int64_t n, void *stream);
+
+/* src/kernels/tree_kernels.py:236-262 reset_tree_buffers(child, parent,
+ * counters, root_idx). */
+int cosmo_reset_tree_buffers(int32_t *child, int32_t *parent,
+                             int32_t *counters, int32_t *root_idx,
+                             int64_t n_nodes, void *stream);
+
+/* src/kernels/tree_kernels.py:60-129 build_tree(codes, children, parents). */
+int cosmo_build_tree(const uint64_t *codes, int32_t *children,
+                     int32_t *parents, int64_t n, void *stream);
+
+/* src/kernels/tree_kernels.py:131-191 compute_multipoles(pos_x, pos_y, pos_z,
+ * mass, children, parents, n_mass, n_com, n_min, n_max, counts); FP64 mass
+ * and mass-weighted CoM sums.  counts must be zero (reset_tree_buffers). */
+int cosmo_compute_multipoles(const float *pos_x, const float *pos_y,
+                             const float *pos_z, const float *mass,
+                             const int32_t *children, const int32_t *parents,
+                             double *n_mass, double *n_com, float *n_min,
+                             float *n_max, int32_t *counts, int64_t n,
+                             void *stream);
+
+/* src/kernels/tree_kernels.py:193-214 normalize_com(n_mass, n_com, n_nodes). */
+int cosmo_normalize_com(const double *n_mass, double *n_com, int64_t n_nodes,
+                        void *stream);
+
+/* src/kernels/tree_kernels.py:216-234 find_root(parents, root_idx): the
+ * largest parentless index (the unique one in a well-formed tree); root_idx
+ * must hold 0 on entry (reset_tree_buffers). */
+int cosmo_find_root(const int32_t *parents, int32_t *root_idx,
+                    int64_t n_nodes, void *stream);
+
+/* src/kernels/tree_kernels.py:264-314 build_top_tree_cache(child, root_idx,
+ * top_nodes, node_to_cache, cache_size); node_to_cache pre-filled with -1. */
+int cosmo_build_top_tree_cache(const int32_t *child, const int32_t *root_idx,
+                               int32_t *top_nodes, int32_t *node_to_cache,
+                               int64_t cache_size, void *stream);
+
+/* Packs the reference-layout node arrays into cosmo_node records for the
+ * acceptance test `mac` (no reference counterpart: the reference reads its
+ * arrays directly, physics_kernels.py:96-113). */
+int cosmo_pack_nodes(const int32_t *children, const double *n_mass,
+                     const double *n_com, const float *n_min,
+                     const float *n_max, int64_t n_nodes, double box, int mac,
+                     cosmo_node *nodes, void *stream);
+
+/* normalize_com and cosmo_pack_nodes fused in one pass. */
+int cosmo_normalize_pack(const int32_t *children, const double *n_mass,
+                         double *n_com, const float *n_min, const float *n_max,
+                         int64_t n_nodes, double box, int mac,
+                         cosmo_node *nodes, void *stream);
+
+/* Barnes-Hut traversal over packed records for particles [off, off+count)
+ * ∩ [0, n_bodies) in Morton-contiguous groups of 64 (one wavefront each),
+ * writing force[i][0..2] (accelerations).  theta_group (nullable, device
+ * float[ceil(count/64)]) gives each group's opening angle in place of
+ * theta.  stats (nullable, device uint64[2]) accumulates interactions and
+ * node visits when flags has COSMO_TRAV_STATS. */
+int cosmo_traverse(const float *pos_x, const float *pos_y, const float *pos_z,
+                   const cosmo_node *nodes, const int32_t *root_idx,
+                   float *force, double theta, const float *theta_group,
+                   double G, double soft, double box, int64_t off,
+                   int64_t count, int64_t n_bodies, int mac, int flags,
+                   uint64_t *stats, void *stream);
+
+/* src/kernels/physics_kernels.py:14-153 compute_forces(pos_x, pos_y, pos_z,
+ * children, n_mass, n_com, n_min, n_max, force, theta, G, soft, box, root,
+ * top_nodes, node_to_cache, off) for particles [off, off+count): packs the
+ * nodes into the context and runs cosmo_traverse with the reference test.
+ * top_nodes/node_to_cache are accepted for signature compatibility; the
+ * reference only uses them as an on-chip copy of the same node data. */
+int cosmo_compute_forces(cosmo_ctx *ctx, const float *pos_x,
+                         const float *pos_y, const float *pos_z,
+                         const int32_t *children, const double *n_mass,
+                         const double *n_com, const float *n_min,
+                         const float *n_max, float *force, double theta,
+                         double G, double soft, double box,
+                         const int32_t *root_idx, const int32_t *top_nodes,
+                         const int32_t *node_to_cache, int64_t off,
+                         int64_t count, int64_t n_bodies, void *stream);
+
+/* src/kernels/physics_kernels.py:155-203 integrate(pos_x, pos_y, pos_z,
+ * vel_x, vel_y, vel_z, force, a, H, dt): c_drag = 1 - H*dt and
+ * c_kick = c_drift = dt/a are formed in FP64 and rounded to FP32, then
+ * cosmo_kick_drift runs. */
+int cosmo_integrate(float *pos_x, float *pos_y, float *pos_z, float *vel_x,
+                    float *vel_y, float *vel_z, const float *force, double a,
+                    double H, double dt, int64_t n, void *stream);
+
+/* v = v*c_drag + F*c_kick;  x = x + v*c_drift;  x -= floor(x)  (FP32). */
+int cosmo_kick_drift(float *pos_x, float *pos_y, float *pos_z, float *vel_x,
+                     float *vel_y, float *vel_z, const float *force,
+                     float c_drag, float c_kick, float c_drift, int64_t n,
+                     void *stream);
+
+/* src/kernels/render_kernels.py:12-47 render_density(pos_x, pos_y, pos_z,
+ * grid, mvp, res) into fixed-point bins: grid[res][res] uint64 gains
+ * rint(2.5/w * 2^frac_bits) per visible particle.  mvp: host float[16],
+ * row-major.  The caller clears grid. */
+int cosmo_render_density(const float *pos_x, const float *pos_y,
+                         const float *pos_z, uint64_t *grid, const float *mvp,
+                         int32_t res, int64_t n, int frac_bits, void *stream);
+
+/* FP64 direct sum (no reference counterpart; the error yardstick): for each
+ * target t, acc[t][k] = sum_j G m_j (d^2+eps^2)^-3/2 d_k over all n sources
+ * with the traversal's minimum image and d^2 > 1e-7 cut. */
+int cosmo_direct_sum(const float *pos_x, const float *pos_y,
+                     const float *pos_z, const float *mass, int64_t n,
+                     const int64_t *targets, int64_t n_targets, double G,
+                     double soft, double *acc, void *stream);
+
+/* Tree validator (no reference counterpart): writes a bit set to
+ * flags_out (device int32[1]): 1 = parentless-node count != 1,
+ * 2 = root != n, 4 = inconsistent child/parent link, 8 = not every leaf
+ * reachable from the root exactly once. */
+int cosmo_validate_tree(cosmo_ctx *ctx, const int32_t *children,
+                        const int32_t *parents, int64_t n, int32_t *flags_out,
+                        void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,417 @@
+"""GPU parity: every stage of the HIP path against the CPU oracle on the
+same seeded inputs (bit-exact for codes, permutation, tree, moments, update
+and density grid; forces within the stated tolerances)."""
+import numpy as np
+import pytest
+
+from conftest import gpu_available, rel_rms, zeldovich
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a HIP device")]
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    return torch
+
+
+def dev(a, torch):
+    a = np.ascontiguousarray(a)
+    if a.dtype == np.uint64:
+        a = a.view(np.int64)
+    elif a.dtype == np.uint32:
+        a = a.view(np.int32)
+    return torch.from_numpy(a).to("cuda")
+
+
+def host(t, dtype=None):
+    a = t.detach().cpu().numpy()
+    return a.view(dtype) if dtype is not None else a
+
+
+def sorted_case(orc, pos, mask_set=0):
+    codes, _ = orc.compute_codes(*pos, mask_set=mask_set)
+    perm = orc.argsort_stable(codes)
+    return [p[perm] for p in pos], codes[perm]
+
+
+# ----------------------------------------------------------------- Morton
+
+@pytest.mark.parametrize("mask_set", [0, 1])
+def test_codes_bit_exact(orc, torch_mod, ics32, mask_set):
+    torch = torch_mod
+    from src.kernels.morton import compute_codes
+    pos = [p.copy() for p in ics32]
+    pos[0][:4] = [1.0, -0.0, 0.5, np.nextafter(np.float32(1.0), np.float32(0.0))]
+    n = len(pos[0])
+    ref, idx = orc.compute_codes(*pos, mask_set=mask_set)
+    codes = torch.zeros(n, dtype=torch.int64, device="cuda")
+    ind = torch.zeros(n, dtype=torch.int32, device="cuda")
+    compute_codes[(n + 255) // 256, 256](*[dev(p, torch) for p in pos], codes, ind, n,
+                                         mask_set=mask_set)
+    assert np.array_equal(host(codes, np.uint64), ref)
+    assert np.array_equal(host(ind, np.uint32), idx)
+
+
+# ------------------------------------------------------------------- sort
+
+def _gpu_sort(torch, codes_np, bits=63):
+    from src.kernels import _hip
+    n = len(codes_np)
+    ctx = _hip.Context(n)
+    c = dev(codes_np, torch)
+    out = torch.empty_like(c)
+    perm = torch.empty(n, dtype=torch.int32, device="cuda")
+    L = _hip.load()
+    _hip.check(L.cosmo_sort_by_code(ctx.handle, _hip.ptr(c, _hip.U64, "c"),
+                                    _hip.ptr(out, _hip.U64, "o"), _hip.ptr(perm, _hip.U32, "p"),
+                                    n, bits, _hip.stream_handle()), "sort")
+    torch.cuda.synchronize()
+    return host(out, np.uint64), host(perm, np.uint32)
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 4095, 4096, 4097, 100003])
+def test_sort_matches_stable_argsort(orc, torch_mod, n):
+    rng = np.random.default_rng(n)
+    for codes in (rng.integers(0, 2 ** 63, size=n, dtype=np.int64).astype(np.uint64),
+                  rng.integers(0, 7, size=n).astype(np.uint64) << np.uint64(40)):
+        out, perm = _gpu_sort(torch_mod, codes)
+        expect = orc.argsort_stable(codes)
+        assert np.array_equal(perm, expect)
+        assert np.array_equal(perm, np.argsort(codes, kind="stable"))
+        assert np.array_equal(out, codes[expect])
+
+
+def test_sort_ref10_bits_and_c0(orc, torch_mod, ics32):
+    for ms, bits in ((0, 63), (1, 30)):
+        codes, _ = orc.compute_codes(*ics32, mask_set=ms)
+        out, perm = _gpu_sort(torch_mod, codes, bits)
+        assert np.array_equal(perm, orc.argsort_stable(codes))
+
+
+def test_state_sort_buffer_by_morton(orc, torch_mod, ics32):
+    torch = torch_mod
+    from src.core.state import SimulationState
+    from src.utils import config as cfg
+    pos, vel = zeldovich(32)
+    n = len(pos[0])
+    st = SimulationState(n, pos, vel)
+    st.mass = dev(np.linspace(0.5, 1.5, n).astype(np.float32), torch)
+    m0 = host(st.mass)
+    st.sort_buffer_by_morton(cfg.preset("small_32"))
+    codes, _ = orc.compute_codes(*pos)
+    perm = orc.argsort_stable(codes)
+    assert np.array_equal(host(st.sort_idx, np.uint32), perm)
+    assert np.array_equal(host(st.codes, np.uint64), codes[perm])
+    for a, b in zip(st.pos + st.vel, pos + vel):
+        assert np.array_equal(host(a), b[perm])
+    assert np.array_equal(host(st.mass), m0[perm])
+
+
+# ------------------------------------------------------------------- tree
+
+def _gpu_tree(torch, codes_sorted):
+    from src.kernels.tree_kernels import build_tree, reset_tree_buffers
+    n = len(codes_sorted)
+    M = 2 * n - 1
+    ch = torch.zeros((M, 2), dtype=torch.int32, device="cuda")
+    par = torch.zeros(M, dtype=torch.int32, device="cuda")
+    cnt = torch.ones(M, dtype=torch.int32, device="cuda")
+    root = torch.ones(1, dtype=torch.int32, device="cuda")
+    reset_tree_buffers[(M + 255) // 256, 256](ch, par, cnt, root)
+    build_tree[(n + 255) // 256, 256](dev(codes_sorted, torch), ch, par)
+    return ch, par, cnt, root
+
+
+def test_build_tree_bit_exact(orc, torch_mod, ics32):
+    torch = torch_mod
+    from src.kernels.tree_kernels import find_root, validate_tree
+    _, codes = sorted_case(orc, ics32)
+    ch, par, cnt, root = _gpu_tree(torch, codes)
+    ref_ch, ref_par = orc.build_tree(codes)
+    assert np.array_equal(host(ch), ref_ch)
+    assert np.array_equal(host(par), ref_par)
+    assert int(host(cnt).max()) == 0
+    M = 2 * len(codes) - 1
+    find_root[(M + 255) // 256, 256](par, root)
+    assert int(root.item()) == orc.find_root(ref_par) == len(codes)
+    assert validate_tree(ch, par, len(codes)) == 0
+
+
+def test_build_tree_equal_codes_and_validator(orc, torch_mod, ics16):
+    """REF10 codes at 16^3 contain equal-code runs; children follow the
+    reference tie-break bit for bit and the validator agrees with the oracle."""
+    torch = torch_mod
+    from src.kernels.tree_kernels import validate_tree
+    _, codes = sorted_case(orc, ics16, mask_set=1)
+    ch, par, _, _ = _gpu_tree(torch, codes)
+    ref_ch, ref_par = orc.build_tree(codes)
+    assert np.array_equal(host(ch), ref_ch)
+    ref_flags = orc.validate_tree(ref_ch, ref_par, len(codes))
+    gpu_flags = validate_tree(ch, par, len(codes))
+    assert (gpu_flags == 0) == (ref_flags == 0)
+    dup = np.full(4, 7, np.uint64)
+    ch, par, _, _ = _gpu_tree(torch, dup)
+    assert validate_tree(ch, par, 4) != 0
+
+
+def _gpu_moments(torch, pos, mass, ch, par):
+    from src.kernels.tree_kernels import compute_multipoles
+    n = len(pos[0])
+    M = 2 * n - 1
+    nm = torch.zeros(M, dtype=torch.float64, device="cuda")
+    nc = torch.zeros((M, 3), dtype=torch.float64, device="cuda")
+    lo = torch.zeros((M, 3), dtype=torch.float32, device="cuda")
+    hi = torch.zeros((M, 3), dtype=torch.float32, device="cuda")
+    cnt = torch.zeros(M, dtype=torch.int32, device="cuda")
+    compute_multipoles[(n + 255) // 256, 256](*[dev(p, torch) for p in pos], dev(mass, torch),
+                                              dev(ch, torch), dev(par, torch), nm, nc, lo, hi,
+                                              cnt)
+    return nm, nc, lo, hi
+
+
+def test_multipoles_bit_exact(orc, torch_mod, ics32):
+    torch = torch_mod
+    from src.kernels.tree_kernels import normalize_com
+    pos, codes = sorted_case(orc, ics32)
+    n = len(codes)
+    mass = np.random.default_rng(7).uniform(0.5, 1.5, n).astype(np.float32) / n
+    ch, par = orc.build_tree(codes)
+    nm, nc, lo, hi = _gpu_moments(torch, pos, mass, ch, par)
+    r_nm, r_nc, r_lo, r_hi = orc.compute_multipoles(*pos, mass, ch, par)
+    assert np.array_equal(host(nm), r_nm)
+    assert np.array_equal(host(nc), r_nc)
+    assert np.array_equal(host(lo), r_lo)
+    assert np.array_equal(host(hi), r_hi)
+    M = 2 * n - 1
+    normalize_com[(M + 255) // 256, 256](nm, nc, M)
+    assert np.array_equal(host(nc), orc.normalize_com(r_nm, r_nc))
+
+
+def test_top_tree_cache(orc, torch_mod, ics16):
+    torch = torch_mod
+    from src.kernels.tree_kernels import build_top_tree_cache
+    _, codes = sorted_case(orc, ics16)
+    ch, par = orc.build_tree(codes)
+    n = len(codes)
+    top = torch.full((256,), -1, dtype=torch.int32, device="cuda")
+    n2c = torch.full((2 * n - 1,), -1, dtype=torch.int32, device="cuda")
+    root = torch.tensor([n], dtype=torch.int32, device="cuda")
+    build_top_tree_cache[1, 1](dev(ch, torch), root, top, n2c, 256)
+    r_top, r_n2c = orc.build_top_tree_cache(ch, n, 256)
+    assert np.array_equal(host(top), r_top)
+    assert np.array_equal(host(n2c), r_n2c)
+
+
+# ----------------------------------------------------------------- forces
+
+def _case(orc, pos, mass=None):
+    pos, codes = sorted_case(orc, pos)
+    n = len(codes)
+    mass = np.full(n, 1.0 / n, np.float32) if mass is None else mass
+    ch, par = orc.build_tree(codes)
+    nm, nc, lo, hi = orc.compute_multipoles(*pos, mass, ch, par)
+    com = orc.normalize_com(nm, nc)
+    return pos, mass, ch, nm, com, lo, hi
+
+
+@pytest.mark.parametrize("theta", [0.3, 0.5, 0.7])
+def test_compute_forces_reference_mac(orc, torch_mod, ics32, theta):
+    """compute_forces drop-in (batched launches like engine.py:91-111):
+    identical interaction sets to the FP32 oracle and <= 1e-5 relative RMS
+    against the reference method's FP64 arithmetic."""
+    torch = torch_mod
+    from src.kernels.physics_kernels import compute_forces
+    pos, mass, ch, nm, com, lo, hi = _case(orc, ics32)
+    n = len(mass)
+    soft = 0.02 / 32
+    force = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+    root = torch.tensor([n], dtype=torch.int32, device="cuda")
+    args = [dev(p, torch) for p in pos] + [dev(ch, torch), dev(nm, torch), dev(com, torch),
+                                           dev(lo, torch), dev(hi, torch), force]
+    for off in range(0, n, 5000):
+        c = min(5000, n - off)
+        compute_forces[(c + 255) // 256, 256](*args, theta, 1.0, soft, 1.0, root, None, None, off)
+    f = host(force)
+    f64 = orc.compute_forces(*pos, ch, nm, com, lo, hi, theta, 1.0, soft, 1.0, n, precision=64)
+    f32 = orc.compute_forces(*pos, ch, nm, com, lo, hi, theta, 1.0, soft, 1.0, n, precision=32)
+    e64, e32 = rel_rms(f, f64), rel_rms(f, f32)
+    print(f"theta={theta} relRMS vs FP64 {e64:.3e} vs FP32 oracle {e32:.3e}")
+    assert e64 <= 1e-5
+    assert e32 <= 2e-6
+
+
+@pytest.mark.parametrize("mac", [0, 1])
+def test_traverse_interaction_counts(orc, torch_mod, ics32, mac):
+    torch = torch_mod
+    from src.kernels.physics_kernels import traverse
+    from src.kernels.tree_kernels import pack_nodes
+    pos, mass, ch, nm, com, lo, hi = _case(orc, ics32)
+    n = len(mass)
+    soft = 0.02 / 32
+    nodes = pack_nodes(dev(ch, torch), dev(nm, torch), dev(com, torch), dev(lo, torch),
+                       dev(hi, torch), 1.0, mac)
+    force = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+    stats = torch.zeros(2, dtype=torch.int64, device="cuda")
+    root = torch.tensor([n], dtype=torch.int32, device="cuda")
+    traverse(*[dev(p, torch) for p in pos], nodes, root, force, 0.5, 1.0, soft, 1.0, mac=mac,
+             stats=stats)
+    f32, ni, nv = orc.compute_forces(*pos, ch, nm, com, lo, hi, 0.5, 1.0, soft, 1.0, n,
+                                     precision=32, mac=mac, counts=True)
+    s = host(stats)
+    assert int(s[0]) == int(ni.sum())
+    assert int(s[1]) == int(nv.sum())
+    assert rel_rms(host(force), f32) <= 2e-6
+
+
+def test_traverse_group_theta_and_compensated(orc, torch_mod, ics32):
+    torch = torch_mod
+    from src.kernels.physics_kernels import traverse
+    from src.kernels.tree_kernels import pack_nodes
+    pos, mass, ch, nm, com, lo, hi = _case(orc, ics32)
+    n = len(mass)
+    soft = 0.02 / 32
+    g = (n + 63) // 64
+    tg = np.random.default_rng(3).uniform(0.3, 0.9, g).astype(np.float32)
+    nodes = pack_nodes(dev(ch, torch), dev(nm, torch), dev(com, torch), dev(lo, torch),
+                       dev(hi, torch), 1.0, 1)
+    root = torch.tensor([n], dtype=torch.int32, device="cuda")
+    P = [dev(p, torch) for p in pos]
+    out = {}
+    for comp in (False, True):
+        force = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+        stats = torch.zeros(2, dtype=torch.int64, device="cuda")
+        traverse(*P, nodes, root, force, 0.5, 1.0, soft, 1.0, mac=1, theta_group=dev(tg, torch),
+                 compensated=comp, stats=stats)
+        out[comp] = (host(force), host(stats))
+    per_particle = np.repeat(tg, 64)[:n]
+    f32, ni, nv = orc.compute_forces(*pos, ch, nm, com, lo, hi, 0.5, 1.0, soft, 1.0, n,
+                                     precision=32, mac=1, theta_g=per_particle, counts=True)
+    f64 = orc.compute_forces(*pos, ch, nm, com, lo, hi, 0.5, 1.0, soft, 1.0, n, precision=64,
+                             mac=1, theta_g=per_particle)
+    for comp in (False, True):
+        assert int(out[comp][1][0]) == int(ni.sum())
+    assert rel_rms(out[False][0], f32) <= 2e-6
+    # compensation can only help against the FP64 arithmetic
+    assert rel_rms(out[True][0], f64) <= rel_rms(out[False][0], f64) * 1.05
+
+
+def test_direct_sum_matches_cpu(orc, torch_mod, ics16):
+    torch = torch_mod
+    from src.kernels.physics_kernels import direct_sum
+    pos = ics16
+    n = len(pos[0])
+    mass = np.full(n, 1.0 / n, np.float32)
+    tg = np.arange(0, n, 7)
+    g = host(direct_sum(*[dev(p, torch) for p in pos], dev(mass, torch), dev(tg, torch), 1.0,
+                        0.02 / 16))
+    c = orc.direct_sum(*pos, mass, tg, 1.0, 0.02 / 16)
+    assert rel_rms(g, c) < 1e-12
+
+
+# --------------------------------------------------------- update / render
+
+def test_integrate_bit_exact(orc, torch_mod, ics32):
+    torch = torch_mod
+    from src.kernels.physics_kernels import integrate
+    pos, vel = zeldovich(32)
+    n = len(pos[0])
+    force = np.random.default_rng(9).standard_normal((n, 3)).astype(np.float32)
+    a, H, dt = 0.02, 23970.3, 6.54e-7
+    P = [dev(p, torch) for p in pos]
+    V = [dev(v, torch) for v in vel]
+    integrate[(n + 255) // 256, 256](*P, *V, dev(force, torch), a, H, dt)
+    ref = orc.integrate(*pos, *vel, force, a, H, dt)
+    for t, r in zip(P + V, ref):
+        assert np.array_equal(host(t), r)
+
+
+@pytest.mark.parametrize("res", [256, 1024])
+def test_render_bit_exact(orc, torch_mod, ics32, res):
+    torch = torch_mod
+    from src.kernels.render_kernels import render_density
+    from src.utils import config as cfg
+    from src.utils.visualizer import get_mvp_matrix
+    c = cfg.preset("repo_128")
+    for step in (0, 777):
+        mvp = get_mvp_matrix(step, (0.5, 0.5, 0.5), 1.0, c)
+        grid = torch.zeros((res, res), dtype=torch.int64, device="cuda")
+        n = len(ics32[0])
+        render_density[(n + 255) // 256, 256](*[dev(p, torch) for p in ics32], grid, mvp, res)
+        ref = orc.render_density(*ics32, mvp, res)
+        assert np.array_equal(host(grid, np.uint64), ref)
+
+
+# --------------------------------------------------------------- end to end
+
+def _run_engine(preset, n_grid, steps, seed=42, **kw):
+    from src.core.engine import SimulationEngine
+    from src.core.state import SimulationState
+    from src.utils import config as cfg
+    c = cfg.preset(preset, N_GRID=n_grid, SOFTENING=0.02 / n_grid, SEED=seed, **kw)
+    pos, vel = zeldovich(n_grid, seed)
+    st = SimulationState(len(pos[0]), pos, vel)
+    eng = SimulationEngine(c, st)
+    for _ in range(steps):
+        eng.step()
+        eng.camera.capture(eng.steps_done - 1, *st.pos)
+    return eng, c, pos, vel
+
+
+def test_engine_steps_match_oracle(orc, torch_mod):
+    """Five steps of small_32 on the GPU against the same five steps done
+    stage by stage with the oracle (bit-exact sort/tree/moments/update for
+    identical forces; forces within 1e-5, so positions are compared with a
+    tolerance after the first step)."""
+    from src.utils.cosmology import get_hubble_param
+    eng, c, pos, vel = _run_engine("small_32", 32, 1)
+    st = eng.state
+    n = len(pos[0])
+    codes, _ = orc.compute_codes(*pos)
+    perm = orc.argsort_stable(codes)
+    p = [x[perm] for x in pos]
+    v = [x[perm] for x in vel]
+    mass = np.full(n, 1.0 / n, np.float32)
+    ch, par = orc.build_tree(codes[perm])
+    nm, nc, lo, hi = orc.compute_multipoles(*p, mass, ch, par)
+    com = orc.normalize_com(nm, nc)
+    assert np.array_equal(host(eng.tree_manager.child), ch)
+    assert np.array_equal(host(eng.tree_manager.com), com)
+    f64 = orc.compute_forces(*p, ch, nm, com, lo, hi, c.THETA, c.G, c.SOFTENING, 1.0, n)
+    assert rel_rms(host(st.force), f64) <= 1e-5
+    a0 = c.A_START
+    H = get_hubble_param(a0, c)
+    dt = c.DA / (a0 * H) * 0.8
+    ref = orc.integrate(*p, *v, host(st.force), a0, H, dt)
+    for t, r in zip(st.pos + st.vel, ref):
+        assert np.array_equal(host(t), r)
+
+
+def test_engine_deterministic(torch_mod):
+    e1, *_ = _run_engine("small_32", 32, 3)
+    e2, *_ = _run_engine("small_32", 32, 3)
+    for a, b in zip(e1.state.pos + e1.state.vel, e2.state.pos + e2.state.vel):
+        assert np.array_equal(host(a), host(b))
+    assert np.array_equal(host(e1.camera.grid_d), host(e2.camera.grid_d))
+
+
+def test_full_size_properties(torch_mod):
+    """repo_128 (N = 128^3): sorted codes, a permutation, a valid tree with
+    root n and root mass 1, finite forces and a non-empty density grid."""
+    torch = torch_mod
+    eng, c, pos, vel = _run_engine("repo_128", 128, 1)
+    st = eng.state
+    n = st.n
+    codes = host(st.codes, np.uint64)
+    assert np.all(codes[1:] >= codes[:-1])
+    perm = host(st.sort_idx, np.uint32)
+    assert np.array_equal(np.sort(perm), np.arange(n, dtype=np.uint32))
+    tm = eng.tree_manager
+    assert tm.validate() == 0
+    assert int(tm.root_idx.item()) == n
+    assert abs(float(tm.mass[n].item()) - 1.0) < 1e-12
+    f = host(st.force)
+    assert np.all(np.isfinite(f))
+    grid = host(eng.camera.grid_d, np.uint64)
+    assert grid.sum() > 0
