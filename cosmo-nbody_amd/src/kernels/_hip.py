@@ -37,7 +37,7 @@ _f64 = C.c_double
 _f32 = C.c_float
 _int = C.c_int
 
-# name -> argtypes, mirroring include/cosmo_abi.h
+# name -> (argtypes, restype), mirroring include/cosmo_abi.h and include/cosmo_mlp.h
 SIGNATURES = {
     "cosmo_status_string": ([_int], C.c_char_p),
     "cosmo_ctx_create": ([_i64, C.POINTER(_p)], _int),
@@ -62,6 +62,14 @@ SIGNATURES = {
     "cosmo_render_density": ([_p, _p, _p, _p, _p, _i32, _i64, _int, _p], _int),
     "cosmo_direct_sum": ([_p, _p, _p, _p, _i64, _p, _i64, _f64, _f64, _p, _p], _int),
     "cosmo_validate_tree": ([_p, _p, _p, _i64, _p, _p], _int),
+    # include/cosmo_mlp.h
+    "cosmo_group_features": ([_p] * 9 + [_i64, _f64, _f64, _p, _p], _int),
+    "cosmo_assemble_rows": ([_p, _i64, _p, _int, _p, _p], _int),
+    "cosmo_mlp_forward": ([_p, _i64, _p, _p, _p, _p], _int),
+    "cosmo_mlp_grad": ([_p, _p, _i64, _p, _p, _p, _int, _p, _p], _int),
+    "cosmo_mlp_adam": ([_p, _p, _p, _p, _f32, _f32, _f32, _f32, _int, _p, _p], _int),
+    "cosmo_mlp_to_bf16": ([_p, _p, _p], _int),
+    "cosmo_select_theta": ([_p, _i64, _p, _p, _int, _f64, _f64, _p, _p, _p], _int),
 }
 
 _lib = None

@@ -1,0 +1,98 @@
+"""Learned opening angle: bindings of the MFMA MLP kernels and of the group
+features / theta selection (include/cosmo_mlp.h).  No reference counterpart.
+
+Tensors: X int16 [rows, 16] (BF16 bit patterns); parameters float32
+[PARAMS]; BF16 weights int16 [BF16_WEIGHTS]; features float32 [groups, 9].
+"""
+from __future__ import annotations
+
+import torch
+
+from ._hip import check, load, ptr, stream_handle
+
+IN, IN_PAD, HIDDEN, GROUP_FEATURES = 10, 16, 64, 9
+PARAMS = 16 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+BF16_WEIGHTS = 16 * 64 + 64 * 64 + 64
+GRAD_BLOCKS = 64
+
+_F32, _I16 = torch.float32, torch.int16
+
+
+def group_features(pos, tree, a, n_grid, out=None):
+    """Nine features per group of 64 sorted particles (tree: OctreeManager)."""
+    n = pos[0].shape[0]
+    G = (n + 63) // 64
+    out = torch.empty((G, GROUP_FEATURES), dtype=_F32, device=pos[0].device) if out is None else out
+    M = 2 * n - 1
+    check(load().cosmo_group_features(
+        *[ptr(t, _F32, "pos", numel=n) for t in pos],
+        ptr(tree.child, torch.int32, "children", numel=2 * M),
+        ptr(tree.parent, torch.int32, "parents", numel=M),
+        ptr(tree.mass, torch.float64, "n_mass", numel=M),
+        ptr(tree.com, torch.float64, "n_com", numel=3 * M),
+        ptr(tree.aabb_min, _F32, "n_min", numel=3 * M),
+        ptr(tree.aabb_max, _F32, "n_max", numel=3 * M), n, float(a), float(n_grid),
+        ptr(out, _F32, "feat", numel=G * GROUP_FEATURES), stream_handle()), "group_features")
+    return out
+
+
+def assemble_rows(feat, thetas, out=None):
+    G = feat.shape[0]
+    K = thetas.shape[0]
+    out = torch.empty((G * K, IN_PAD), dtype=_I16, device=feat.device) if out is None else out
+    check(load().cosmo_assemble_rows(ptr(feat, _F32, "feat"), G, ptr(thetas, _F32, "thetas"), K,
+                                     ptr(out, _I16, "X", numel=G * K * IN_PAD), stream_handle()),
+          "assemble_rows")
+    return out
+
+
+def forward(X, wbf, params, out=None):
+    rows = X.shape[0]
+    out = torch.empty(rows, dtype=_F32, device=X.device) if out is None else out
+    check(load().cosmo_mlp_forward(ptr(X, _I16, "X", numel=rows * IN_PAD), rows,
+                                   ptr(wbf, _I16, "wbf", numel=BF16_WEIGHTS),
+                                   ptr(params, _F32, "params", numel=PARAMS),
+                                   ptr(out, _F32, "Y", numel=rows), stream_handle()),
+          "mlp_forward")
+    return out
+
+
+def grad(X, T, wbf, params, partial=None, out=None, nblocks=GRAD_BLOCKS):
+    """Gradient of mean((MLP(X) - T)^2); out[PARAMS] = sum of squared errors."""
+    rows = X.shape[0]
+    dev = X.device
+    partial = (torch.empty(nblocks * (PARAMS + 1), dtype=_F32, device=dev)
+               if partial is None else partial)
+    out = torch.empty(PARAMS + 1, dtype=_F32, device=dev) if out is None else out
+    check(load().cosmo_mlp_grad(ptr(X, _I16, "X", numel=rows * IN_PAD), ptr(T, _F32, "T"), rows,
+                                ptr(wbf, _I16, "wbf", numel=BF16_WEIGHTS),
+                                ptr(params, _F32, "params", numel=PARAMS),
+                                ptr(partial, _F32, "partial", numel=nblocks * (PARAMS + 1)),
+                                int(nblocks), ptr(out, _F32, "grad", numel=PARAMS + 1),
+                                stream_handle()), "mlp_grad")
+    return out
+
+
+def adam(params, m, v, g, lr, step, wbf, beta1=0.9, beta2=0.999, eps=1e-8):
+    check(load().cosmo_mlp_adam(ptr(params, _F32, "params"), ptr(m, _F32, "m"), ptr(v, _F32, "v"),
+                                ptr(g, _F32, "grad"), float(lr), float(beta1), float(beta2),
+                                float(eps), int(step), ptr(wbf, _I16, "wbf"), stream_handle()),
+          "mlp_adam")
+
+
+def to_bf16(params, wbf):
+    check(load().cosmo_mlp_to_bf16(ptr(params, _F32, "params", numel=PARAMS),
+                                   ptr(wbf, _I16, "wbf", numel=BF16_WEIGHTS), stream_handle()),
+          "mlp_to_bf16")
+
+
+def select_theta(pred, groups, thetas, cost, lam, fallback, out, n_fallback=None):
+    K = thetas.shape[0]
+    check(load().cosmo_select_theta(ptr(pred, _F32, "pred", numel=groups * K), int(groups),
+                                    ptr(thetas, _F32, "thetas"), ptr(cost, _F32, "cost", numel=K),
+                                    K, float(lam), float(fallback),
+                                    ptr(out, _F32, "theta_group", numel=groups),
+                                    None if n_fallback is None else
+                                    ptr(n_fallback, (torch.int64, torch.uint64), "n_fallback"),
+                                    stream_handle()), "select_theta")
+    return out

@@ -53,10 +53,12 @@ def compute_forces(pos_x, pos_y, pos_z, children, n_mass, n_com, n_min, n_max, f
 
 
 def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0, count=None,
-             mac=MAC_REF, theta_group=None, compensated=False, stats=None):
+             mac=MAC_REF, theta_group=None, compensated=False, stats=None, n_nodes=None):
     """Traversal over packed node records; stats: int64[2] tensor accumulating
-    (interactions, node visits) when given."""
+    (interactions, node visits) when given.  The targets (pos_*) need not be
+    the tree's particles: n_nodes gives the tree size when they differ."""
     n = pos_x.shape[0]
+    n_nodes = 2 * n - 1 if n_nodes is None else int(n_nodes)
     count = n - off if count is None else count
     flags = (TRAV_COMPENSATED if compensated else 0) | (TRAV_STATS if stats is not None else 0)
     groups = (min(n, off + count) - off + 63) // 64
@@ -64,7 +66,7 @@ def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0
                                               numel=max(groups, 0))
     st = None if stats is None else ptr(stats, (torch.int64, torch.uint64), "stats", numel=2)
     check(load().cosmo_traverse(
-        *_xyz(pos_x, pos_y, pos_z, n), ptr(nodes, _F32, "nodes", numel=8 * (2 * n - 1)),
+        *_xyz(pos_x, pos_y, pos_z, n), ptr(nodes, _F32, "nodes", numel=8 * n_nodes),
         ptr(root, torch.int32, "root"), ptr(force, _F32, "force", numel=3 * n), float(theta), tg,
         float(G), float(soft), float(box), int(off), int(count), n, int(mac), flags, st,
         stream_handle()), "traverse")

@@ -1,0 +1,84 @@
+/*
+ * cosmo_mlp.h -- C ABI of the learned opening angle (part of
+ * libcosmo_hip.so).  No reference counterpart: the reference uses one fixed
+ * theta (src/utils/config.py:31, physics_kernels.py:87).
+ *
+ * A group is 64 Morton-consecutive particles (one wavefront of the
+ * traversal).  For every group, cosmo_group_features computes nine
+ * features; the MLP input row is [theta, features..., 0-padding to 16].
+ * The network 10 -> 64 -> 64 -> 1 (ReLU, BF16 weights, FP32 accumulation on
+ * MFMA) predicts log2 of the group's mean squared acceleration error
+ * relative to the mean squared acceleration of all particles.
+ * cosmo_select_theta picks, per group, the candidate theta minimising
+ * cost[k] + lambda * 2^prediction, with a fixed theta as fallback.
+ *
+ * Parameters: one FP32 vector of COSMO_MLP_PARAMS values laid out as
+ * W1[16][64] (rows >= 10 zero), b1[64], W2[64][64], b2[64], W3[64], b3, and
+ * a BF16 copy of W1|W2|W3 (COSMO_MLP_BF16_WEIGHTS values) read by the
+ * kernels.  Weights are row-major [inputs][outputs].
+ */
+#ifndef COSMO_MLP_H
+#define COSMO_MLP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define COSMO_MLP_IN 10
+#define COSMO_MLP_IN_PAD 16
+#define COSMO_MLP_HIDDEN 64
+#define COSMO_MLP_GROUP_FEATURES 9
+#define COSMO_MLP_PARAMS (16 * 64 + 64 + 64 * 64 + 64 + 64 + 1)
+#define COSMO_MLP_BF16_WEIGHTS (16 * 64 + 64 * 64 + 64)
+
+/* Per-group features feat[g][9] for groups g = 0..ceil(n/64)-1 from the
+ * sorted positions and the tree (reference layouts; FP64 normalised CoM):
+ *  0 log2(group extent * n_grid)        1 log2(relative number density)
+ *  2 scale factor a
+ *  3,5,7 extent/distance of the sibling of the group's node and of its
+ *        two next ancestors' siblings   4,6,8 log2 of their mass * n / cnt */
+int cosmo_group_features(const float *pos_x, const float *pos_y,
+                         const float *pos_z, const int32_t *children,
+                         const int32_t *parents, const double *n_mass,
+                         const double *n_com, const float *n_min,
+                         const float *n_max, int64_t n, double a,
+                         double n_grid, float *feat, void *stream);
+
+/* X[g*K + k][0..15] = BF16 [thetas[k], feat[g][0..8], 0...]. */
+int cosmo_assemble_rows(const float *feat, int64_t groups,
+                        const float *thetas, int K, uint16_t *X, void *stream);
+
+/* Y[r] = MLP(X[r]) for rows r < rows; X is BF16 [rows][16]. */
+int cosmo_mlp_forward(const uint16_t *X, int64_t rows, const uint16_t *wbf,
+                      const float *params, float *Y, void *stream);
+
+/* Gradient of mean((Y - T)^2) over `rows` samples: grad[COSMO_MLP_PARAMS]
+ * plus grad[COSMO_MLP_PARAMS] = sum of squared errors.  partial is device
+ * scratch of nblocks * (COSMO_MLP_PARAMS + 1) floats; blocks are reduced
+ * in a fixed order. */
+int cosmo_mlp_grad(const uint16_t *X, const float *T, int64_t rows,
+                   const uint16_t *wbf, const float *params, float *partial,
+                   int nblocks, float *grad, void *stream);
+
+/* One Adam step (bias-corrected, step >= 1) on the FP32 parameters; also
+ * refreshes the BF16 weights. */
+int cosmo_mlp_adam(float *params, float *m, float *v, const float *grad,
+                   float lr, float beta1, float beta2, float eps, int step,
+                   uint16_t *wbf, void *stream);
+
+/* BF16 copy of the weights of an FP32 parameter vector. */
+int cosmo_mlp_to_bf16(const float *params, uint16_t *wbf, void *stream);
+
+/* theta_group[g] = thetas[argmin_k cost[k] + lambda * 2^pred[g*K+k]]; a
+ * group with any non-finite prediction gets `fallback` and increments
+ * *n_fallback (nullable, device uint64). */
+int cosmo_select_theta(const float *pred, int64_t groups, const float *thetas,
+                       const float *cost, int K, double lambda, double fallback,
+                       float *theta_group, uint64_t *n_fallback, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
