@@ -21,12 +21,27 @@
 namespace cosmo {
 
 struct TravParams {
+  double th2d;      // REF: theta^2 in FP64 (the reference's arithmetic)
   float th2;        // REF: theta^2
   float inv_theta;  // AABB: 1/theta
   float G;
   float eps2;
   float half_box;
 };
+
+// Unit roundoff of FP32.
+constexpr float kU = 5.9604645e-8f;
+
+// Minimum image of r = fl(c - p) when |r| == 0.5.  Only then can the FP32
+// difference and the exact one fall on different sides of 1/2 (rounding is
+// monotonic and 1/2 is representable), so only then is the image decided
+// from the exact difference: c - p > 1/2  <=>  c - 1/2 > p, where c - 1/2 is
+// exact for c in [1/4, 1] (Sterbenz), which holds whenever c - p ~ 1/2 with
+// p >= 0.  An exact tie keeps r, as rint(+-0.5) = 0 does.
+__device__ __forceinline__ float wrap_tie(float c, float p, float r) {
+  const float s = (c - 0.5f > p) ? 1.0f : ((p - 0.5f > c) ? -1.0f : 0.0f);
+  return r - s;
+}
 
 __device__ __forceinline__ void neumaier_add(float &s, float &c, float x) {
   const float t = s + x;
@@ -57,10 +72,12 @@ __global__ __launch_bounds__(256) void k_traverse(
     z = pz[i];
   }
   float th2 = p.th2, inv_theta = p.inv_theta;
+  double th2d = p.th2d;
   if (GROUP_THETA) {
     const float t = theta_group[group];
     th2 = t * t;
     inv_theta = 1.0f / t;
+    th2d = static_cast<double>(t) * static_cast<double>(t);
   }
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   float cx = 0.0f, cy = 0.0f, cz = 0.0f;
@@ -85,14 +102,41 @@ __global__ __launch_bounds__(256) void k_traverse(
     if (rec.m <= 0.0f) continue;
     bool open = false;
     if (in) {
-      float dx = rec.cx - x, dy = rec.cy - y, dz = rec.cz - z;
-      dx = dx - rintf(dx);
-      dy = dy - rintf(dy);
-      dz = dz - rintf(dz);
+      const float rx = rec.cx - x, ry = rec.cy - y, rz = rec.cz - z;
+      float dx = rx - rintf(rx), dy = ry - rintf(ry), dz = rz - rintf(rz);
+      if (fabsf(rx) == 0.5f || fabsf(ry) == 0.5f || fabsf(rz) == 0.5f) {
+        dx = (fabsf(rx) == 0.5f) ? wrap_tie(rec.cx, x, rx) : dx;
+        dy = (fabsf(ry) == 0.5f) ? wrap_tie(rec.cy, y, ry) : dy;
+        dz = (fabsf(rz) == 0.5f) ? wrap_tie(rec.cz, z, rz) : dz;
+      }
       const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-      bool accept;
+      bool accept, near_ok = d2 > 1e-7f;
       if (MAC == COSMO_MAC_REF) {
-        accept = rec.a < th2 * d2;
+        // The reference decides in FP64 (physics_kernels.py:122-136).  The
+        // FP32 test is final unless its rounding error, bounded by `band`
+        // (|d - d_exact| <= u|c - p| per axis, a few u on the products), could
+        // change a decision; such lanes redo the test in FP64 exactly as the
+        // reference does, so accepted sets equal the FP64 walk's.
+        const float sx2 = rec.a * rec.a;
+        const float t = th2 * d2;
+        accept = sx2 < t;
+        const float l1 = fabsf(dx) + fabsf(dy) + fabsf(dz);
+        const float band = __builtin_fmaf(2.0f * kU, sx2,
+                                          th2 * __builtin_fmaf(8.0f * kU, d2, 3.0f * kU * l1));
+        const bool unsure = (fabsf(sx2 - t) <= band) ||
+                            (fabsf(d2 - 1e-7f) <= __builtin_fmaf(4.0f * kU, l1, 1e-13f));
+        if (unsure) {
+          double ex = static_cast<double>(rec.cx) - static_cast<double>(x);
+          double ey = static_cast<double>(rec.cy) - static_cast<double>(y);
+          double ez = static_cast<double>(rec.cz) - static_cast<double>(z);
+          ex -= rint(ex);
+          ey -= rint(ey);
+          ez -= rint(ez);
+          const double e2 = ex * ex + ey * ey + ez * ez;
+          const double sxd = static_cast<double>(rec.a);
+          accept = (sxd * sxd) < (th2d * e2);
+          near_ok = e2 > 1e-7;
+        }
       } else {
         const float r = __builtin_fmaf(rec.a, inv_theta, rec.b);
         float dm = fabsf(dx);
@@ -101,7 +145,7 @@ __global__ __launch_bounds__(256) void k_traverse(
         accept = (r * r < d2) && (dm + rec.a < p.half_box);
       }
       if (rec.c1 == -1 || accept) {
-        if (d2 > 1e-7f) {
+        if (near_ok) {
           const float inv = __builtin_amdgcn_rsqf(d2 + p.eps2);
           const float f = (p.G * rec.m) * ((inv * inv) * inv);
           if (COMP) {
@@ -263,6 +307,7 @@ int cosmo_traverse(const float *px, const float *py, const float *pz, const cosm
   if (!px || !py || !pz || !nodes || !root_idx || !force) return COSMO_E_INVALID;
   TravParams p;
   const float thf = static_cast<float>(theta);
+  p.th2d = theta * theta;
   p.th2 = thf * thf;
   p.inv_theta = 1.0f / thf;
   p.G = static_cast<float>(G);

@@ -226,7 +226,7 @@ __global__ __launch_bounds__(256) void k_pack_nodes(
     // physics_kernels.py:129-130
     float sx = hi0 - lo0;
     sx = (sx > boxf * 0.5f) ? boxf : sx;
-    a = sx * sx;
+    a = sx;
     b = 0.0f;
   } else {
     const float e0 = hi0 - lo0, e1 = hi1 - lo1, e2 = hi2 - lo2;

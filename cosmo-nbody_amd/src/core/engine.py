@@ -14,12 +14,15 @@ reference are not part of this package.
 """
 from __future__ import annotations
 
+import numpy as np
 import torch
 
 from src.core.camera import Camera
 from src.core.octree import OctreeManager
+from src.kernels import _hip
 from src.kernels.physics_kernels import MAC_AABB, MAC_REF, integrate, traverse
 from src.utils.cosmology import get_hubble_param
+from src.utils.visualizer import get_mvp_matrix
 
 
 def mac_of(config) -> int:
@@ -75,3 +78,79 @@ class SimulationEngine:
             self.step()
             if capture:
                 self.camera.capture(self.steps_done - 1, *self.state.pos)
+
+    # ------------------------------------------------------------ HIP graphs
+    #
+    # The whole step (sort, tree, traversal, update, density splat) is
+    # captured once per ping-pong parity of the particle buffers.  Per-step
+    # scalars (kick-drift coefficients from H(a), dt and a; the orbiting MVP)
+    # are precomputed on the host for the next `steps` steps, exactly as
+    # step() would compute them, and a device step counter selects the row
+    # inside the graph, so a replay needs no host-to-device copy.
+
+    def _graph_body(self):
+        s, c = self.state, self.config
+        self._coefs.copy_(torch.index_select(self._coef_table, 0, self._step_dev).reshape(-1))
+        self._mvp.copy_(torch.index_select(self._mvp_table, 0, self._step_dev).reshape(-1))
+        self.build_tree()
+        self.compute_forces()
+        L = _hip.load()
+        P = [_hip.ptr(t, torch.float32, "pos") for t in s.pos]
+        V = [_hip.ptr(t, torch.float32, "vel") for t in s.vel]
+        _hip.check(L.cosmo_kick_drift_dev(*P, *V, _hip.ptr(s.force, torch.float32, "force"),
+                                          _hip.ptr(self._coefs, torch.float32, "coefs"), s.n,
+                                          _hip.stream_handle()), "kick_drift_dev")
+        grid = self.camera.grid_d
+        grid.zero_()
+        _hip.check(L.cosmo_render_density_dev(
+            *[_hip.ptr(t, torch.float32, "pos") for t in s.pos], _hip.ptr(grid, _hip.U64, "grid"),
+            _hip.ptr(self._mvp, torch.float32, "mvp"), self.camera.res, s.n,
+            self.camera.frac_bits, _hip.stream_handle()), "render_density_dev")
+        self._step_dev += 1
+
+    def capture_graphs(self, steps):
+        """Prepares graph replays for the next `steps` steps (density each step)."""
+        c, dev = self.config, self.state.pos[0].device
+        coefs, mvps, a = [], [], self.a
+        for k in range(steps):
+            H = get_hubble_param(a, c)
+            dt = self.da / (a * H) * 0.8
+            cd = np.float32(1.0 - H * dt)
+            ck = np.float32(dt / a)
+            coefs.append([cd, ck, ck])
+            mvps.append(get_mvp_matrix(self.steps_done + k, self.camera.box_center, c.BOX_SIZE,
+                                       c).reshape(-1))
+            a += self.da
+        self._coef_table = torch.tensor(np.array(coefs, np.float32), device=dev)
+        self._mvp_table = torch.tensor(np.array(mvps, np.float32), device=dev)
+        self._step_dev = torch.zeros(1, dtype=torch.int64, device=dev)
+        self._coefs = torch.zeros(3, dtype=torch.float32, device=dev)
+        self._mvp = torch.zeros(16, dtype=torch.float32, device=dev)
+        self._graph_steps = steps
+        self._graph_done = 0
+        st = self.state
+        _ = st.ctx  # lazily created scratch must exist before capture
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        self._graphs, self._views = [], []
+        for _parity in range(2):
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=side):
+                self._graph_body()
+            self._graphs.append(g)
+            self._views.append((st.pos, st.vel, st.mass, st._alt))
+        torch.cuda.current_stream().wait_stream(side)
+        self._step_dev.zero_()
+        self._parity = 0
+
+    def step_graph(self):
+        """One step by graph replay (after capture_graphs)."""
+        if self._graph_done >= self._graph_steps:
+            raise RuntimeError("captured step tables exhausted; call capture_graphs again")
+        self._graphs[self._parity].replay()
+        st = self.state
+        st.pos, st.vel, st.mass, st._alt = self._views[self._parity]
+        self._parity ^= 1
+        self._graph_done += 1
+        self.a += self.da
+        self.steps_done += 1

@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import time
 
 import numpy as np
 import torch
@@ -111,32 +112,40 @@ def gather_groups(pos, groups):
     return [p.index_select(0, it).contiguous() for p in pos], idx, valid
 
 
-def group_errors(engine, groups, thetas, theta_group=None):
-    """BH forces of the sampled groups at each theta (or one per-group theta
-    vector) against the FP64 direct sum.  Returns (err2[groups, K],
-    inter[K] per particle, norm = sum |a_direct|^2 over valid targets)."""
-    st, tm, c = engine.state, engine.tree_manager, engine.config
-    n = st.n
-    tpos, idx, valid = gather_groups(st.pos, groups)
-    T = tpos[0].shape[0]
-    tidx = torch.from_numpy(idx[valid]).to(st.pos[0].device)
-    a_ref = direct_sum(*st.pos, st.mass, tidx, c.G, c.SOFTENING)
-    vmask = torch.from_numpy(valid).to(a_ref.device)
-    norm = float((a_ref ** 2).sum().item())
-    force = torch.empty((T, 3), dtype=torch.float32, device=a_ref.device)
-    stats = torch.zeros(2, dtype=torch.int64, device=a_ref.device)
-    runs = [(th, None) for th in thetas] if theta_group is None else [(0.5, theta_group)]
-    err2, inter = [], []
-    for th, tg in runs:
-        stats.zero_()
-        traverse(*tpos, tm.nodes, tm.root_idx, force, th, c.G, c.SOFTENING, c.BOX_SIZE, mac=engine.mac,
-                 theta_group=tg, stats=stats, n_nodes=tm.n_nodes)
-        d = torch.zeros((T, 3), dtype=torch.float64, device=a_ref.device)
-        d[vmask] = force[vmask].double() - a_ref
-        e = (d ** 2).sum(1).reshape(len(groups), 64).sum(1)
-        err2.append(e)
-        inter.append(float(stats[0].item()) / T)
-    return torch.stack(err2, 1), np.array(inter), norm
+class GroupProbe:
+    """Sampled groups of the engine's current tree with their FP64 direct-sum
+    accelerations, for repeated error measurements."""
+
+    def __init__(self, engine, groups):
+        st, c = engine.state, engine.config
+        self.engine = engine
+        self.groups = np.asarray(groups)
+        self.tpos, idx, valid = gather_groups(st.pos, self.groups)
+        dev = st.pos[0].device
+        self.T = self.tpos[0].shape[0]
+        self.vmask = torch.from_numpy(valid).to(dev)
+        tidx = torch.from_numpy(idx[valid]).to(dev)
+        self.a_ref = direct_sum(*st.pos, st.mass, tidx, c.G, c.SOFTENING)
+        self.norm = float((self.a_ref ** 2).sum().item())
+        self.counts = np.minimum(64, st.n - self.groups * 64).astype(np.float64)
+        self._force = torch.empty((self.T, 3), dtype=torch.float32, device=dev)
+        self._stats = torch.zeros(2, dtype=torch.int64, device=dev)
+
+    def errors(self, theta=0.5, theta_group=None):
+        """(squared-error sum per group, interactions per target)."""
+        e, tm, c = self.engine, self.engine.tree_manager, self.engine.config
+        self._stats.zero_()
+        traverse(*self.tpos, tm.nodes, tm.root_idx, self._force, theta, c.G, c.SOFTENING,
+                 c.BOX_SIZE, mac=e.mac, theta_group=theta_group, stats=self._stats,
+                 n_nodes=tm.n_nodes)
+        d = torch.zeros((self.T, 3), dtype=torch.float64, device=self.a_ref.device)
+        d[self.vmask] = self._force[self.vmask].double() - self.a_ref
+        err2 = (d ** 2).sum(1).reshape(len(self.groups), 64).sum(1)
+        return err2, float(self._stats[0].item()) / float(self.counts.sum())
+
+    def rel_rms(self, theta=0.5, theta_group=None):
+        err2, inter = self.errors(theta, theta_group)
+        return math.sqrt(float(err2.sum().item()) / self.norm), inter
 
 
 def sample(engine, n_groups, thetas=DEFAULT_THETAS, seed=0):
@@ -148,12 +157,17 @@ def sample(engine, n_groups, thetas=DEFAULT_THETAS, seed=0):
     feat = K.group_features(st.pos, tm, engine.a, c.N_GRID)
     th = torch.tensor(thetas, dtype=torch.float32, device=feat.device)
     X = K.assemble_rows(feat[torch.from_numpy(groups).to(feat.device)].contiguous(), th)
-    err2, cost, norm = group_errors(engine, groups, thetas)
-    counts = np.minimum(64, st.n - groups * 64).astype(np.float64)
-    mean_norm = norm / counts.sum()
-    cnt = torch.from_numpy(counts).to(err2.device)[:, None]
+    probe = GroupProbe(engine, groups)
+    cols, cost = [], []
+    for t in thetas:
+        e2, inter = probe.errors(t)
+        cols.append(e2)
+        cost.append(inter)
+    err2 = torch.stack(cols, 1)
+    mean_norm = probe.norm / probe.counts.sum()
+    cnt = torch.from_numpy(probe.counts).to(err2.device)[:, None]
     T = torch.log2(err2 / cnt / mean_norm + 1e-30).float().reshape(-1).contiguous()
-    return Samples(X, T, err2, cost, norm, groups)
+    return Samples(X, T, err2, np.array(cost), probe.norm, groups)
 
 
 class LearnedTheta:
@@ -202,3 +216,147 @@ class LearnedTheta:
         d = np.load(path, allow_pickle=False)
         model = ThetaModel(torch.from_numpy(d["params"]), device=device)
         return cls(model, d["thetas"].tolist(), d["cost"], float(d["lam"]), float(d["fallback"]))
+
+
+def _time_traversal(engine, theta, theta_group, reps=5):
+    """Average HIP-event time (ms) of the full traversal (all particles)."""
+    st, tm, c = engine.state, engine.tree_manager, engine.config
+    traverse(*st.pos, tm.nodes, tm.root_idx, st.force, theta, c.G, c.SOFTENING, c.BOX_SIZE,
+             mac=engine.mac, theta_group=theta_group)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        traverse(*st.pos, tm.nodes, tm.root_idx, st.force, theta, c.G, c.SOFTENING, c.BOX_SIZE,
+                 mac=engine.mac, theta_group=theta_group)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def _bisect(f, lo, hi, target, iters=20, log=False, increasing=True):
+    """x in [lo, hi] with f(x) ~ target for a monotone f (returns the side
+    that meets the target: f(x) <= target)."""
+    best = None
+    for _ in range(iters):
+        mid = math.sqrt(lo * hi) if log else 0.5 * (lo + hi)
+        v = f(mid)
+        ok = v <= target
+        if ok:
+            best = mid
+        if ok == increasing:
+            lo = mid
+        else:
+            hi = mid
+    return best
+
+
+def experiment(preset="repo_128", target=1e-3, mac="aabb", train_steps=(0, 10, 20),
+               val_step=15, n_groups=2048, n_val_groups=1024, fit_steps=3000, seed=0,
+               thetas=DEFAULT_THETAS, log=print):
+    """Trains the model on the GPU from samples the method generates, then
+    compares fixed theta and learned per-group theta at equal relative RMS
+    error (FP64 direct sum on held-out groups) by traversal time."""
+    from src.core.engine import SimulationEngine
+    from src.core.state import SimulationState
+    from src.utils import config as cfg
+    from src.utils.cosmology import generate_zeldovich_ics
+    c = cfg.preset(preset, MAC=mac)
+    pos, vel = generate_zeldovich_ics(c)
+    eng = SimulationEngine(c, SimulationState(len(pos[0]), pos, vel))
+    Xs, Ts, costs = [], [], []
+    snaps = sorted(set(train_steps) | {val_step})
+    val = None
+    for sidx in snaps:
+        while eng.steps_done < sidx:
+            eng.step()
+        eng.build_tree()
+        if sidx in train_steps:
+            smp = sample(eng, n_groups, thetas, seed=seed + sidx)
+            Xs.append(smp.X)
+            Ts.append(smp.T)
+            costs.append(smp.cost)
+            log(f"samples at step {sidx}: {smp.X.shape[0]} rows, cost {np.round(smp.cost, 1)}")
+        if sidx == val_step:
+            break
+    X = torch.cat(Xs)
+    T = torch.cat(Ts)
+    cost = np.mean(np.stack(costs), 0)
+    model = ThetaModel(seed=seed)
+    t0 = time.perf_counter()
+    hist = model.fit(X, T, steps=fit_steps, log=max(1, fit_steps // 10))
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t0
+    log(f"fit {fit_steps} steps in {t_fit:.1f}s: loss {hist[0][1]:.3f} -> {hist[-1][1]:.3f}")
+    # held-out snapshot: eng is at val_step with its tree built
+    G = (eng.state.n + 63) // 64
+    rng = np.random.default_rng(seed + 12345)
+    vgroups = np.sort(rng.choice(G, size=min(n_val_groups, G), replace=False))
+    probe = GroupProbe(eng, vgroups)
+    fixed_curve = [(t, *probe.rel_rms(t)) for t in thetas]
+    log("fixed theta curve (theta, rel_rms, interactions): " +
+        ", ".join(f"({t:.2f}, {e:.2e}, {i:.0f})" for t, e, i in fixed_curve))
+    theta_star = _bisect(lambda t: probe.rel_rms(t)[0], min(thetas) * 0.5, max(thetas), target,
+                         iters=18)
+    if theta_star is None:
+        return {"reachable": False, "target": target, "fixed_curve": fixed_curve}
+    e_fixed, i_fixed = probe.rel_rms(theta_star)
+    lt = LearnedTheta(model, thetas, cost, 1.0, theta_star)
+    vg = torch.from_numpy(vgroups).to(eng.state.pos[0].device)
+
+    def learned_err(lam):
+        tg = lt.theta_groups(eng, lam)
+        return probe.rel_rms(0.5, tg.index_select(0, vg).contiguous())[0]
+
+    lam = _bisect(learned_err, 1e-2, 1e12, target, iters=40, log=True, increasing=False)
+    if lam is None:
+        return {"reachable": False, "target": target, "theta_fixed": theta_star}
+    lt.lam = lam
+    tg = lt.theta_groups(eng).clone()
+    e_learned, i_learned = probe.rel_rms(0.5, tg.index_select(0, vg).contiguous())
+    t_fixed = _time_traversal(eng, theta_star, None)
+    t_learned = _time_traversal(eng, 0.5, tg)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        lt.theta_groups(eng)
+    e1.record()
+    torch.cuda.synchronize()
+    t_select = e0.elapsed_time(e1) / 5
+    res = {"reachable": True, "preset": preset, "mac": mac, "target": target,
+           "n": eng.state.n, "val_step": val_step, "train_rows": int(X.shape[0]),
+           "fit_s": t_fit, "loss_first": hist[0][1], "loss_last": hist[-1][1],
+           "theta_fixed": theta_star, "err_fixed": e_fixed, "inter_fixed": i_fixed,
+           "lambda": lam, "err_learned": e_learned, "inter_learned": i_learned,
+           "traversal_ms_fixed": t_fixed, "traversal_ms_learned": t_learned,
+           "select_ms": t_select,
+           "speedup_traversal": t_fixed / t_learned,
+           "speedup_with_select": t_fixed / (t_learned + t_select),
+           "theta_group_mean": float(tg.mean().item()),
+           "fallbacks": lt.n_fallback}
+    return res, lt
+
+
+def main():
+    import argparse
+    import json
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--preset", default="repo_128")
+    ap.add_argument("--mac", default="aabb")
+    ap.add_argument("--target", type=float, default=1e-3)
+    ap.add_argument("--fit-steps", type=int, default=3000)
+    ap.add_argument("--groups", type=int, default=2048)
+    ap.add_argument("--save", default="")
+    args = ap.parse_args()
+    out = experiment(args.preset, args.target, args.mac, fit_steps=args.fit_steps,
+                     n_groups=args.groups, log=lambda m: print(m, flush=True))
+    if isinstance(out, tuple):
+        res, lt = out
+        if args.save:
+            lt.save(args.save)
+    else:
+        res = out
+    print(json.dumps(res, default=float), flush=True)
+
+
+if __name__ == "__main__":
+    main()

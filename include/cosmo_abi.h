@@ -56,7 +56,7 @@ extern "C" {
 #define COSMO_RENDER_FRAC_BITS 20
 
 /* Packed FP32 node record read by the traversal (32 bytes).
- * REF:  a = sx*sx (x-extent, clamped to box when > box/2), b = 0.
+ * REF:  a = sx (x-extent, replaced by box when > box/2), b = 0.
  * AABB: a = largest AABB extent s, b = |CoM - AABB centre|.
  * c1 == -1 marks a leaf. */
 typedef struct cosmo_node {
@@ -189,6 +189,14 @@ int cosmo_kick_drift(float *pos_x, float *pos_y, float *pos_z, float *vel_x,
                      float c_drag, float c_kick, float c_drift, int64_t n,
                      void *stream);
 
+/* cosmo_kick_drift with the three coefficients read from device memory
+ * (coefs[0..2] = c_drag, c_kick, c_drift), so that a captured HIP graph can
+ * be replayed with new coefficients written before each launch. */
+int cosmo_kick_drift_dev(float *pos_x, float *pos_y, float *pos_z,
+                         float *vel_x, float *vel_y, float *vel_z,
+                         const float *force, const float *coefs, int64_t n,
+                         void *stream);
+
 /* src/kernels/render_kernels.py:12-47 render_density(pos_x, pos_y, pos_z,
  * grid, mvp, res) into fixed-point bins: grid[res][res] uint64 gains
  * rint(2.5/w * 2^frac_bits) per visible particle.  mvp: host float[16],
@@ -196,6 +204,13 @@ int cosmo_kick_drift(float *pos_x, float *pos_y, float *pos_z, float *vel_x,
 int cosmo_render_density(const float *pos_x, const float *pos_y,
                          const float *pos_z, uint64_t *grid, const float *mvp,
                          int32_t res, int64_t n, int frac_bits, void *stream);
+
+/* cosmo_render_density with the MVP matrix read from device memory
+ * (mvp_dev: float[16], row-major), for captured HIP graphs. */
+int cosmo_render_density_dev(const float *pos_x, const float *pos_y,
+                             const float *pos_z, uint64_t *grid,
+                             const float *mvp_dev, int32_t res, int64_t n,
+                             int frac_bits, void *stream);
 
 /* FP64 direct sum (no reference counterpart; the error yardstick): for each
  * target t, acc[t][k] = sum_j G m_j (d^2+eps^2)^-3/2 d_k over all n sources

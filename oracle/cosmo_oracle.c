@@ -395,6 +395,19 @@ static void bh_one_f64(const float *px, const float *py, const float *pz,
   if (n_vis) *n_vis = nv;
 }
 
+/* Minimum image of c - p in FP32 with the image chosen from the exact
+ * difference: r = fl(c - p); when |r| == 0.5 the exact difference may lie on
+ * the other side of 1/2, so the shift comes from c - 1/2 > p (exact for c in
+ * [1/4, 1]); otherwise r - rint(r).  Equals the FP64 image choice. */
+static inline float min_image_f32(float c, float p) {
+  const float r = c - p;
+  if (fabsf(r) == 0.5f) {
+    const float s = (c - 0.5f > p) ? 1.0f : ((p - 0.5f > c) ? -1.0f : 0.0f);
+    return r - s;
+  }
+  return r - rintf(r);
+}
+
 /* Neumaier's compensated summation (Neumaier 1974): s + c carries the sum
  * of all added terms with the rounding error of each addition kept in c. */
 static inline void neumaier_add(float *s, float *c, float x) {
@@ -409,17 +422,19 @@ static inline void neumaier_add(float *s, float *c, float x) {
 /* The FP32 arithmetic of the HIP traversal.  Per node (computed once when
  * the node records are packed): REF sx2 = sx*sx; AABB s = max extent and
  * dl = sqrt(fma(oz,oz,fma(oy,oy,ox*ox))) with o = CoM - 0.5*(hi+lo).
- * Per visit: d = CoM - p minus rint; d2 = fma(dz,dz,fma(dy,dy,dx*dx));
- * REF: sx2 < (theta*theta)*d2;  AABB: r = fma(s, 1/theta, dl), r*r < d2 and
+ * Per visit: d = CoM - p with min_image_f32; d2 = fma(dz,dz,fma(dy,dy,dx*dx));
+ * REF: acceptance and the d2 > 1e-7 cut decided in FP64 exactly as the
+ * reference does (the GPU takes the FP32 decision only where its error
+ * bound proves it equal);  AABB: r = fma(s, 1/theta, dl), r*r < d2 and
  * max|d_k| + s < box/2; force 1/sqrt(d2+eps2) (IEEE here, hardware rsq on
  * the GPU), f = (G*m)*((inv*inv)*inv), a = fma(f, d, a). */
 static void bh_one_f32(const float *px, const float *py, const float *pz,
                        const int32_t *children, const float *n_mass,
                        const float *n_com, const float *n_min,
-                       const float *n_max, int mac, float thf, float Gf,
-                       float softf, float boxf, int32_t root, int64_t i,
-                       int compensated, double out[3], int64_t *n_int,
-                       int64_t *n_vis) {
+                       const float *n_max, int mac, float thf, double th2d,
+                       float Gf, float softf, float boxf, int32_t root,
+                       int64_t i, int compensated, double out[3],
+                       int64_t *n_int, int64_t *n_vis) {
   const float x = px[i], y = py[i], z = pz[i];
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   float cx = 0.0f, cy = 0.0f, cz = 0.0f; /* Neumaier compensation terms */
@@ -427,7 +442,6 @@ static void bh_one_f32(const float *px, const float *py, const float *pz,
   int top = 0;
   int64_t ni = 0, nv = 0;
   stack[top++] = root;
-  const float th2 = thf * thf;
   const float inv_theta = 1.0f / thf;
   const float eps2 = softf * softf;
   const float half = boxf * 0.5f;
@@ -436,20 +450,26 @@ static void bh_one_f32(const float *px, const float *py, const float *pz,
     const float nm = n_mass[node];
     const int32_t c1 = children[2 * node], c2 = children[2 * node + 1];
     if (nm <= 0.0f) continue;
-    float dx = n_com[3 * node + 0] - x;
-    float dy = n_com[3 * node + 1] - y;
-    float dz = n_com[3 * node + 2] - z;
-    dx = dx - rintf(dx);
-    dy = dy - rintf(dy);
-    dz = dz - rintf(dz);
+    const float dx = min_image_f32(n_com[3 * node + 0], x);
+    const float dy = min_image_f32(n_com[3 * node + 1], y);
+    const float dz = min_image_f32(n_com[3 * node + 2], z);
     const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
     nv++;
-    int accept;
+    int accept, near_ok = d2 > 1e-7f;
     if (mac == ORC_MAC_REF) {
+      /* decisions in FP64 exactly as the reference (the GPU reaches the same
+       * decisions: FP32 where provably equal, FP64 otherwise) */
       float sx = n_max[3 * node] - n_min[3 * node];
       sx = (sx > half) ? boxf : sx;
-      const float sx2 = sx * sx;
-      accept = sx2 < th2 * d2;
+      double ex = (double)n_com[3 * node + 0] - (double)x;
+      double ey = (double)n_com[3 * node + 1] - (double)y;
+      double ez = (double)n_com[3 * node + 2] - (double)z;
+      ex -= nearbyint(ex);
+      ey -= nearbyint(ey);
+      ez -= nearbyint(ez);
+      const double e2 = ex * ex + ey * ey + ez * ez;
+      accept = ((double)sx * (double)sx) < (th2d * e2);
+      near_ok = e2 > 1e-7;
     } else {
       float s = 0.0f, o[3];
       for (int k = 0; k < 3; ++k) {
@@ -466,7 +486,7 @@ static void bh_one_f32(const float *px, const float *py, const float *pz,
       accept = (r * r < d2) && (dm + s < half);
     }
     if (c1 == -1 || accept) {
-      if (d2 > 1e-7f) {
+      if (near_ok) {
         const float r2 = d2 + eps2;
         const float inv = 1.0f / sqrtf(r2);
         const float f = (Gf * nm) * ((inv * inv) * inv);
@@ -513,9 +533,10 @@ void orc_compute_forces(const float *px, const float *py, const float *pz,
                  soft, box, root, i, a, &ni, &nv);
     } else {
       const float th = theta_g ? theta_g[i] : (float)theta;
+      const double th2d = theta_g ? (double)th * (double)th : theta * theta;
       bh_one_f32(px, py, pz, children, n_mass, n_com, n_min, n_max, mac, th,
-                 (float)G, (float)soft, (float)box, root, i, precision == 31,
-                 a, &ni, &nv);
+                 th2d, (float)G, (float)soft, (float)box, root, i,
+                 precision == 31, a, &ni, &nv);
     }
     force[3 * i + 0] = (float)a[0];
     force[3 * i + 1] = (float)a[1];
