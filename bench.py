@@ -84,7 +84,7 @@ def main():
     torch.cuda.set_device(0)
     st = SimulationState(n, pos, vel)
     eng = SimulationEngine(c, st)
-    eng.stats = torch.zeros(2, dtype=torch.int64, device="cuda")
+    eng.stats = torch.zeros(3, dtype=torch.int64, device="cuda")
 
     def one_step(ev=None):
         H, dt = eng.timestep()
@@ -117,7 +117,7 @@ def main():
     names = ["sort", "tree", "forces", "update", "splat"]
     stage = {nm: float(np.mean([evs[k][j].elapsed_time(evs[k][j + 1])
                                 for k in range(args.steps)])) for j, nm in enumerate(names)}
-    inter, visits = (int(x) for x in eng.stats.cpu().tolist())
+    inter, visits, pops = (int(x) for x in eng.stats.cpu().tolist())
     ms = elapsed * 1e3 / args.steps
     trav_s = stage["forces"] * 1e-3
     flops = FLOP_PER_INTERACTION * inter / args.steps
@@ -142,6 +142,8 @@ def main():
         "stages_ms": stage,
         "interactions_per_particle": inter / args.steps / n,
         "visits_per_particle": visits / args.steps / n,
+        "wave_pops_per_particle": pops / args.steps / n,
+        "traversal": "pairs (128 particles per wavefront)" if eng.pairs else "64 per wavefront",
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "kernel": "k_traverse (FP32 VALU; 20 flop per accepted interaction)"},

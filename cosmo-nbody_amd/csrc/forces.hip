@@ -52,7 +52,13 @@ __device__ __forceinline__ void neumaier_add(float &s, float &c, float x) {
   s = t;
 }
 
-template <int MAC, bool COMP, bool STATS, bool GROUP_THETA>
+// PPL particles per lane: a wavefront walks for 64*PPL Morton-consecutive
+// particles (lane l holds particles base + 64h + l, h < PPL).  Each stack
+// entry carries one 64-bit lane mask per half; the masks of each half are
+// nested along the stack, so the depth bound of 64 still holds.  With PPL = 2
+// one scalar node load serves twice the particles, at the price of a larger
+// union of visited nodes.
+template <int MAC, bool COMP, bool STATS, bool GROUP_THETA, int PPL>
 __global__ __launch_bounds__(256) void k_traverse(
     const float *__restrict__ px, const float *__restrict__ py,
     const float *__restrict__ pz, const cosmo_node *__restrict__ nodes,
@@ -60,54 +66,72 @@ __global__ __launch_bounds__(256) void k_traverse(
     const float *__restrict__ theta_group, int64_t lo, int64_t hi,
     unsigned long long *__restrict__ stats) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t group = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
-  const int64_t base = lo + group * kWave;
+  const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  const int64_t base = lo + wave * (kWave * PPL);
   if (base >= hi) return;  // wave-uniform
-  const int64_t i = base + lane;
-  const bool active = i < hi;
-  float x = 0.0f, y = 0.0f, z = 0.0f;
-  if (active) {
-    x = px[i];
-    y = py[i];
-    z = pz[i];
-  }
-  float th2 = p.th2, inv_theta = p.inv_theta;
-  double th2d = p.th2d;
-  if (GROUP_THETA) {
-    const float t = theta_group[group];
-    th2 = t * t;
-    inv_theta = 1.0f / t;
-    th2d = static_cast<double>(t) * static_cast<double>(t);
-  }
-  float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-  uint32_t n_int = 0, n_vis = 0;
-  int depth = active ? 1 : 0;
-
-  const uint64_t amask = __ballot(active);
+  float x[PPL], y[PPL], z[PPL], th2[PPL], inv_theta[PPL];
+  double th2d[PPL];
+  float ax[PPL], ay[PPL], az[PPL], cx[PPL], cy[PPL], cz[PPL];
+  int depth[PPL];
+  bool active[PPL];
+  uint32_t n_int = 0, n_vis = 0, n_pop = 0;
+  int stk_node = 0;
+  uint32_t stk_lo[PPL], stk_hi[PPL];
   const int root = root_idx[0];
-  int stk_node = (lane == 0) ? root : 0;
-  uint32_t stk_lo = (lane == 0) ? static_cast<uint32_t>(amask) : 0u;
-  uint32_t stk_hi = (lane == 0) ? static_cast<uint32_t>(amask >> 32) : 0u;
+#pragma unroll
+  for (int h = 0; h < PPL; ++h) {
+    const int64_t i = base + h * kWave + lane;
+    active[h] = i < hi;
+    x[h] = y[h] = z[h] = 0.0f;
+    if (active[h]) {
+      x[h] = px[i];
+      y[h] = py[i];
+      z[h] = pz[i];
+    }
+    th2[h] = p.th2;
+    inv_theta[h] = p.inv_theta;
+    th2d[h] = p.th2d;
+    if (GROUP_THETA) {
+      const int64_t g = wave * PPL + h;
+      const float t = (base + h * kWave < hi) ? theta_group[g] : 0.5f;
+      th2[h] = t * t;
+      inv_theta[h] = 1.0f / t;
+      th2d[h] = static_cast<double>(t) * static_cast<double>(t);
+    }
+    ax[h] = ay[h] = az[h] = cx[h] = cy[h] = cz[h] = 0.0f;
+    depth[h] = active[h] ? 1 : 0;
+    const uint64_t am = __ballot(active[h]);
+    stk_lo[h] = (lane == 0) ? static_cast<uint32_t>(am) : 0u;
+    stk_hi[h] = (lane == 0) ? static_cast<uint32_t>(am >> 32) : 0u;
+  }
+  if (lane == 0) stk_node = root;
   int top = 1;
 
   while (top > 0) {
     --top;
+    if (STATS) n_pop++;
     const int node = __builtin_amdgcn_readlane(stk_node, top);
-    const uint32_t mlo = __builtin_amdgcn_readlane(stk_lo, top);
-    const uint32_t mhi = __builtin_amdgcn_readlane(stk_hi, top);
-    const bool in = ((lane < 32 ? mlo >> lane : mhi >> (lane - 32)) & 1u) != 0;
-    depth -= in ? 1 : 0;
+    bool in[PPL];
+#pragma unroll
+    for (int h = 0; h < PPL; ++h) {
+      const uint32_t mlo = __builtin_amdgcn_readlane(stk_lo[h], top);
+      const uint32_t mhi = __builtin_amdgcn_readlane(stk_hi[h], top);
+      in[h] = ((lane < 32 ? mlo >> lane : mhi >> (lane - 32)) & 1u) != 0;
+      depth[h] -= in[h] ? 1 : 0;
+    }
     const cosmo_node rec = nodes[node];
     if (rec.m <= 0.0f) continue;
-    bool open = false;
-    if (in) {
-      const float rx = rec.cx - x, ry = rec.cy - y, rz = rec.cz - z;
+    bool open[PPL];
+#pragma unroll
+    for (int h = 0; h < PPL; ++h) {
+      open[h] = false;
+      if (!in[h]) continue;
+      const float rx = rec.cx - x[h], ry = rec.cy - y[h], rz = rec.cz - z[h];
       float dx = rx - rintf(rx), dy = ry - rintf(ry), dz = rz - rintf(rz);
       if (fabsf(rx) == 0.5f || fabsf(ry) == 0.5f || fabsf(rz) == 0.5f) {
-        dx = (fabsf(rx) == 0.5f) ? wrap_tie(rec.cx, x, rx) : dx;
-        dy = (fabsf(ry) == 0.5f) ? wrap_tie(rec.cy, y, ry) : dy;
-        dz = (fabsf(rz) == 0.5f) ? wrap_tie(rec.cz, z, rz) : dz;
+        dx = (fabsf(rx) == 0.5f) ? wrap_tie(rec.cx, x[h], rx) : dx;
+        dy = (fabsf(ry) == 0.5f) ? wrap_tie(rec.cy, y[h], ry) : dy;
+        dz = (fabsf(rz) == 0.5f) ? wrap_tie(rec.cz, z[h], rz) : dz;
       }
       const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
       bool accept, near_ok = d2 > 1e-7f;
@@ -118,27 +142,27 @@ __global__ __launch_bounds__(256) void k_traverse(
         // change a decision; such lanes redo the test in FP64 exactly as the
         // reference does, so accepted sets equal the FP64 walk's.
         const float sx2 = rec.a * rec.a;
-        const float t = th2 * d2;
+        const float t = th2[h] * d2;
         accept = sx2 < t;
         const float l1 = fabsf(dx) + fabsf(dy) + fabsf(dz);
         const float band = __builtin_fmaf(2.0f * kU, sx2,
-                                          th2 * __builtin_fmaf(8.0f * kU, d2, 3.0f * kU * l1));
+                                          th2[h] * __builtin_fmaf(8.0f * kU, d2, 3.0f * kU * l1));
         const bool unsure = (fabsf(sx2 - t) <= band) ||
                             (fabsf(d2 - 1e-7f) <= __builtin_fmaf(4.0f * kU, l1, 1e-13f));
         if (unsure) {
-          double ex = static_cast<double>(rec.cx) - static_cast<double>(x);
-          double ey = static_cast<double>(rec.cy) - static_cast<double>(y);
-          double ez = static_cast<double>(rec.cz) - static_cast<double>(z);
+          double ex = static_cast<double>(rec.cx) - static_cast<double>(x[h]);
+          double ey = static_cast<double>(rec.cy) - static_cast<double>(y[h]);
+          double ez = static_cast<double>(rec.cz) - static_cast<double>(z[h]);
           ex -= rint(ex);
           ey -= rint(ey);
           ez -= rint(ez);
           const double e2 = ex * ex + ey * ey + ez * ez;
           const double sxd = static_cast<double>(rec.a);
-          accept = (sxd * sxd) < (th2d * e2);
+          accept = (sxd * sxd) < (th2d[h] * e2);
           near_ok = e2 > 1e-7;
         }
       } else {
-        const float r = __builtin_fmaf(rec.a, inv_theta, rec.b);
+        const float r = __builtin_fmaf(rec.a, inv_theta[h], rec.b);
         float dm = fabsf(dx);
         dm = fabsf(dy) > dm ? fabsf(dy) : dm;
         dm = fabsf(dz) > dm ? fabsf(dz) : dm;
@@ -149,46 +173,66 @@ __global__ __launch_bounds__(256) void k_traverse(
           const float inv = __builtin_amdgcn_rsqf(d2 + p.eps2);
           const float f = (p.G * rec.m) * ((inv * inv) * inv);
           if (COMP) {
-            neumaier_add(ax, cx, f * dx);
-            neumaier_add(ay, cy, f * dy);
-            neumaier_add(az, cz, f * dz);
+            neumaier_add(ax[h], cx[h], f * dx);
+            neumaier_add(ay[h], cy[h], f * dy);
+            neumaier_add(az[h], cz[h], f * dz);
           } else {
-            ax = __builtin_fmaf(f, dx, ax);
-            ay = __builtin_fmaf(f, dy, ay);
-            az = __builtin_fmaf(f, dz, az);
+            ax[h] = __builtin_fmaf(f, dx, ax[h]);
+            ay[h] = __builtin_fmaf(f, dy, ay[h]);
+            az[h] = __builtin_fmaf(f, dz, az[h]);
           }
           if (STATS) n_int++;
         }
       } else {
-        open = depth + 1 < 64;
+        open[h] = depth[h] + 1 < 64;
       }
       if (STATS) n_vis++;
     }
-    const uint64_t om = __ballot(open);
-    if (om) {
-      const uint32_t olo = static_cast<uint32_t>(om), ohi = static_cast<uint32_t>(om >> 32);
-      if (open) depth += (rec.c1 != -1 ? 1 : 0) + (rec.c2 != -1 ? 1 : 0);
-      // push = write lane `top` of the three stack registers
+    uint32_t olo[PPL], ohi[PPL];
+    bool any = false;
+#pragma unroll
+    for (int h = 0; h < PPL; ++h) {
+      const uint64_t om = __ballot(open[h]);
+      olo[h] = static_cast<uint32_t>(om);
+      ohi[h] = static_cast<uint32_t>(om >> 32);
+      any = any || (om != 0);
+    }
+    if (any) {
+      const int np = (rec.c1 != -1 ? 1 : 0) + (rec.c2 != -1 ? 1 : 0);
+#pragma unroll
+      for (int h = 0; h < PPL; ++h)
+        if (open[h]) depth[h] += np;
+      // push = write lane `top` of the stack registers: c2, then c1
       if (rec.c1 != -1) {
         const bool here = (lane == top);
         stk_node = here ? rec.c2 : stk_node;
-        stk_lo = here ? olo : stk_lo;
-        stk_hi = here ? ohi : stk_hi;
+#pragma unroll
+        for (int h = 0; h < PPL; ++h) {
+          stk_lo[h] = here ? olo[h] : stk_lo[h];
+          stk_hi[h] = here ? ohi[h] : stk_hi[h];
+        }
         ++top;
       }
       if (rec.c2 != -1) {
         const bool here = (lane == top);
         stk_node = here ? rec.c1 : stk_node;
-        stk_lo = here ? olo : stk_lo;
-        stk_hi = here ? ohi : stk_hi;
+#pragma unroll
+        for (int h = 0; h < PPL; ++h) {
+          stk_lo[h] = here ? olo[h] : stk_lo[h];
+          stk_hi[h] = here ? ohi[h] : stk_hi[h];
+        }
         ++top;
       }
     }
   }
-  if (active) {
-    force[3 * i + 0] = ax + cx;
-    force[3 * i + 1] = ay + cy;
-    force[3 * i + 2] = az + cz;
+#pragma unroll
+  for (int h = 0; h < PPL; ++h) {
+    if (active[h]) {
+      const int64_t i = base + h * kWave + lane;
+      force[3 * i + 0] = ax[h] + cx[h];
+      force[3 * i + 1] = ay[h] + cy[h];
+      force[3 * i + 2] = az[h] + cz[h];
+    }
   }
   if (STATS) {
     uint32_t a = n_int, b = n_vis;
@@ -200,6 +244,7 @@ __global__ __launch_bounds__(256) void k_traverse(
     if (lane == 0) {
       atomicAdd(&stats[0], static_cast<unsigned long long>(a));
       atomicAdd(&stats[1], static_cast<unsigned long long>(b));
+      atomicAdd(&stats[2], static_cast<unsigned long long>(n_pop));
     }
   }
 }
@@ -257,32 +302,45 @@ __global__ __launch_bounds__(1024) void k_direct_sum(
   }
 }
 
-template <int MAC, bool COMP, bool STATS, bool GT>
-static void launch_traverse(unsigned grid, hipStream_t s, const float *px, const float *py,
-                            const float *pz, const cosmo_node *nodes, const int32_t *root_idx,
-                            float *force, TravParams p, const float *tg, int64_t lo, int64_t hi,
+template <int MAC, bool COMP, bool STATS, bool GT, int PPL>
+static void launch_traverse(hipStream_t s, const float *px, const float *py, const float *pz,
+                            const cosmo_node *nodes, const int32_t *root_idx, float *force,
+                            TravParams p, const float *tg, int64_t lo, int64_t hi,
                             uint64_t *stats) {
-  k_traverse<MAC, COMP, STATS, GT><<<grid, 256, 0, s>>>(
+  const int64_t waves = (hi - lo + kWave * PPL - 1) / (kWave * PPL);
+  const unsigned grid = blocks_for(waves * kWave, 256);
+  k_traverse<MAC, COMP, STATS, GT, PPL><<<grid, 256, 0, s>>>(
       px, py, pz, nodes, root_idx, force, p, tg, lo, hi,
       reinterpret_cast<unsigned long long *>(stats));
 }
 
+template <int MAC, bool COMP, bool STATS, bool GT>
+static void dispatch_ppl(bool pairs, hipStream_t s, const float *px, const float *py,
+                         const float *pz, const cosmo_node *nodes, const int32_t *root_idx,
+                         float *force, TravParams p, const float *tg, int64_t lo, int64_t hi,
+                         uint64_t *stats) {
+  if (pairs)
+    launch_traverse<MAC, COMP, STATS, GT, 2>(s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
+  else
+    launch_traverse<MAC, COMP, STATS, GT, 1>(s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
+}
+
 template <int MAC, bool COMP>
-static void dispatch_traverse2(bool stats_on, bool gt, unsigned grid, hipStream_t s,
+static void dispatch_traverse2(bool stats_on, bool gt, bool pairs, hipStream_t s,
                                const float *px, const float *py, const float *pz,
                                const cosmo_node *nodes, const int32_t *root_idx, float *force,
                                TravParams p, const float *tg, int64_t lo, int64_t hi,
                                uint64_t *stats) {
   if (stats_on) {
     if (gt)
-      launch_traverse<MAC, COMP, true, true>(grid, s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
+      dispatch_ppl<MAC, COMP, true, true>(pairs, s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
     else
-      launch_traverse<MAC, COMP, true, false>(grid, s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
+      dispatch_ppl<MAC, COMP, true, false>(pairs, s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
   } else {
     if (gt)
-      launch_traverse<MAC, COMP, false, true>(grid, s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
+      dispatch_ppl<MAC, COMP, false, true>(pairs, s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
     else
-      launch_traverse<MAC, COMP, false, false>(grid, s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
+      dispatch_ppl<MAC, COMP, false, false>(pairs, s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
   }
 }
 
@@ -314,21 +372,20 @@ int cosmo_traverse(const float *px, const float *py, const float *pz, const cosm
   const float sf = static_cast<float>(soft);
   p.eps2 = sf * sf;
   p.half_box = static_cast<float>(box) * 0.5f;
-  const int64_t groups = (hi - lo + kWave - 1) / kWave;
-  const unsigned grid = blocks_for(groups * kWave, 256);
   hipStream_t s = as_stream(stream);
   const bool comp = (flags & COSMO_TRAV_COMPENSATED) != 0;
+  const bool pairs = (flags & COSMO_TRAV_PAIRS) != 0;
   const bool gt = theta_group != nullptr;
   if (mac == COSMO_MAC_REF) {
     if (comp)
-      dispatch_traverse2<COSMO_MAC_REF, true>(stats_on, gt, grid, s, px, py, pz, nodes, root_idx, force, p, theta_group, lo, hi, stats);
+      dispatch_traverse2<COSMO_MAC_REF, true>(stats_on, gt, pairs, s, px, py, pz, nodes, root_idx, force, p, theta_group, lo, hi, stats);
     else
-      dispatch_traverse2<COSMO_MAC_REF, false>(stats_on, gt, grid, s, px, py, pz, nodes, root_idx, force, p, theta_group, lo, hi, stats);
+      dispatch_traverse2<COSMO_MAC_REF, false>(stats_on, gt, pairs, s, px, py, pz, nodes, root_idx, force, p, theta_group, lo, hi, stats);
   } else {
     if (comp)
-      dispatch_traverse2<COSMO_MAC_AABB, true>(stats_on, gt, grid, s, px, py, pz, nodes, root_idx, force, p, theta_group, lo, hi, stats);
+      dispatch_traverse2<COSMO_MAC_AABB, true>(stats_on, gt, pairs, s, px, py, pz, nodes, root_idx, force, p, theta_group, lo, hi, stats);
     else
-      dispatch_traverse2<COSMO_MAC_AABB, false>(stats_on, gt, grid, s, px, py, pz, nodes, root_idx, force, p, theta_group, lo, hi, stats);
+      dispatch_traverse2<COSMO_MAC_AABB, false>(stats_on, gt, pairs, s, px, py, pz, nodes, root_idx, force, p, theta_group, lo, hi, stats);
   }
   COSMO_CHECK_LAUNCH();
   return COSMO_OK;

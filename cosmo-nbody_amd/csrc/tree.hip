@@ -5,9 +5,16 @@
 // delta function including its equal-code tie-break (:55-56), so `children`
 // is bit-identical to the reference algorithm's for any input.  The upward
 // pass climbs from each leaf with an arrival counter per node (:164-191):
-// the second arrival forms the parent as left + right in FP64.  Arrival
-// counters are agent-scope acquire/release atomics, which make a child's
-// stores visible to the sibling's thread across the eight XCD L2 caches.
+// the second arrival forms the parent as left + right in FP64.  The eight
+// XCDs have separate L2 caches, so a child's moments must be visible to the
+// sibling's thread on another XCD.  Acquire/release at agent scope would do
+// it with a write-back and an invalidation of the whole L2 per climb step
+// (8 ms at 128^3); instead every moment store and load of the climb is a
+// relaxed agent-scope atomic (coherent at the device: written through and
+// read past the local L2), and each thread waits for its stores to complete
+// (s_waitcnt vmcnt(0)) before it bumps the arrival counter.  The second
+// arrival reads the sibling only after its counter update returned, i.e.
+// after the first arrival's stores completed.
 #include "common.h"
 
 namespace cosmo {
@@ -75,6 +82,41 @@ __global__ __launch_bounds__(256) void k_build_tree(const uint64_t *__restrict__
   parents[c1] = idx;
 }
 
+// Device-coherent (agent-scope) relaxed accesses for data exchanged between
+// threads of one kernel on different XCDs.
+template <typename T>
+__device__ __forceinline__ void st_dev(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_dev(const T *p) {
+  return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Wait until this thread's earlier memory operations have completed
+// (vmcnt(0), expcnt/lgkmcnt unconstrained); compiler fences keep the
+// surrounding accesses on their side.
+__device__ __forceinline__ void wait_stores() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+__device__ __forceinline__ void store_moments(double *n_mass, double *n_com, float *n_min,
+                                              float *n_max, int64_t k, double m, double c0,
+                                              double c1, double c2, float lo0, float lo1,
+                                              float lo2, float hi0, float hi1, float hi2) {
+  st_dev(&n_mass[k], m);
+  st_dev(&n_com[3 * k + 0], c0);
+  st_dev(&n_com[3 * k + 1], c1);
+  st_dev(&n_com[3 * k + 2], c2);
+  st_dev(&n_min[3 * k + 0], lo0);
+  st_dev(&n_min[3 * k + 1], lo1);
+  st_dev(&n_min[3 * k + 2], lo2);
+  st_dev(&n_max[3 * k + 0], hi0);
+  st_dev(&n_max[3 * k + 1], hi1);
+  st_dev(&n_max[3 * k + 2], hi2);
+}
+
 __device__ __forceinline__ float min_ref(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float max_ref(float a, float b) { return (b > a) ? b : a; }
 
@@ -96,30 +138,25 @@ __global__ __launch_bounds__(256) void k_multipoles(
   double c0 = static_cast<double>(x) * m, c1 = static_cast<double>(y) * m,
          c2 = static_cast<double>(z) * m;
   float lo0 = x, lo1 = y, lo2 = z, hi0 = x, hi1 = y, hi2 = z;
-  n_mass[i] = m;
-  n_com[3 * int64_t(i) + 0] = c0;
-  n_com[3 * int64_t(i) + 1] = c1;
-  n_com[3 * int64_t(i) + 2] = c2;
-  n_min[3 * int64_t(i) + 0] = lo0;
-  n_min[3 * int64_t(i) + 1] = lo1;
-  n_min[3 * int64_t(i) + 2] = lo2;
-  n_max[3 * int64_t(i) + 0] = hi0;
-  n_max[3 * int64_t(i) + 1] = hi1;
-  n_max[3 * int64_t(i) + 2] = hi2;
+  store_moments(n_mass, n_com, n_min, n_max, i, m, c0, c1, c2, lo0, lo1, lo2, hi0, hi1, hi2);
   int curr = i;
   for (;;) {
     const int p = parents[curr];
     if (p == -1) break;
-    const int old = __hip_atomic_fetch_add(&counts[p], 1, __ATOMIC_ACQ_REL,
+    wait_stores();
+    const int old = __hip_atomic_fetch_add(&counts[p], 1, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
     if (old == 0) break;
     const int2 ch = reinterpret_cast<const int2 *>(children)[p];
     const bool mine_left = (ch.x == curr);
     const int64_t sib = mine_left ? ch.y : ch.x;
-    const double sm = n_mass[sib];
-    const double s0 = n_com[3 * sib + 0], s1 = n_com[3 * sib + 1], s2 = n_com[3 * sib + 2];
-    const float sl0 = n_min[3 * sib + 0], sl1 = n_min[3 * sib + 1], sl2 = n_min[3 * sib + 2];
-    const float sh0 = n_max[3 * sib + 0], sh1 = n_max[3 * sib + 1], sh2 = n_max[3 * sib + 2];
+    const double sm = ld_dev(&n_mass[sib]);
+    const double s0 = ld_dev(&n_com[3 * sib + 0]), s1 = ld_dev(&n_com[3 * sib + 1]),
+                 s2 = ld_dev(&n_com[3 * sib + 2]);
+    const float sl0 = ld_dev(&n_min[3 * sib + 0]), sl1 = ld_dev(&n_min[3 * sib + 1]),
+                sl2 = ld_dev(&n_min[3 * sib + 2]);
+    const float sh0 = ld_dev(&n_max[3 * sib + 0]), sh1 = ld_dev(&n_max[3 * sib + 1]),
+                sh2 = ld_dev(&n_max[3 * sib + 2]);
     m = m + sm;
     c0 = c0 + s0;
     c1 = c1 + s1;
@@ -131,17 +168,7 @@ __global__ __launch_bounds__(256) void k_multipoles(
       lo0 = min_ref(sl0, lo0); lo1 = min_ref(sl1, lo1); lo2 = min_ref(sl2, lo2);
       hi0 = max_ref(sh0, hi0); hi1 = max_ref(sh1, hi1); hi2 = max_ref(sh2, hi2);
     }
-    const int64_t q = p;
-    n_mass[q] = m;
-    n_com[3 * q + 0] = c0;
-    n_com[3 * q + 1] = c1;
-    n_com[3 * q + 2] = c2;
-    n_min[3 * q + 0] = lo0;
-    n_min[3 * q + 1] = lo1;
-    n_min[3 * q + 2] = lo2;
-    n_max[3 * q + 0] = hi0;
-    n_max[3 * q + 1] = hi1;
-    n_max[3 * q + 2] = hi2;
+    store_moments(n_mass, n_com, n_min, n_max, p, m, c0, c1, c2, lo0, lo1, lo2, hi0, hi1, hi2);
     curr = p;
   }
 }

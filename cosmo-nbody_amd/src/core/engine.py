@@ -44,7 +44,8 @@ class SimulationEngine:
                                         device=device)
         self.mac = mac_of(config)
         self.theta_group = None     # per-group opening angles (learned theta), or None
-        self.stats = None           # int64[2] interaction / visit counters, or None
+        self.stats = None           # int64[3] interaction / visit / pop counters, or None
+        self.pairs = bool(getattr(config, "TRAVERSAL_PAIRS", True))
         self.steps_done = 0
 
     def timestep(self):
@@ -61,7 +62,8 @@ class SimulationEngine:
         s, c = self.state, self.config
         traverse(*s.pos, self.tree_manager.nodes, self.tree_manager.root_idx, s.force, c.THETA,
                  c.G, c.SOFTENING, c.BOX_SIZE, mac=self.mac, theta_group=self.theta_group,
-                 compensated=bool(getattr(c, "COMPENSATED", False)), stats=self.stats)
+                 compensated=bool(getattr(c, "COMPENSATED", False)), stats=self.stats,
+                 pairs=self.pairs)
 
     def step(self):
         H, dt = self.timestep()

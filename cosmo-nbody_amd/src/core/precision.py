@@ -48,7 +48,7 @@ def sweep(engine, thetas, n_targets=16384, seed=1234, macs=(MAC_REF, MAC_AABB),
     t_direct = time.perf_counter() - t0
     out = []
     force = torch.empty((len(idx), 3), dtype=torch.float32, device=ref.device)
-    stats = torch.zeros(2, dtype=torch.int64, device=ref.device)
+    stats = torch.zeros(3, dtype=torch.int64, device=ref.device)
     for mac in macs:
         pack_nodes(tm.child, tm.mass, tm.com, tm.aabb_min, tm.aabb_max, c.BOX_SIZE, mac, tm.nodes)
         for th in thetas:

@@ -27,6 +27,7 @@ MAC_REF = 0
 MAC_AABB = 1
 TRAV_COMPENSATED = 1
 TRAV_STATS = 2
+TRAV_PAIRS = 4
 RENDER_FRAC_BITS = 20
 NODE_BYTES = 32
 

@@ -16,8 +16,8 @@ from __future__ import annotations
 
 import torch
 
-from ._hip import (MAC_AABB, MAC_REF, TRAV_COMPENSATED, TRAV_STATS, check, kernel, load, ptr,
-                   shared_context, stream_handle)
+from ._hip import (MAC_AABB, MAC_REF, TRAV_COMPENSATED, TRAV_PAIRS, TRAV_STATS, check, kernel,
+                   load, ptr, shared_context, stream_handle)
 
 __all__ = ["compute_forces", "integrate", "kick_drift", "traverse", "direct_sum", "MAC_REF",
            "MAC_AABB"]
@@ -53,18 +53,21 @@ def compute_forces(pos_x, pos_y, pos_z, children, n_mass, n_com, n_min, n_max, f
 
 
 def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0, count=None,
-             mac=MAC_REF, theta_group=None, compensated=False, stats=None, n_nodes=None):
-    """Traversal over packed node records; stats: int64[2] tensor accumulating
-    (interactions, node visits) when given.  The targets (pos_*) need not be
-    the tree's particles: n_nodes gives the tree size when they differ."""
+             mac=MAC_REF, theta_group=None, compensated=False, stats=None, n_nodes=None,
+             pairs=False):
+    """Traversal over packed node records; stats: int64[3] tensor accumulating
+    (interactions, node visits, wavefront stack pops) when given.  The targets
+    (pos_*) need not be the tree's particles: n_nodes gives the tree size when
+    they differ.  pairs: two particles per lane (128-particle groups)."""
     n = pos_x.shape[0]
     n_nodes = 2 * n - 1 if n_nodes is None else int(n_nodes)
     count = n - off if count is None else count
-    flags = (TRAV_COMPENSATED if compensated else 0) | (TRAV_STATS if stats is not None else 0)
+    flags = ((TRAV_COMPENSATED if compensated else 0) | (TRAV_STATS if stats is not None else 0) |
+             (TRAV_PAIRS if pairs else 0))
     groups = (min(n, off + count) - off + 63) // 64
     tg = None if theta_group is None else ptr(theta_group, _F32, "theta_group",
                                               numel=max(groups, 0))
-    st = None if stats is None else ptr(stats, (torch.int64, torch.uint64), "stats", numel=2)
+    st = None if stats is None else ptr(stats, (torch.int64, torch.uint64), "stats", numel=3)
     check(load().cosmo_traverse(
         *_xyz(pos_x, pos_y, pos_z, n), ptr(nodes, _F32, "nodes", numel=8 * n_nodes),
         ptr(root, torch.int32, "root"), ptr(force, _F32, "force", numel=3 * n), float(theta), tg,

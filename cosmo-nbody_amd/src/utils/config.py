@@ -57,6 +57,7 @@ RUN_STEPS = N_STEPS
 RENDER_FRAC_BITS = 20      # fixed-point fraction bits of the density bins
 COMPENSATED = False        # Neumaier-compensated far-field accumulation
 MAC = "ref"                # acceptance test: "ref" (physics_kernels.py:128-133) or "aabb"
+TRAVERSAL_PAIRS = True     # two particles per lane in the traversal (same results)
 LEARNED_THETA = False
 
 
@@ -88,6 +89,7 @@ class Config:
     RENDER_FRAC_BITS: int = RENDER_FRAC_BITS
     COMPENSATED: bool = COMPENSATED
     MAC: str = MAC
+    TRAVERSAL_PAIRS: bool = TRAVERSAL_PAIRS
     LEARNED_THETA: bool = LEARNED_THETA
     NAME: str = "reference_defaults"
 

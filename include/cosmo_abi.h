@@ -51,6 +51,8 @@ extern "C" {
 /* traversal flags */
 #define COSMO_TRAV_COMPENSATED 1 /* Neumaier-compensated accumulation     */
 #define COSMO_TRAV_STATS 2       /* count interactions / node visits      */
+#define COSMO_TRAV_PAIRS 4       /* two particles per lane (128-particle
+                                    groups per wavefront)                 */
 
 /* Fixed-point fraction bits of the density bins used by the drop-ins. */
 #define COSMO_RENDER_FRAC_BITS 20
@@ -150,8 +152,9 @@ int cosmo_normalize_pack(const int32_t *children, const double *n_mass,
  * ∩ [0, n_bodies) in Morton-contiguous groups of 64 (one wavefront each),
  * writing force[i][0..2] (accelerations).  theta_group (nullable, device
  * float[ceil(count/64)]) gives each group's opening angle in place of
- * theta.  stats (nullable, device uint64[2]) accumulates interactions and
- * node visits when flags has COSMO_TRAV_STATS. */
+ * theta.  stats (nullable, device uint64[3]) accumulates interactions,
+ * node visits (per particle) and stack pops (per wavefront) when flags has
+ * COSMO_TRAV_STATS. */
 int cosmo_traverse(const float *pos_x, const float *pos_y, const float *pos_z,
                    const cosmo_node *nodes, const int32_t *root_idx,
                    float *force, double theta, const float *theta_group,

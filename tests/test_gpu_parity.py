@@ -253,7 +253,7 @@ def test_traverse_interaction_counts(orc, torch_mod, ics32, mac):
     nodes = pack_nodes(dev(ch, torch), dev(nm, torch), dev(com, torch), dev(lo, torch),
                        dev(hi, torch), 1.0, mac)
     force = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
-    stats = torch.zeros(2, dtype=torch.int64, device="cuda")
+    stats = torch.zeros(3, dtype=torch.int64, device="cuda")
     root = torch.tensor([n], dtype=torch.int32, device="cuda")
     traverse(*[dev(p, torch) for p in pos], nodes, root, force, 0.5, 1.0, soft, 1.0, mac=mac,
              stats=stats)
@@ -263,6 +263,36 @@ def test_traverse_interaction_counts(orc, torch_mod, ics32, mac):
     assert int(s[0]) == int(ni.sum())
     assert int(s[1]) == int(nv.sum())
     assert rel_rms(host(force), f32) <= 2e-6
+
+
+@pytest.mark.parametrize("mac", [0, 1])
+def test_traverse_pairs_identical(orc, torch_mod, ics32, mac):
+    """Two particles per lane (128-particle groups) gives the same forces bit
+    for bit and the same per-particle interaction and visit counts."""
+    torch = torch_mod
+    from src.kernels.physics_kernels import traverse
+    from src.kernels.tree_kernels import pack_nodes
+    pos, mass, ch, nm, com, lo, hi = _case(orc, ics32)
+    n = len(mass)
+    nodes = pack_nodes(dev(ch, torch), dev(nm, torch), dev(com, torch), dev(lo, torch),
+                       dev(hi, torch), 1.0, mac)
+    root = torch.tensor([n], dtype=torch.int32, device="cuda")
+    P = [dev(p, torch) for p in pos]
+    tg = torch.from_numpy(np.random.default_rng(4).uniform(0.3, 0.8, (n + 63) // 64)
+                          .astype(np.float32)).cuda()
+    for group_theta in (None, tg):
+        res = []
+        for pairs in (False, True):
+            f = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+            st = torch.zeros(3, dtype=torch.int64, device="cuda")
+            # an odd-sized slice exercises partial groups and half-empty waves
+            traverse(*P, nodes, root, f, 0.5, 1.0, 0.02 / 32, 1.0, off=37, count=n - 37 - 70,
+                     mac=mac, theta_group=None if group_theta is None else group_theta,
+                     stats=st, pairs=pairs)
+            res.append((host(f), host(st)))
+        assert np.array_equal(res[0][0], res[1][0])
+        assert np.array_equal(res[0][1][:2], res[1][1][:2])
+        assert res[1][1][2] < res[0][1][2]  # fewer wavefront pops with pairs
 
 
 def test_traverse_group_theta_and_compensated(orc, torch_mod, ics32):
@@ -281,7 +311,7 @@ def test_traverse_group_theta_and_compensated(orc, torch_mod, ics32):
     out = {}
     for comp in (False, True):
         force = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
-        stats = torch.zeros(2, dtype=torch.int64, device="cuda")
+        stats = torch.zeros(3, dtype=torch.int64, device="cuda")
         traverse(*P, nodes, root, force, 0.5, 1.0, soft, 1.0, mac=1, theta_group=dev(tg, torch),
                  compensated=comp, stats=stats)
         out[comp] = (host(force), host(stats))
