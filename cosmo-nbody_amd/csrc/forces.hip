@@ -43,6 +43,13 @@ __device__ __forceinline__ float wrap_tie(float c, float p, float r) {
   return r - s;
 }
 
+// Keeps the compiler from hoisting work on v out of a rarely taken branch
+// (an empty asm that claims to modify v; emits no instruction).
+__device__ __forceinline__ float pin(float v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 __device__ __forceinline__ void neumaier_add(float &s, float &c, float x) {
   const float t = s + x;
   if (fabsf(s) >= fabsf(x))
@@ -121,6 +128,7 @@ __global__ __launch_bounds__(256) void k_traverse(
     }
     const cosmo_node rec = nodes[node];
     if (rec.m <= 0.0f) continue;
+    const bool leaf = rec.c1 == -1;
     bool open[PPL];
 #pragma unroll
     for (int h = 0; h < PPL; ++h) {
@@ -128,48 +136,54 @@ __global__ __launch_bounds__(256) void k_traverse(
       if (!in[h]) continue;
       const float rx = rec.cx - x[h], ry = rec.cy - y[h], rz = rec.cz - z[h];
       float dx = rx - rintf(rx), dy = ry - rintf(ry), dz = rz - rintf(rz);
-      if (fabsf(rx) == 0.5f || fabsf(ry) == 0.5f || fabsf(rz) == 0.5f) {
-        dx = (fabsf(rx) == 0.5f) ? wrap_tie(rec.cx, x[h], rx) : dx;
-        dy = (fabsf(ry) == 0.5f) ? wrap_tie(rec.cy, y[h], ry) : dy;
-        dz = (fabsf(rz) == 0.5f) ? wrap_tie(rec.cz, z[h], rz) : dz;
-      }
       const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-      bool accept, near_ok = d2 > 1e-7f;
+      bool accept = true, near_ok = d2 > 1e-7f, unsure = false;
       if (MAC == COSMO_MAC_REF) {
         // The reference decides in FP64 (physics_kernels.py:122-136).  The
-        // FP32 test is final unless its rounding error, bounded by `band`
-        // (|d - d_exact| <= u|c - p| per axis, a few u on the products), could
-        // change a decision; such lanes redo the test in FP64 exactly as the
-        // reference does, so accepted sets equal the FP64 walk's.
-        const float sx2 = rec.a * rec.a;
-        const float t = th2[h] * d2;
-        accept = sx2 < t;
-        const float l1 = fabsf(dx) + fabsf(dy) + fabsf(dz);
-        const float band = __builtin_fmaf(2.0f * kU, sx2,
-                                          th2[h] * __builtin_fmaf(8.0f * kU, d2, 3.0f * kU * l1));
-        const bool unsure = (fabsf(sx2 - t) <= band) ||
-                            (fabsf(d2 - 1e-7f) <= __builtin_fmaf(4.0f * kU, l1, 1e-13f));
-        if (unsure) {
-          double ex = static_cast<double>(rec.cx) - static_cast<double>(x[h]);
-          double ey = static_cast<double>(rec.cy) - static_cast<double>(y[h]);
-          double ez = static_cast<double>(rec.cz) - static_cast<double>(z[h]);
+        // FP32 decision is final unless rounding could flip it: with
+        // |d_k - d_exact,k| <= u |c_k - p_k| <= u and a few u on the products,
+        // |error of sx^2 - theta^2 d^2| <= theta^2 u (10.1 d^2 + 3 |d|_1)
+        // <= 11.8 u theta^2 |d|_1 (|d| <= sqrt(3)/2), and |d^2 - d_exact^2|
+        // <= 2e-10 near the 1e-7 cut.  Lanes inside those bands redo both
+        // tests in FP64 exactly as the reference does.
+        unsure = fabsf(d2 - 1e-7f) <= 2e-10f;
+        if (!leaf) {
+          const float t = th2[h] * d2;
+          const float sx2 = rec.a * rec.a;
+          accept = sx2 < t;
+          const float l1 = fabsf(dx) + fabsf(dy) + fabsf(dz);
+          unsure = unsure || (fabsf(sx2 - t) <= (16.0f * kU) * th2[h] * l1);
+        }
+        if (__builtin_expect(unsure, 0)) {
+          double ex = static_cast<double>(pin(rec.cx)) - static_cast<double>(x[h]);
+          double ey = static_cast<double>(pin(rec.cy)) - static_cast<double>(y[h]);
+          double ez = static_cast<double>(pin(rec.cz)) - static_cast<double>(z[h]);
           ex -= rint(ex);
           ey -= rint(ey);
           ez -= rint(ez);
           const double e2 = ex * ex + ey * ey + ez * ez;
-          const double sxd = static_cast<double>(rec.a);
-          accept = (sxd * sxd) < (th2d[h] * e2);
+          if (!leaf) {
+            const double sxd = static_cast<double>(pin(rec.a));
+            accept = (sxd * sxd) < (th2d[h] * e2);
+          }
           near_ok = e2 > 1e-7;
         }
-      } else {
+      } else if (!leaf) {
         const float r = __builtin_fmaf(rec.a, inv_theta[h], rec.b);
         float dm = fabsf(dx);
         dm = fabsf(dy) > dm ? fabsf(dy) : dm;
         dm = fabsf(dz) > dm ? fabsf(dz) : dm;
         accept = (r * r < d2) && (dm + rec.a < p.half_box);
       }
-      if (rec.c1 == -1 || accept) {
+      if (accept) {
         if (near_ok) {
+          // A component with |c - p| rounding to exactly 1/2 may belong to
+          // the other image; d^2 is the same either way, the direction is not.
+          if (__builtin_expect(d2 >= 0.25f, 0)) {
+            dx = (fabsf(rx) == 0.5f) ? wrap_tie(pin(rec.cx), x[h], rx) : dx;
+            dy = (fabsf(ry) == 0.5f) ? wrap_tie(pin(rec.cy), y[h], ry) : dy;
+            dz = (fabsf(rz) == 0.5f) ? wrap_tie(pin(rec.cz), z[h], rz) : dz;
+          }
           const float inv = __builtin_amdgcn_rsqf(d2 + p.eps2);
           const float f = (p.G * rec.m) * ((inv * inv) * inv);
           if (COMP) {
