@@ -30,8 +30,8 @@ def parse():
     ap.add_argument("--preset", default="repo_128")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--no-pairs", action="store_true",
-                    help="64 particles per wavefront in the traversal instead of 128")
+    ap.add_argument("--pairs", action="store_true",
+                    help="128 particles per wavefront in the traversal instead of 64")
     return ap.parse_args()
 
 
@@ -80,7 +80,7 @@ def main():
     from src.utils import config as cfg
     from src.utils.cosmology import generate_zeldovich_ics
 
-    c = cfg.preset(args.preset, TRAVERSAL_PAIRS=not args.no_pairs)
+    c = cfg.preset(args.preset, TRAVERSAL_PAIRS=args.pairs)
     pos, vel = generate_zeldovich_ics(c)
     n = len(pos[0])
     torch.cuda.set_device(0)

@@ -45,7 +45,7 @@ class SimulationEngine:
         self.mac = mac_of(config)
         self.theta_group = None     # per-group opening angles (learned theta), or None
         self.stats = None           # int64[3] interaction / visit / pop counters, or None
-        self.pairs = bool(getattr(config, "TRAVERSAL_PAIRS", True))
+        self.pairs = bool(getattr(config, "TRAVERSAL_PAIRS", False))
         self.steps_done = 0
 
     def timestep(self):

@@ -251,11 +251,12 @@ def _bisect(f, lo, hi, target, iters=20, log=False, increasing=True):
 
 
 def experiment(preset="repo_128", target=1e-3, mac="aabb", train_steps=(0, 10, 20),
-               val_step=15, n_groups=2048, n_val_groups=1024, fit_steps=3000, seed=0,
-               thetas=DEFAULT_THETAS, log=print):
+               val_step=30, n_groups=2048, n_val_groups=1024, fit_steps=3000, seed=0,
+               thetas=DEFAULT_THETAS, target_theta=None, log=print):
     """Trains the model on the GPU from samples the method generates, then
     compares fixed theta and learned per-group theta at equal relative RMS
-    error (FP64 direct sum on held-out groups) by traversal time."""
+    error (FP64 direct sum on held-out groups) by traversal time.  With
+    target_theta the error target is the fixed-theta error at that theta."""
     from src.core.engine import SimulationEngine
     from src.core.state import SimulationState
     from src.utils import config as cfg
@@ -295,6 +296,9 @@ def experiment(preset="repo_128", target=1e-3, mac="aabb", train_steps=(0, 10, 2
     fixed_curve = [(t, *probe.rel_rms(t)) for t in thetas]
     log("fixed theta curve (theta, rel_rms, interactions): " +
         ", ".join(f"({t:.2f}, {e:.2e}, {i:.0f})" for t, e, i in fixed_curve))
+    if target_theta is not None:
+        target = probe.rel_rms(target_theta)[0]
+        log(f"target = error of fixed theta {target_theta}: {target:.4e}")
     theta_star = _bisect(lambda t: probe.rel_rms(t)[0], min(thetas) * 0.5, max(thetas), target,
                          iters=18)
     if theta_star is None:
@@ -345,10 +349,12 @@ def main():
     ap.add_argument("--target", type=float, default=1e-3)
     ap.add_argument("--fit-steps", type=int, default=3000)
     ap.add_argument("--groups", type=int, default=2048)
+    ap.add_argument("--target-theta", type=float, default=None)
     ap.add_argument("--save", default="")
     args = ap.parse_args()
     out = experiment(args.preset, args.target, args.mac, fit_steps=args.fit_steps,
-                     n_groups=args.groups, log=lambda m: print(m, flush=True))
+                     n_groups=args.groups, target_theta=args.target_theta,
+                     log=lambda m: print(m, flush=True))
     if isinstance(out, tuple):
         res, lt = out
         if args.save:

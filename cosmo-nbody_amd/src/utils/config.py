@@ -57,7 +57,7 @@ RUN_STEPS = N_STEPS
 RENDER_FRAC_BITS = 20      # fixed-point fraction bits of the density bins
 COMPENSATED = False        # Neumaier-compensated far-field accumulation
 MAC = "ref"                # acceptance test: "ref" (physics_kernels.py:128-133) or "aabb"
-TRAVERSAL_PAIRS = True     # two particles per lane in the traversal (same results)
+TRAVERSAL_PAIRS = False    # two particles per lane in the traversal (same results, slower at 128^3)
 LEARNED_THETA = False
 
 
