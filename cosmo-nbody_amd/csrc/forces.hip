@@ -169,11 +169,14 @@ __global__ __launch_bounds__(256) void k_traverse(
           near_ok = e2 > 1e-7;
         }
       } else if (!leaf) {
+        // (s/theta + delta)^2 < d^2, and every particle of the node sits on
+        // the same side of the minimum-image boundary as its CoM in each axis
         const float r = __builtin_fmaf(rec.a, inv_theta[h], rec.b);
-        float dm = fabsf(dx);
-        dm = fabsf(dy) > dm ? fabsf(dy) : dm;
-        dm = fabsf(dz) > dm ? fabsf(dz) : dm;
-        accept = (r * r < d2) && (dm + rec.a < p.half_box);
+        const float hb = p.half_box;
+        const bool inside = (dx + rec.hi[0] < hb) && (dx - rec.lo[0] > -hb) &&
+                            (dy + rec.hi[1] < hb) && (dy - rec.lo[1] > -hb) &&
+                            (dz + rec.hi[2] < hb) && (dz - rec.lo[2] > -hb);
+        accept = (r * r < d2) && inside;
       }
       if (accept) {
         if (near_ok) {

@@ -270,6 +270,10 @@ __global__ __launch_bounds__(256) void k_pack_nodes(
   float4 *dst = reinterpret_cast<float4 *>(out + k);
   dst[0] = make_float4(fx, fy, fz, static_cast<float>(nm));
   dst[1] = make_float4(a, b, __int_as_float(ch.x), __int_as_float(ch.y));
+  if (MAC == COSMO_MAC_AABB) {
+    dst[2] = make_float4(fx - lo0, fy - lo1, fz - lo2, hi0 - fx);
+    dst[3] = make_float4(hi1 - fy, hi2 - fz, 0.0f, 0.0f);
+  }
 }
 
 // ------------------------------------------------------------ validator

@@ -29,7 +29,7 @@ TRAV_COMPENSATED = 1
 TRAV_STATS = 2
 TRAV_PAIRS = 4
 RENDER_FRAC_BITS = 20
-NODE_BYTES = 32
+NODE_BYTES = 64
 
 _p = C.c_void_p
 _i64 = C.c_int64

@@ -69,7 +69,7 @@ def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0
                                               numel=max(groups, 0))
     st = None if stats is None else ptr(stats, (torch.int64, torch.uint64), "stats", numel=3)
     check(load().cosmo_traverse(
-        *_xyz(pos_x, pos_y, pos_z, n), ptr(nodes, _F32, "nodes", numel=8 * n_nodes),
+        *_xyz(pos_x, pos_y, pos_z, n), ptr(nodes, _F32, "nodes", numel=16 * n_nodes),
         ptr(root, torch.int32, "root"), ptr(force, _F32, "force", numel=3 * n), float(theta), tg,
         float(G), float(soft), float(box), int(off), int(count), n, int(mac), flags, st,
         stream_handle()), "traverse")

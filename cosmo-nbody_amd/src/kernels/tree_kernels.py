@@ -17,7 +17,7 @@ the reference stores float32).  build_tree and compute_multipoles need a
 launch that covers every node, as all of the reference's callers use.
 
 Additions without a reference counterpart: pack_nodes / normalize_pack (the
-32-byte FP32 records the traversal reads) and validate_tree.
+64-byte FP32 records the traversal reads) and validate_tree.
 """
 from __future__ import annotations
 
@@ -101,7 +101,7 @@ def build_top_tree_cache(child, root_idx, top_nodes, node_to_cache, cache_size, 
 
 
 def node_buffer(n_nodes: int, device=None) -> torch.Tensor:
-    """Uninitialised storage for n_nodes packed 32-byte node records."""
+    """Uninitialised storage for n_nodes packed 64-byte node records."""
     return torch.empty((n_nodes, NODE_BYTES // 4), dtype=torch.float32, device=device or "cuda")
 
 
@@ -112,7 +112,7 @@ def _pack(fn, children, n_mass, n_com, n_min, n_max, box, mac, nodes, what):
              ptr(n_com, torch.float64, "n_com", numel=3 * m),
              ptr(n_min, torch.float32, "n_min", numel=3 * m),
              ptr(n_max, torch.float32, "n_max", numel=3 * m), m, float(box), int(mac),
-             ptr(nodes, torch.float32, "nodes", numel=8 * m), stream_handle()), what)
+             ptr(nodes, torch.float32, "nodes", numel=16 * m), stream_handle()), what)
     return nodes
 
 

@@ -44,7 +44,8 @@ extern "C" {
 
 /* Acceptance tests of the traversal: REF = the reference's
  * (physics_kernels.py:128-133: x-extent of the AABB vs theta * distance to
- * the CoM); AABB = largest extent + CoM offset + minimum-image guard. */
+ * the CoM); AABB = largest extent + CoM offset, and the node's box must
+ * lie on one side of the minimum-image boundary in every axis. */
 #define COSMO_MAC_REF 0
 #define COSMO_MAC_AABB 1
 
@@ -57,14 +58,19 @@ extern "C" {
 /* Fixed-point fraction bits of the density bins used by the drop-ins. */
 #define COSMO_RENDER_FRAC_BITS 20
 
-/* Packed FP32 node record read by the traversal (32 bytes).
+/* Packed FP32 node record read by the traversal (64 bytes; the reference
+ * test reads only the first 32).
  * REF:  a = sx (x-extent, replaced by box when > box/2), b = 0.
- * AABB: a = largest AABB extent s, b = |CoM - AABB centre|.
+ * AABB: a = largest AABB extent s, b = |CoM - AABB centre|,
+ *       lo[k] = CoM_k - min_k, hi[k] = max_k - CoM_k (extent of the node's
+ *       box below / above its CoM, for the minimum-image guard).
  * c1 == -1 marks a leaf. */
 typedef struct cosmo_node {
   float cx, cy, cz, m;
   float a, b;
   int32_t c1, c2;
+  float lo[3], hi[3];
+  float pad[2];
 } cosmo_node;
 
 typedef struct cosmo_ctx cosmo_ctx;

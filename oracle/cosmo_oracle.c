@@ -321,9 +321,10 @@ int orc_validate_tree(const int32_t *children, const int32_t *parents,
  *   replaced by box when sx > box/2; accept iff sx^2 < theta^2 d^2, with d
  *   the minimum-image distance to the CoM.
  * ORC_MAC_AABB (no reference counterpart): s = largest extent of the tight
- *   AABB, dl = |CoM - AABB centre|; accept iff (s/theta + dl)^2 < d^2 and
- *   max_k |d_k| + s < box/2, so that no particle of an accepted node sits
- *   across the minimum-image boundary from its CoM.
+ *   AABB, dl = |CoM - AABB centre|; accept iff (s/theta + dl)^2 < d^2 and,
+ *   in every axis, d_k + (max_k - CoM_k) < box/2 and
+ *   d_k - (CoM_k - min_k) > -box/2, so that no particle of an accepted node
+ *   sits across the minimum-image boundary from its CoM.
  * Leaves are always accepted; interactions with d^2 <= 1e-7 are dropped. */
 
 /* physics_kernels.py:79-153, one particle, FP64 arithmetic on FP32 data
@@ -369,10 +370,15 @@ static void bh_one_f64(const float *px, const float *py, const float *pz,
         dl2 += off * off;
       }
       const double r_open = s * inv_theta + sqrt(dl2);
-      double dm = fabs(dx);
-      dm = fabs(dy) > dm ? fabs(dy) : dm;
-      dm = fabs(dz) > dm ? fabs(dz) : dm;
-      accept = (r_open * r_open < d2) && (dm + s < 0.5 * box);
+      const double dd[3] = {dx, dy, dz};
+      int inside = 1;
+      for (int k = 0; k < 3; ++k) {
+        const double c = n_com[3 * node + k];
+        const double up = dd[k] + ((double)n_max[3 * node + k] - c);
+        const double dn = dd[k] - (c - (double)n_min[3 * node + k]);
+        inside = inside && (up < 0.5 * box) && (dn > -0.5 * box);
+      }
+      accept = (r_open * r_open < d2) && inside;
     }
     if (c1 == -1 || accept) {
       if (d2 > 1e-7) {
@@ -425,8 +431,9 @@ static inline void neumaier_add(float *s, float *c, float x) {
  * Per visit: d = CoM - p with min_image_f32; d2 = fma(dz,dz,fma(dy,dy,dx*dx));
  * REF: acceptance and the d2 > 1e-7 cut decided in FP64 exactly as the
  * reference does (the GPU takes the FP32 decision only where its error
- * bound proves it equal);  AABB: r = fma(s, 1/theta, dl), r*r < d2 and
- * max|d_k| + s < box/2; force 1/sqrt(d2+eps2) (IEEE here, hardware rsq on
+ * bound proves it equal);  AABB: r = fma(s, 1/theta, dl), r*r < d2 and per
+ * axis d_k + (max_k - c_k) < box/2, d_k - (c_k - min_k) > -box/2 with FP32
+ * differences; force 1/sqrt(d2+eps2) (IEEE here, hardware rsq on
  * the GPU), f = (G*m)*((inv*inv)*inv), a = fma(f, d, a). */
 static void bh_one_f32(const float *px, const float *py, const float *pz,
                        const int32_t *children, const float *n_mass,
@@ -472,18 +479,19 @@ static void bh_one_f32(const float *px, const float *py, const float *pz,
       near_ok = e2 > 1e-7;
     } else {
       float s = 0.0f, o[3];
+      const float dd[3] = {dx, dy, dz};
+      int inside = 1;
       for (int k = 0; k < 3; ++k) {
         const float lo = n_min[3 * node + k], hi = n_max[3 * node + k];
+        const float c = n_com[3 * node + k];
         const float e = hi - lo;
         s = e > s ? e : s;
-        o[k] = n_com[3 * node + k] - 0.5f * (hi + lo);
+        o[k] = c - 0.5f * (hi + lo);
+        inside = inside && (dd[k] + (hi - c) < half) && (dd[k] - (c - lo) > -half);
       }
       const float dl = sqrtf(fmaf(o[2], o[2], fmaf(o[1], o[1], o[0] * o[0])));
       const float r = fmaf(s, inv_theta, dl);
-      float dm = fabsf(dx);
-      dm = fabsf(dy) > dm ? fabsf(dy) : dm;
-      dm = fabsf(dz) > dm ? fabsf(dz) : dm;
-      accept = (r * r < d2) && (dm + s < half);
+      accept = (r * r < d2) && inside;
     }
     if (c1 == -1 || accept) {
       if (near_ok) {
