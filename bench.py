@@ -32,6 +32,11 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--pairs", action="store_true",
                     help="128 particles per wavefront in the traversal instead of 64")
+    ap.add_argument("--kernel-probe", action="store_true",
+                    help="short run for hardware counters: 3 traversal launches on a warmed "
+                         "state plus a 256 MiB device copy (counter scale)")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay the whole step from HIP graphs (per-stage times not split)")
     return ap.parse_args()
 
 
@@ -107,25 +112,59 @@ def main():
     for _ in range(args.warmup):
         one_step()
     torch.cuda.synchronize()
-    eng.stats.zero_()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        one_step(evs[k])
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-
+    if args.kernel_probe:
+        eng.stats = None
+        eng.build_tree()
+        for _ in range(3):
+            eng.compute_forces()
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"probe": "traversal x3 + copy 256 MiB", "n": n}))
+        return
     names = ["sort", "tree", "forces", "update", "splat"]
-    stage = {nm: float(np.mean([evs[k][j].elapsed_time(evs[k][j + 1])
-                                for k in range(args.steps)])) for j, nm in enumerate(names)}
-    inter, visits, pops = (int(x) for x in eng.stats.cpu().tolist())
+    if args.graph:
+        eng.capture_graphs(args.steps)
+        torch.cuda.synchronize()
+        eng.stats.zero_()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            eng.step_graph()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        inter, visits, pops = (int(x) for x in eng.stats.cpu().tolist())
+        # traversal launch time on the final state, same stream, HIP events
+        eng.build_tree()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        saved = eng.stats
+        eng.stats = None
+        e0.record()
+        for _ in range(3):
+            eng.compute_forces()
+        e1.record()
+        torch.cuda.synchronize()
+        eng.stats = saved
+        stage = {"forces": e0.elapsed_time(e1) / 3}
+    else:
+        eng.stats.zero_()
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(6)]
+               for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            one_step(evs[k])
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        stage = {nm: float(np.mean([evs[k][j].elapsed_time(evs[k][j + 1])
+                                    for k in range(args.steps)])) for j, nm in enumerate(names)}
+        inter, visits, pops = (int(x) for x in eng.stats.cpu().tolist())
     ms = elapsed * 1e3 / args.steps
     trav_s = stage["forces"] * 1e-3
     flops = FLOP_PER_INTERACTION * inter / args.steps
     achieved = flops / trav_s / 1e12
     out = {
-        "metric": "ms/step + particles/s at N=128^3 (full step)",
+        "metric": f"ms/step + particles/s at N={c.N_GRID}^3 (full step)",
         "value": n * args.steps / elapsed,
         "unit": "particles/s",
         "n_gpus": 1,
@@ -140,7 +179,7 @@ def main():
         "config": {"workload": f"{c.NAME}: N={n} Zel'dovich ICs seed {c.SEED}, theta={c.THETA}, "
                                f"eps={c.SOFTENING:.4g}, MAC={c.MAC}, {c.RES}^2 splat, "
                                f"reference schedule da={c.DA:.3g}",
-                   "n_bodies": n, "theta": c.THETA, "res": c.RES},
+                   "n_bodies": n, "theta": c.THETA, "res": c.RES, "hip_graph": args.graph},
         "stages_ms": stage,
         "interactions_per_particle": inter / args.steps / n,
         "visits_per_particle": visits / args.steps / n,

@@ -28,8 +28,8 @@ import torch
 from src.kernels import mlp_kernels as K
 from src.kernels.physics_kernels import direct_sum, traverse
 
-DEFAULT_THETAS = (0.15, 0.2, 0.25, 0.3, 0.35, 0.4, 0.45, 0.5, 0.55, 0.6, 0.65, 0.7, 0.8, 0.9,
-                  1.0, 1.2)
+DEFAULT_THETAS = (0.04, 0.05, 0.06, 0.07, 0.08, 0.1, 0.12, 0.15, 0.2, 0.25, 0.3, 0.4, 0.5, 0.6,
+                  0.8, 1.0)
 
 
 def init_params(seed=0, device="cuda"):
