@@ -69,6 +69,6 @@ struct cosmo_ctx {
   uint32_t *vals_b;     // [n_max]
   uint32_t *hist;       // [kRadix * max_tiles]
   uint32_t *digit_tot;  // [kRadix]
-  cosmo_node *nodes;    // [2 n_max - 1]
+  void *nodes;          // packed records, cosmo_tree_bytes(n_max, AABB)
   int32_t *scratch_i32; // [4 n_max + 4] validator counters
 };

@@ -245,14 +245,13 @@ int cosmo_ctx_create(int64_t n_max, cosmo_ctx **out) {
   COSMO_CHECK_HIP(hipGetDevice(&c->device));
   c->n_max = n_max;
   c->max_tiles = (n_max + kSortTile - 1) / kSortTile;
-  const int64_t M = 2 * n_max - 1;
   hipError_t e = hipSuccess;
   if (e == hipSuccess) e = hipMalloc(&c->keys_alt, sizeof(uint64_t) * n_max);
   if (e == hipSuccess) e = hipMalloc(&c->vals_a, sizeof(uint32_t) * n_max);
   if (e == hipSuccess) e = hipMalloc(&c->vals_b, sizeof(uint32_t) * n_max);
   if (e == hipSuccess) e = hipMalloc(&c->hist, sizeof(uint32_t) * kRadix * c->max_tiles);
   if (e == hipSuccess) e = hipMalloc(&c->digit_tot, sizeof(uint32_t) * kRadix);
-  if (e == hipSuccess) e = hipMalloc(&c->nodes, sizeof(cosmo_node) * M);
+  if (e == hipSuccess) e = hipMalloc(&c->nodes, cosmo_tree_bytes(n_max, COSMO_MAC_AABB));
   if (e == hipSuccess) e = hipMalloc(&c->scratch_i32, sizeof(int32_t) * (4 * n_max + 4));
   if (e != hipSuccess) {
     set_hip_error(e);
@@ -276,7 +275,14 @@ int cosmo_ctx_destroy(cosmo_ctx *c) {
   return COSMO_OK;
 }
 
-cosmo_node *cosmo_ctx_nodes(cosmo_ctx *c) { return c ? c->nodes : nullptr; }
+void *cosmo_ctx_nodes(cosmo_ctx *c) { return c ? c->nodes : nullptr; }
+
+int64_t cosmo_tree_bytes(int64_t n, int mac) {
+  if (n < 1) return 0;
+  const int64_t inode = (mac == COSMO_MAC_AABB) ? int64_t(sizeof(cosmo_inode_aabb))
+                                                : int64_t(sizeof(cosmo_inode_ref));
+  return n * int64_t(sizeof(cosmo_leaf)) + (n - 1) * inode;
+}
 
 int cosmo_compute_codes(const float *px, const float *py, const float *pz,
                         uint64_t *codes, uint32_t *indices, int64_t n,

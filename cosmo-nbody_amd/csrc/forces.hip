@@ -59,19 +59,95 @@ __device__ __forceinline__ void neumaier_add(float &s, float &c, float x) {
   s = t;
 }
 
+template <int MAC>
+struct InodeOf;
+template <>
+struct InodeOf<COSMO_MAC_REF> {
+  typedef cosmo_inode_ref type;
+};
+template <>
+struct InodeOf<COSMO_MAC_AABB> {
+  typedef cosmo_inode_aabb type;
+};
+
+// Minimum-image displacement from a particle to a record's CoM (rint image,
+// which gives the right d^2 in every case; see fix_ties for the direction).
+struct Disp {
+  float rx, ry, rz, dx, dy, dz, d2;
+};
+
+__device__ __forceinline__ Disp displace(float cx, float cy, float cz, float x, float y,
+                                         float z) {
+  Disp d;
+  d.rx = cx - x;
+  d.ry = cy - y;
+  d.rz = cz - z;
+  d.dx = d.rx - rintf(d.rx);
+  d.dy = d.ry - rintf(d.ry);
+  d.dz = d.rz - rintf(d.rz);
+  d.d2 = __builtin_fmaf(d.dz, d.dz, __builtin_fmaf(d.dy, d.dy, d.dx * d.dx));
+  return d;
+}
+
+// FP64 restatement of the reference's displacement (physics_kernels.py:118-126).
+__device__ __forceinline__ double exact_d2(float cx, float cy, float cz, float x, float y,
+                                          float z) {
+  double ex = static_cast<double>(pin(cx)) - static_cast<double>(x);
+  double ey = static_cast<double>(pin(cy)) - static_cast<double>(y);
+  double ez = static_cast<double>(pin(cz)) - static_cast<double>(z);
+  ex -= rint(ex);
+  ey -= rint(ey);
+  ez -= rint(ez);
+  return ex * ex + ey * ey + ez * ez;
+}
+
+// A component with |c - p| rounding to exactly 1/2 may belong to the other
+// image: d^2 is the same either way, the direction is not.  Such components
+// only occur for d^2 >= 1/4.
+__device__ __forceinline__ void fix_ties(Disp &d, float cx, float cy, float cz, float x, float y,
+                                         float z) {
+  if (__builtin_expect(d.d2 >= 0.25f, 0)) {
+    d.dx = (fabsf(d.rx) == 0.5f) ? wrap_tie(pin(cx), x, d.rx) : d.dx;
+    d.dy = (fabsf(d.ry) == 0.5f) ? wrap_tie(pin(cy), y, d.ry) : d.dy;
+    d.dz = (fabsf(d.rz) == 0.5f) ? wrap_tie(pin(cz), z, d.rz) : d.dz;
+  }
+}
+
+template <bool COMP>
+__device__ __forceinline__ void accumulate(const Disp &d, float gm, float eps2, float &ax,
+                                           float &ay, float &az, float &cx, float &cy,
+                                           float &cz) {
+  const float inv = __builtin_amdgcn_rsqf(d.d2 + eps2);
+  const float f = gm * ((inv * inv) * inv);
+  if (COMP) {
+    neumaier_add(ax, cx, f * d.dx);
+    neumaier_add(ay, cy, f * d.dy);
+    neumaier_add(az, cz, f * d.dz);
+  } else {
+    ax = __builtin_fmaf(f, d.dx, ax);
+    ay = __builtin_fmaf(f, d.dy, ay);
+    az = __builtin_fmaf(f, d.dz, az);
+  }
+}
+
 // PPL particles per lane: a wavefront walks for 64*PPL Morton-consecutive
 // particles (lane l holds particles base + 64h + l, h < PPL).  Each stack
 // entry carries one 64-bit lane mask per half; the masks of each half are
 // nested along the stack, so the depth bound of 64 still holds.  With PPL = 2
 // one scalar node load serves twice the particles, at the price of a larger
-// union of visited nodes.
+// union of visited nodes.  Leaves (node ids < n_tree) take a separate path:
+// they are always accepted and never push.
 template <int MAC, bool COMP, bool STATS, bool GROUP_THETA, int PPL>
 __global__ __launch_bounds__(256) void k_traverse(
     const float *__restrict__ px, const float *__restrict__ py,
-    const float *__restrict__ pz, const cosmo_node *__restrict__ nodes,
+    const float *__restrict__ pz, const void *__restrict__ tree, int n_tree,
     const int32_t *__restrict__ root_idx, float *__restrict__ force, TravParams p,
     const float *__restrict__ theta_group, int64_t lo, int64_t hi,
     unsigned long long *__restrict__ stats) {
+  typedef typename InodeOf<MAC>::type Inode;
+  const float4 *__restrict__ leaves = reinterpret_cast<const float4 *>(tree);
+  const Inode *__restrict__ inodes = reinterpret_cast<const Inode *>(
+      reinterpret_cast<const char *>(tree) + int64_t(n_tree) * sizeof(cosmo_leaf));
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const int64_t base = lo + wave * (kWave * PPL);
@@ -126,19 +202,40 @@ __global__ __launch_bounds__(256) void k_traverse(
       in[h] = ((lane < 32 ? mlo >> lane : mhi >> (lane - 32)) & 1u) != 0;
       depth[h] -= in[h] ? 1 : 0;
     }
-    const cosmo_node rec = nodes[node];
+    if (node < n_tree) {
+      // leaf: accepted unconditionally (physics_kernels.py:128,135)
+      const float4 L = leaves[node];
+      if (L.w <= 0.0f) continue;
+      const float gm = p.G * L.w;
+#pragma unroll
+      for (int h = 0; h < PPL; ++h) {
+        if (!in[h]) continue;
+        Disp d = displace(L.x, L.y, L.z, x[h], y[h], z[h]);
+        bool near_ok = d.d2 > 1e-7f;
+        // the reference's d^2 > 1e-7 cut in FP64 where FP32 rounding (at most
+        // 2e-10 near the cut) could decide otherwise
+        if (MAC == COSMO_MAC_REF && __builtin_expect(fabsf(d.d2 - 1e-7f) <= 2e-10f, 0))
+          near_ok = exact_d2(L.x, L.y, L.z, x[h], y[h], z[h]) > 1e-7;
+        if (near_ok) {
+          fix_ties(d, L.x, L.y, L.z, x[h], y[h], z[h]);
+          accumulate<COMP>(d, gm, p.eps2, ax[h], ay[h], az[h], cx[h], cy[h], cz[h]);
+          if (STATS) n_int++;
+        }
+        if (STATS) n_vis++;
+      }
+      continue;
+    }
+    const Inode rec = inodes[node - n_tree];
     if (rec.m <= 0.0f) continue;
-    const bool leaf = rec.c1 == -1;
+    const float gm = p.G * rec.m;
     bool open[PPL];
 #pragma unroll
     for (int h = 0; h < PPL; ++h) {
       open[h] = false;
       if (!in[h]) continue;
-      const float rx = rec.cx - x[h], ry = rec.cy - y[h], rz = rec.cz - z[h];
-      float dx = rx - rintf(rx), dy = ry - rintf(ry), dz = rz - rintf(rz);
-      const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-      bool accept = true, near_ok = d2 > 1e-7f, unsure = false;
-      if (MAC == COSMO_MAC_REF) {
+      Disp d = displace(rec.cx, rec.cy, rec.cz, x[h], y[h], z[h]);
+      bool accept, near_ok = d.d2 > 1e-7f;
+      if constexpr (MAC == COSMO_MAC_REF) {
         // The reference decides in FP64 (physics_kernels.py:122-136).  The
         // FP32 decision is final unless rounding could flip it: with
         // |d_k - d_exact,k| <= u |c_k - p_k| <= u and a few u on the products,
@@ -146,58 +243,34 @@ __global__ __launch_bounds__(256) void k_traverse(
         // <= 11.8 u theta^2 |d|_1 (|d| <= sqrt(3)/2), and |d^2 - d_exact^2|
         // <= 2e-10 near the 1e-7 cut.  Lanes inside those bands redo both
         // tests in FP64 exactly as the reference does.
-        unsure = fabsf(d2 - 1e-7f) <= 2e-10f;
-        if (!leaf) {
-          const float t = th2[h] * d2;
-          const float sx2 = rec.a * rec.a;
-          accept = sx2 < t;
-          const float l1 = fabsf(dx) + fabsf(dy) + fabsf(dz);
-          unsure = unsure || (fabsf(sx2 - t) <= (16.0f * kU) * th2[h] * l1);
-        }
+        const float sx = rec.sx;
+        const float t = th2[h] * d.d2;
+        const float sx2 = sx * sx;
+        accept = sx2 < t;
+        const float l1 = fabsf(d.dx) + fabsf(d.dy) + fabsf(d.dz);
+        const bool unsure = (fabsf(d.d2 - 1e-7f) <= 2e-10f) ||
+                            (fabsf(sx2 - t) <= (16.0f * kU) * th2[h] * l1);
         if (__builtin_expect(unsure, 0)) {
-          double ex = static_cast<double>(pin(rec.cx)) - static_cast<double>(x[h]);
-          double ey = static_cast<double>(pin(rec.cy)) - static_cast<double>(y[h]);
-          double ez = static_cast<double>(pin(rec.cz)) - static_cast<double>(z[h]);
-          ex -= rint(ex);
-          ey -= rint(ey);
-          ez -= rint(ez);
-          const double e2 = ex * ex + ey * ey + ez * ez;
-          if (!leaf) {
-            const double sxd = static_cast<double>(pin(rec.a));
-            accept = (sxd * sxd) < (th2d[h] * e2);
-          }
+          const double e2 = exact_d2(rec.cx, rec.cy, rec.cz, x[h], y[h], z[h]);
+          const double sxd = static_cast<double>(pin(sx));
+          accept = (sxd * sxd) < (th2d[h] * e2);
           near_ok = e2 > 1e-7;
         }
-      } else if (!leaf) {
-        // (s/theta + delta)^2 < d^2, and every particle of the node sits on
-        // the same side of the minimum-image boundary as its CoM in each axis
-        const float r = __builtin_fmaf(rec.a, inv_theta[h], rec.b);
+      } else {
+        // (s/theta + dl)^2 < d^2, and every particle of the node sits on the
+        // same side of the minimum-image boundary as its CoM in each axis
+        const cosmo_inode_aabb &ra = rec;
+        const float r = __builtin_fmaf(ra.s, inv_theta[h], ra.dl);
         const float hb = p.half_box;
-        const bool inside = (dx + rec.hi[0] < hb) && (dx - rec.lo[0] > -hb) &&
-                            (dy + rec.hi[1] < hb) && (dy - rec.lo[1] > -hb) &&
-                            (dz + rec.hi[2] < hb) && (dz - rec.lo[2] > -hb);
-        accept = (r * r < d2) && inside;
+        const bool inside = (d.dx + ra.hi[0] < hb) && (d.dx - ra.lo[0] > -hb) &&
+                            (d.dy + ra.hi[1] < hb) && (d.dy - ra.lo[1] > -hb) &&
+                            (d.dz + ra.hi[2] < hb) && (d.dz - ra.lo[2] > -hb);
+        accept = (r * r < d.d2) && inside;
       }
       if (accept) {
         if (near_ok) {
-          // A component with |c - p| rounding to exactly 1/2 may belong to
-          // the other image; d^2 is the same either way, the direction is not.
-          if (__builtin_expect(d2 >= 0.25f, 0)) {
-            dx = (fabsf(rx) == 0.5f) ? wrap_tie(pin(rec.cx), x[h], rx) : dx;
-            dy = (fabsf(ry) == 0.5f) ? wrap_tie(pin(rec.cy), y[h], ry) : dy;
-            dz = (fabsf(rz) == 0.5f) ? wrap_tie(pin(rec.cz), z[h], rz) : dz;
-          }
-          const float inv = __builtin_amdgcn_rsqf(d2 + p.eps2);
-          const float f = (p.G * rec.m) * ((inv * inv) * inv);
-          if (COMP) {
-            neumaier_add(ax[h], cx[h], f * dx);
-            neumaier_add(ay[h], cy[h], f * dy);
-            neumaier_add(az[h], cz[h], f * dz);
-          } else {
-            ax[h] = __builtin_fmaf(f, dx, ax[h]);
-            ay[h] = __builtin_fmaf(f, dy, ay[h]);
-            az[h] = __builtin_fmaf(f, dz, az[h]);
-          }
+          fix_ties(d, rec.cx, rec.cy, rec.cz, x[h], y[h], z[h]);
+          accumulate<COMP>(d, gm, p.eps2, ax[h], ay[h], az[h], cx[h], cy[h], cz[h]);
           if (STATS) n_int++;
         }
       } else {
@@ -215,31 +288,26 @@ __global__ __launch_bounds__(256) void k_traverse(
       any = any || (om != 0);
     }
     if (any) {
-      const int np = (rec.c1 != -1 ? 1 : 0) + (rec.c2 != -1 ? 1 : 0);
+      // internal nodes have two children (build_tree): push c2, then c1
 #pragma unroll
       for (int h = 0; h < PPL; ++h)
-        if (open[h]) depth[h] += np;
-      // push = write lane `top` of the stack registers: c2, then c1
-      if (rec.c1 != -1) {
-        const bool here = (lane == top);
-        stk_node = here ? rec.c2 : stk_node;
+        if (open[h]) depth[h] += 2;
+      bool here = (lane == top);
+      stk_node = here ? rec.c2 : stk_node;
 #pragma unroll
-        for (int h = 0; h < PPL; ++h) {
-          stk_lo[h] = here ? olo[h] : stk_lo[h];
-          stk_hi[h] = here ? ohi[h] : stk_hi[h];
-        }
-        ++top;
+      for (int h = 0; h < PPL; ++h) {
+        stk_lo[h] = here ? olo[h] : stk_lo[h];
+        stk_hi[h] = here ? ohi[h] : stk_hi[h];
       }
-      if (rec.c2 != -1) {
-        const bool here = (lane == top);
-        stk_node = here ? rec.c1 : stk_node;
+      ++top;
+      here = (lane == top);
+      stk_node = here ? rec.c1 : stk_node;
 #pragma unroll
-        for (int h = 0; h < PPL; ++h) {
-          stk_lo[h] = here ? olo[h] : stk_lo[h];
-          stk_hi[h] = here ? ohi[h] : stk_hi[h];
-        }
-        ++top;
+      for (int h = 0; h < PPL; ++h) {
+        stk_lo[h] = here ? olo[h] : stk_lo[h];
+        stk_hi[h] = here ? ohi[h] : stk_hi[h];
       }
+      ++top;
     }
   }
 #pragma unroll
@@ -321,43 +389,43 @@ __global__ __launch_bounds__(1024) void k_direct_sum(
 
 template <int MAC, bool COMP, bool STATS, bool GT, int PPL>
 static void launch_traverse(hipStream_t s, const float *px, const float *py, const float *pz,
-                            const cosmo_node *nodes, const int32_t *root_idx, float *force,
+                            const void *nodes, int n_tree, const int32_t *root_idx, float *force,
                             TravParams p, const float *tg, int64_t lo, int64_t hi,
                             uint64_t *stats) {
   const int64_t waves = (hi - lo + kWave * PPL - 1) / (kWave * PPL);
   const unsigned grid = blocks_for(waves * kWave, 256);
   k_traverse<MAC, COMP, STATS, GT, PPL><<<grid, 256, 0, s>>>(
-      px, py, pz, nodes, root_idx, force, p, tg, lo, hi,
+      px, py, pz, nodes, n_tree, root_idx, force, p, tg, lo, hi,
       reinterpret_cast<unsigned long long *>(stats));
 }
 
 template <int MAC, bool COMP, bool STATS, bool GT>
 static void dispatch_ppl(bool pairs, hipStream_t s, const float *px, const float *py,
-                         const float *pz, const cosmo_node *nodes, const int32_t *root_idx,
+                         const float *pz, const void *nodes, int n_tree, const int32_t *root_idx,
                          float *force, TravParams p, const float *tg, int64_t lo, int64_t hi,
                          uint64_t *stats) {
   if (pairs)
-    launch_traverse<MAC, COMP, STATS, GT, 2>(s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
+    launch_traverse<MAC, COMP, STATS, GT, 2>(s, px, py, pz, nodes, n_tree, root_idx, force, p, tg, lo, hi, stats);
   else
-    launch_traverse<MAC, COMP, STATS, GT, 1>(s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
+    launch_traverse<MAC, COMP, STATS, GT, 1>(s, px, py, pz, nodes, n_tree, root_idx, force, p, tg, lo, hi, stats);
 }
 
 template <int MAC, bool COMP>
 static void dispatch_traverse2(bool stats_on, bool gt, bool pairs, hipStream_t s,
                                const float *px, const float *py, const float *pz,
-                               const cosmo_node *nodes, const int32_t *root_idx, float *force,
-                               TravParams p, const float *tg, int64_t lo, int64_t hi,
+                               const void *nodes, int n_tree, const int32_t *root_idx,
+                               float *force, TravParams p, const float *tg, int64_t lo, int64_t hi,
                                uint64_t *stats) {
   if (stats_on) {
     if (gt)
-      dispatch_ppl<MAC, COMP, true, true>(pairs, s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
+      dispatch_ppl<MAC, COMP, true, true>(pairs, s, px, py, pz, nodes, n_tree, root_idx, force, p, tg, lo, hi, stats);
     else
-      dispatch_ppl<MAC, COMP, true, false>(pairs, s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
+      dispatch_ppl<MAC, COMP, true, false>(pairs, s, px, py, pz, nodes, n_tree, root_idx, force, p, tg, lo, hi, stats);
   } else {
     if (gt)
-      dispatch_ppl<MAC, COMP, false, true>(pairs, s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
+      dispatch_ppl<MAC, COMP, false, true>(pairs, s, px, py, pz, nodes, n_tree, root_idx, force, p, tg, lo, hi, stats);
     else
-      dispatch_ppl<MAC, COMP, false, false>(pairs, s, px, py, pz, nodes, root_idx, force, p, tg, lo, hi, stats);
+      dispatch_ppl<MAC, COMP, false, false>(pairs, s, px, py, pz, nodes, n_tree, root_idx, force, p, tg, lo, hi, stats);
   }
 }
 
@@ -367,12 +435,14 @@ using namespace cosmo;
 
 extern "C" {
 
-int cosmo_traverse(const float *px, const float *py, const float *pz, const cosmo_node *nodes,
-                   const int32_t *root_idx, float *force, double theta,
+int cosmo_traverse(const float *px, const float *py, const float *pz, const void *nodes,
+                   int64_t n_tree, const int32_t *root_idx, float *force, double theta,
                    const float *theta_group, double G, double soft, double box, int64_t off,
                    int64_t count, int64_t n_bodies, int mac, int flags, uint64_t *stats,
                    void *stream) {
-  if (n_bodies < 0 || n_bodies > kMaxBodies || off < 0 || count < 0) return COSMO_E_INVALID;
+  if (n_bodies < 0 || n_bodies > kMaxBodies || off < 0 || count < 0 || n_tree < 1 ||
+      n_tree > kMaxBodies)
+    return COSMO_E_INVALID;
   if (mac != COSMO_MAC_REF && mac != COSMO_MAC_AABB) return COSMO_E_INVALID;
   const bool stats_on = (flags & COSMO_TRAV_STATS) != 0;
   if (stats_on && !stats) return COSMO_E_INVALID;
@@ -395,14 +465,14 @@ int cosmo_traverse(const float *px, const float *py, const float *pz, const cosm
   const bool gt = theta_group != nullptr;
   if (mac == COSMO_MAC_REF) {
     if (comp)
-      dispatch_traverse2<COSMO_MAC_REF, true>(stats_on, gt, pairs, s, px, py, pz, nodes, root_idx, force, p, theta_group, lo, hi, stats);
+      dispatch_traverse2<COSMO_MAC_REF, true>(stats_on, gt, pairs, s, px, py, pz, nodes, static_cast<int>(n_tree), root_idx, force, p, theta_group, lo, hi, stats);
     else
-      dispatch_traverse2<COSMO_MAC_REF, false>(stats_on, gt, pairs, s, px, py, pz, nodes, root_idx, force, p, theta_group, lo, hi, stats);
+      dispatch_traverse2<COSMO_MAC_REF, false>(stats_on, gt, pairs, s, px, py, pz, nodes, static_cast<int>(n_tree), root_idx, force, p, theta_group, lo, hi, stats);
   } else {
     if (comp)
-      dispatch_traverse2<COSMO_MAC_AABB, true>(stats_on, gt, pairs, s, px, py, pz, nodes, root_idx, force, p, theta_group, lo, hi, stats);
+      dispatch_traverse2<COSMO_MAC_AABB, true>(stats_on, gt, pairs, s, px, py, pz, nodes, static_cast<int>(n_tree), root_idx, force, p, theta_group, lo, hi, stats);
     else
-      dispatch_traverse2<COSMO_MAC_AABB, false>(stats_on, gt, pairs, s, px, py, pz, nodes, root_idx, force, p, theta_group, lo, hi, stats);
+      dispatch_traverse2<COSMO_MAC_AABB, false>(stats_on, gt, pairs, s, px, py, pz, nodes, static_cast<int>(n_tree), root_idx, force, p, theta_group, lo, hi, stats);
   }
   COSMO_CHECK_LAUNCH();
   return COSMO_OK;
@@ -422,8 +492,8 @@ int cosmo_compute_forces(cosmo_ctx *ctx, const float *px, const float *py, const
   const int rc = cosmo_pack_nodes(children, n_mass, n_com, n_min, n_max, 2 * n_bodies - 1, box,
                                   COSMO_MAC_REF, ctx->nodes, stream);
   if (rc != COSMO_OK) return rc;
-  return cosmo_traverse(px, py, pz, ctx->nodes, root_idx, force, theta, nullptr, G, soft, box,
-                        off, count, n_bodies, COSMO_MAC_REF, 0, nullptr, stream);
+  return cosmo_traverse(px, py, pz, ctx->nodes, n_bodies, root_idx, force, theta, nullptr, G,
+                        soft, box, off, count, n_bodies, COSMO_MAC_REF, 0, nullptr, stream);
 }
 
 int cosmo_direct_sum(const float *px, const float *py, const float *pz, const float *mass,

@@ -224,15 +224,17 @@ __global__ void k_top_tree_cache(const int32_t *__restrict__ child,
   while (write_idx < cache_size) top_nodes[write_idx++] = -1;
 }
 
-// Packs node k into a 32-byte FP32 record; NORMALIZE also divides the FP64
-// CoM sums by the mass first (normalize_com fused) and writes them back.
+// Packs node k into its FP32 record (leaf or internal, see cosmo_abi.h);
+// NORMALIZE also divides the FP64 CoM sums by the mass first (normalize_com
+// fused) and writes them back.
 template <int MAC, bool NORMALIZE>
 __global__ __launch_bounds__(256) void k_pack_nodes(
     const int32_t *__restrict__ children, const double *__restrict__ n_mass,
     double *__restrict__ n_com, const float *__restrict__ n_min,
-    const float *__restrict__ n_max, int64_t m, float boxf, cosmo_node *__restrict__ out) {
+    const float *__restrict__ n_max, int64_t m, float boxf, void *__restrict__ out) {
   const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k >= m) return;
+  const int64_t n = (m + 1) / 2;
   const double nm = n_mass[k];
   double c0 = n_com[3 * k + 0], c1 = n_com[3 * k + 1], c2 = n_com[3 * k + 2];
   if (NORMALIZE && nm > 0) {
@@ -244,17 +246,23 @@ __global__ __launch_bounds__(256) void k_pack_nodes(
     n_com[3 * k + 1] = c1;
     n_com[3 * k + 2] = c2;
   }
+  const float fx = static_cast<float>(c0), fy = static_cast<float>(c1), fz = static_cast<float>(c2);
+  const float fm = static_cast<float>(nm);
+  if (k < n) {
+    reinterpret_cast<float4 *>(out)[k] = make_float4(fx, fy, fz, fm);
+    return;
+  }
   const int2 ch = reinterpret_cast<const int2 *>(children)[k];
   const float lo0 = n_min[3 * k + 0], lo1 = n_min[3 * k + 1], lo2 = n_min[3 * k + 2];
   const float hi0 = n_max[3 * k + 0], hi1 = n_max[3 * k + 1], hi2 = n_max[3 * k + 2];
-  const float fx = static_cast<float>(c0), fy = static_cast<float>(c1), fz = static_cast<float>(c2);
-  float a, b;
+  char *ibase = reinterpret_cast<char *>(out) + n * sizeof(cosmo_leaf);
   if (MAC == COSMO_MAC_REF) {
     // physics_kernels.py:129-130
     float sx = hi0 - lo0;
     sx = (sx > boxf * 0.5f) ? boxf : sx;
-    a = sx;
-    b = 0.0f;
+    float4 *dst = reinterpret_cast<float4 *>(ibase + (k - n) * sizeof(cosmo_inode_ref));
+    dst[0] = make_float4(fx, fy, fz, fm);
+    dst[1] = make_float4(sx, __int_as_float(ch.x), __int_as_float(ch.y), 0.0f);
   } else {
     const float e0 = hi0 - lo0, e1 = hi1 - lo1, e2 = hi2 - lo2;
     float s = 0.0f;
@@ -264,13 +272,10 @@ __global__ __launch_bounds__(256) void k_pack_nodes(
     const float o0 = fx - 0.5f * (hi0 + lo0);
     const float o1 = fy - 0.5f * (hi1 + lo1);
     const float o2 = fz - 0.5f * (hi2 + lo2);
-    a = s;
-    b = sqrtf(__builtin_fmaf(o2, o2, __builtin_fmaf(o1, o1, o0 * o0)));
-  }
-  float4 *dst = reinterpret_cast<float4 *>(out + k);
-  dst[0] = make_float4(fx, fy, fz, static_cast<float>(nm));
-  dst[1] = make_float4(a, b, __int_as_float(ch.x), __int_as_float(ch.y));
-  if (MAC == COSMO_MAC_AABB) {
+    const float dl = sqrtf(__builtin_fmaf(o2, o2, __builtin_fmaf(o1, o1, o0 * o0)));
+    float4 *dst = reinterpret_cast<float4 *>(ibase + (k - n) * sizeof(cosmo_inode_aabb));
+    dst[0] = make_float4(fx, fy, fz, fm);
+    dst[1] = make_float4(s, dl, __int_as_float(ch.x), __int_as_float(ch.y));
     dst[2] = make_float4(fx - lo0, fy - lo1, fz - lo2, hi0 - fx);
     dst[3] = make_float4(hi1 - fy, hi2 - fz, 0.0f, 0.0f);
   }
@@ -424,8 +429,8 @@ int cosmo_build_top_tree_cache(const int32_t *child, const int32_t *root_idx,
 
 static int pack_impl(const int32_t *children, const double *n_mass, double *n_com,
                      const float *n_min, const float *n_max, int64_t m, double box, int mac,
-                     bool normalize, cosmo_node *nodes, void *stream) {
-  if (m < 0 || m > 2 * kMaxBodies) return COSMO_E_INVALID;
+                     bool normalize, void *nodes, void *stream) {
+  if (m < 0 || m > 2 * kMaxBodies || (m > 0 && (m & 1) == 0)) return COSMO_E_INVALID;
   if (m == 0) return COSMO_OK;
   if (!children || !n_mass || !n_com || !n_min || !n_max || !nodes) return COSMO_E_INVALID;
   const unsigned g = blocks_for(m, 256);
@@ -447,14 +452,14 @@ static int pack_impl(const int32_t *children, const double *n_mass, double *n_co
 
 int cosmo_pack_nodes(const int32_t *children, const double *n_mass, const double *n_com,
                      const float *n_min, const float *n_max, int64_t n_nodes, double box,
-                     int mac, cosmo_node *nodes, void *stream) {
+                     int mac, void *nodes, void *stream) {
   return pack_impl(children, n_mass, const_cast<double *>(n_com), n_min, n_max, n_nodes, box,
                    mac, false, nodes, stream);
 }
 
 int cosmo_normalize_pack(const int32_t *children, const double *n_mass, double *n_com,
                          const float *n_min, const float *n_max, int64_t n_nodes, double box,
-                         int mac, cosmo_node *nodes, void *stream) {
+                         int mac, void *nodes, void *stream) {
   return pack_impl(children, n_mass, n_com, n_min, n_max, n_nodes, box, mac, true, nodes,
                    stream);
 }

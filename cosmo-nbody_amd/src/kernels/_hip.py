@@ -29,7 +29,6 @@ TRAV_COMPENSATED = 1
 TRAV_STATS = 2
 TRAV_PAIRS = 4
 RENDER_FRAC_BITS = 20
-NODE_BYTES = 64
 
 _p = C.c_void_p
 _i64 = C.c_int64
@@ -44,6 +43,7 @@ SIGNATURES = {
     "cosmo_ctx_create": ([_i64, C.POINTER(_p)], _int),
     "cosmo_ctx_destroy": ([_p], _int),
     "cosmo_ctx_nodes": ([_p], _p),
+    "cosmo_tree_bytes": ([_i64, _int], _i64),
     "cosmo_compute_codes": ([_p, _p, _p, _p, _p, _i64, _int, _p], _int),
     "cosmo_sort_by_code": ([_p, _p, _p, _p, _i64, _int, _p], _int),
     "cosmo_permute": ([_p, _int, C.POINTER(_p), C.POINTER(_p), _i64, _p], _int),
@@ -55,8 +55,8 @@ SIGNATURES = {
     "cosmo_build_top_tree_cache": ([_p, _p, _p, _p, _i64, _p], _int),
     "cosmo_pack_nodes": ([_p] * 5 + [_i64, _f64, _int, _p, _p], _int),
     "cosmo_normalize_pack": ([_p] * 5 + [_i64, _f64, _int, _p, _p], _int),
-    "cosmo_traverse": ([_p] * 6 + [_f64, _p, _f64, _f64, _f64, _i64, _i64, _i64, _int, _int, _p,
-                                    _p], _int),
+    "cosmo_traverse": ([_p] * 4 + [_i64, _p, _p, _f64, _p, _f64, _f64, _f64, _i64, _i64, _i64,
+                                    _int, _int, _p, _p], _int),
     "cosmo_compute_forces": ([_p] * 10 + [_f64] * 4 + [_p, _p, _p, _i64, _i64, _i64, _p], _int),
     "cosmo_integrate": ([_p] * 7 + [_f64, _f64, _f64, _i64, _p], _int),
     "cosmo_kick_drift": ([_p] * 7 + [_f32, _f32, _f32, _i64, _p], _int),

@@ -61,6 +61,7 @@ def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0
     they differ.  pairs: two particles per lane (128-particle groups)."""
     n = pos_x.shape[0]
     n_nodes = 2 * n - 1 if n_nodes is None else int(n_nodes)
+    n_tree = (n_nodes + 1) // 2
     count = n - off if count is None else count
     flags = ((TRAV_COMPENSATED if compensated else 0) | (TRAV_STATS if stats is not None else 0) |
              (TRAV_PAIRS if pairs else 0))
@@ -69,7 +70,7 @@ def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0
                                               numel=max(groups, 0))
     st = None if stats is None else ptr(stats, (torch.int64, torch.uint64), "stats", numel=3)
     check(load().cosmo_traverse(
-        *_xyz(pos_x, pos_y, pos_z, n), ptr(nodes, _F32, "nodes", numel=16 * n_nodes),
+        *_xyz(pos_x, pos_y, pos_z, n), ptr(nodes, torch.uint8, "nodes"), n_tree,
         ptr(root, torch.int32, "root"), ptr(force, _F32, "force", numel=3 * n), float(theta), tg,
         float(G), float(soft), float(box), int(off), int(count), n, int(mac), flags, st,
         stream_handle()), "traverse")

@@ -17,14 +17,15 @@ the reference stores float32).  build_tree and compute_multipoles need a
 launch that covers every node, as all of the reference's callers use.
 
 Additions without a reference counterpart: pack_nodes / normalize_pack (the
-64-byte FP32 records the traversal reads) and validate_tree.
+packed FP32 leaf and internal-node records the traversal reads, a uint8
+buffer of tree_bytes(n_nodes, mac) bytes) and validate_tree.
 """
 from __future__ import annotations
 
 import torch
 
-from ._hip import (MAC_AABB, MAC_REF, NODE_BYTES, U32, U64, check, kernel, launch_extent, load,
-                   ptr, shared_context, stream_handle)
+from ._hip import (MAC_AABB, MAC_REF, U32, U64, check, kernel, launch_extent, load, ptr,
+                   shared_context, stream_handle)
 
 __all__ = ["reset_tree_buffers", "build_tree", "compute_multipoles", "normalize_com",
            "find_root", "build_top_tree_cache", "pack_nodes", "normalize_pack",
@@ -100,9 +101,14 @@ def build_top_tree_cache(child, root_idx, top_nodes, node_to_cache, cache_size, 
         "build_top_tree_cache")
 
 
-def node_buffer(n_nodes: int, device=None) -> torch.Tensor:
-    """Uninitialised storage for n_nodes packed 64-byte node records."""
-    return torch.empty((n_nodes, NODE_BYTES // 4), dtype=torch.float32, device=device or "cuda")
+def tree_bytes(n_nodes: int, mac=MAC_AABB) -> int:
+    """Bytes of the packed records of a tree with n_nodes = 2n-1 nodes."""
+    return int(load().cosmo_tree_bytes((n_nodes + 1) // 2, int(mac)))
+
+
+def node_buffer(n_nodes: int, device=None, mac=MAC_AABB) -> torch.Tensor:
+    """Uninitialised byte storage for the packed records (room for `mac`)."""
+    return torch.empty(tree_bytes(n_nodes, mac), dtype=torch.uint8, device=device or "cuda")
 
 
 def _pack(fn, children, n_mass, n_com, n_min, n_max, box, mac, nodes, what):
@@ -112,20 +118,20 @@ def _pack(fn, children, n_mass, n_com, n_min, n_max, box, mac, nodes, what):
              ptr(n_com, torch.float64, "n_com", numel=3 * m),
              ptr(n_min, torch.float32, "n_min", numel=3 * m),
              ptr(n_max, torch.float32, "n_max", numel=3 * m), m, float(box), int(mac),
-             ptr(nodes, torch.float32, "nodes", numel=16 * m), stream_handle()), what)
+             ptr(nodes, torch.uint8, "nodes", numel=tree_bytes(m, mac)), stream_handle()), what)
     return nodes
 
 
 def pack_nodes(children, n_mass, n_com, n_min, n_max, box, mac=MAC_REF, nodes=None):
     """Packs normalised node arrays into FP32 records for acceptance test mac."""
-    nodes = node_buffer(children.shape[0], children.device) if nodes is None else nodes
+    nodes = node_buffer(children.shape[0], children.device, mac) if nodes is None else nodes
     return _pack(load().cosmo_pack_nodes, children, n_mass, n_com, n_min, n_max, box, mac,
                  nodes, "pack_nodes")
 
 
 def normalize_pack(children, n_mass, n_com, n_min, n_max, box, mac=MAC_REF, nodes=None):
     """normalize_com fused with pack_nodes (n_com is normalised in place)."""
-    nodes = node_buffer(children.shape[0], children.device) if nodes is None else nodes
+    nodes = node_buffer(children.shape[0], children.device, mac) if nodes is None else nodes
     return _pack(load().cosmo_normalize_pack, children, n_mass, n_com, n_min, n_max, box, mac,
                  nodes, "normalize_pack")
 

@@ -58,20 +58,37 @@ extern "C" {
 /* Fixed-point fraction bits of the density bins used by the drop-ins. */
 #define COSMO_RENDER_FRAC_BITS 20
 
-/* Packed FP32 node record read by the traversal (64 bytes; the reference
- * test reads only the first 32).
- * REF:  a = sx (x-extent, replaced by box when > box/2), b = 0.
- * AABB: a = largest AABB extent s, b = |CoM - AABB centre|,
- *       lo[k] = CoM_k - min_k, hi[k] = max_k - CoM_k (extent of the node's
- *       box below / above its CoM, for the minimum-image guard).
- * c1 == -1 marks a leaf. */
-typedef struct cosmo_node {
+/* Packed FP32 tree records read by the traversal.  One buffer of
+ * cosmo_tree_bytes(n, mac) bytes holds
+ *   n leaf records (cosmo_leaf, 16 B; node ids 0..n-1): CoM and mass;
+ *   then n-1 internal records for node ids n..2n-2, of the test's type:
+ *     REF  cosmo_inode_ref  (32 B): sx = x-extent, replaced by box when
+ *          > box/2 (physics_kernels.py:129-130);
+ *     AABB cosmo_inode_aabb (64 B): s = largest extent of the node's box,
+ *          dl = |CoM - box centre|, lo[k] = CoM_k - min_k, hi[k] = max_k - CoM_k.
+ * Leaves are the node ids below n, as build_tree produces them (a leaf is
+ * what the reference recognises by children == -1). */
+typedef struct cosmo_leaf {
+  float x, y, z, m;
+} cosmo_leaf;
+
+typedef struct cosmo_inode_ref {
   float cx, cy, cz, m;
-  float a, b;
+  float sx;
+  int32_t c1, c2;
+  float pad;
+} cosmo_inode_ref;
+
+typedef struct cosmo_inode_aabb {
+  float cx, cy, cz, m;
+  float s, dl;
   int32_t c1, c2;
   float lo[3], hi[3];
   float pad[2];
-} cosmo_node;
+} cosmo_inode_aabb;
+
+/* Size in bytes of the packed records of an n-leaf tree for test `mac`. */
+int64_t cosmo_tree_bytes(int64_t n, int mac);
 
 typedef struct cosmo_ctx cosmo_ctx;
 
@@ -81,8 +98,8 @@ const char *cosmo_status_string(int status);
  * validator, sized for up to n_max particles on the current device. */
 int cosmo_ctx_create(int64_t n_max, cosmo_ctx **out);
 int cosmo_ctx_destroy(cosmo_ctx *ctx);
-/* Node records inside the context (2*n_max-1 entries). */
-cosmo_node *cosmo_ctx_nodes(cosmo_ctx *ctx);
+/* Packed-record buffer inside the context (room for either test at n_max). */
+void *cosmo_ctx_nodes(cosmo_ctx *ctx);
 
 /* src/kernels/morton.py:65-91 compute_codes(pos_x, pos_y, pos_z, codes,
  * indices, n_bodies).  indices may be NULL. */
@@ -140,29 +157,31 @@ int cosmo_build_top_tree_cache(const int32_t *child, const int32_t *root_idx,
                                int32_t *top_nodes, int32_t *node_to_cache,
                                int64_t cache_size, void *stream);
 
-/* Packs the reference-layout node arrays into cosmo_node records for the
- * acceptance test `mac` (no reference counterpart: the reference reads its
- * arrays directly, physics_kernels.py:96-113). */
+/* Packs the reference-layout node arrays (n_nodes = 2n-1) into the records
+ * for the acceptance test `mac` (no reference counterpart: the reference
+ * reads its arrays directly, physics_kernels.py:96-113). */
 int cosmo_pack_nodes(const int32_t *children, const double *n_mass,
                      const double *n_com, const float *n_min,
                      const float *n_max, int64_t n_nodes, double box, int mac,
-                     cosmo_node *nodes, void *stream);
+                     void *nodes, void *stream);
 
 /* normalize_com and cosmo_pack_nodes fused in one pass. */
 int cosmo_normalize_pack(const int32_t *children, const double *n_mass,
                          double *n_com, const float *n_min, const float *n_max,
                          int64_t n_nodes, double box, int mac,
-                         cosmo_node *nodes, void *stream);
+                         void *nodes, void *stream);
 
-/* Barnes-Hut traversal over packed records for particles [off, off+count)
- * ∩ [0, n_bodies) in Morton-contiguous groups of 64 (one wavefront each),
+/* Barnes-Hut traversal over the packed records of an n_tree-leaf tree for
+ * the particles [off, off+count) ∩ [0, n_bodies) of pos_* (the tree's own
+ * particles, or any other targets) in Morton-contiguous groups of 64 (one
+ * wavefront each),
  * writing force[i][0..2] (accelerations).  theta_group (nullable, device
  * float[ceil(count/64)]) gives each group's opening angle in place of
  * theta.  stats (nullable, device uint64[3]) accumulates interactions,
  * node visits (per particle) and stack pops (per wavefront) when flags has
  * COSMO_TRAV_STATS. */
 int cosmo_traverse(const float *pos_x, const float *pos_y, const float *pos_z,
-                   const cosmo_node *nodes, const int32_t *root_idx,
+                   const void *nodes, int64_t n_tree, const int32_t *root_idx,
                    float *force, double theta, const float *theta_group,
                    double G, double soft, double box, int64_t off,
                    int64_t count, int64_t n_bodies, int mac, int flags,
