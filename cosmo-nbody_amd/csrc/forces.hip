@@ -24,6 +24,7 @@ struct TravParams {
   double th2d;      // REF: theta^2 in FP64 (the reference's arithmetic)
   float th2;        // REF: theta^2
   float inv_theta;  // AABB: 1/theta
+  float theta;
   float G;
   float eps2;
   float half_box;
@@ -47,6 +48,17 @@ __device__ __forceinline__ float wrap_tie(float c, float p, float r) {
 // (an empty asm that claims to modify v; emits no instruction).
 __device__ __forceinline__ float pin(float v) {
   asm volatile("" : "+v"(v));
+  return v;
+}
+
+// Writes `value` into lane `slot` of v (v_writelane ignores EXEC).  The slot
+// goes through M0: a VALU instruction reads at most one SGPR, and value and
+// slot are both wave-uniform.
+__device__ __forceinline__ int write_lane(int v, int value, int slot) {
+  asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tv_writelane_b32 %0, %1, m0"
+               : "+v"(v)
+               : "s"(value), "s"(slot)
+               : "m0");
   return v;
 }
 
@@ -131,12 +143,15 @@ __device__ __forceinline__ void accumulate(const Disp &d, float gm, float eps2, 
 }
 
 // PPL particles per lane: a wavefront walks for 64*PPL Morton-consecutive
-// particles (lane l holds particles base + 64h + l, h < PPL).  Each stack
-// entry carries one 64-bit lane mask per half; the masks of each half are
-// nested along the stack, so the depth bound of 64 still holds.  With PPL = 2
-// one scalar node load serves twice the particles, at the price of a larger
-// union of visited nodes.  Leaves (node ids < n_tree) take a separate path:
-// they are always accepted and never push.
+// particles (lane l holds particles base + 64h + l, h < PPL).  The stack holds
+// node ids only.  The lanes of an entry are those that opened its parent, so
+// along the stack (bottom to top) the lane sets are nested: the entries that
+// hold a lane are a prefix of the stack, and the lane's own depth (the depth
+// of the reference's private stack) says how long that prefix is.  A lane
+// therefore visits the popped entry exactly when its depth exceeds the new
+// top.  With PPL = 2 one scalar node load serves twice the particles, at the
+// price of a larger union of visited nodes.  Leaves (node ids < n_tree) run
+// the same code with the acceptance test forced true.
 template <int MAC, bool COMP, bool STATS, bool GROUP_THETA, int PPL>
 __global__ __launch_bounds__(256) void k_traverse(
     const float *__restrict__ px, const float *__restrict__ py,
@@ -146,20 +161,18 @@ __global__ __launch_bounds__(256) void k_traverse(
     unsigned long long *__restrict__ stats) {
   typedef typename InodeOf<MAC>::type Inode;
   const float4 *__restrict__ leaves = reinterpret_cast<const float4 *>(tree);
-  const Inode *__restrict__ inodes = reinterpret_cast<const Inode *>(
-      reinterpret_cast<const char *>(tree) + int64_t(n_tree) * sizeof(cosmo_leaf));
+  const char *__restrict__ ibase =
+      reinterpret_cast<const char *>(tree) + int64_t(n_tree) * sizeof(cosmo_leaf);
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
   const int64_t base = lo + wave * (kWave * PPL);
   if (base >= hi) return;  // wave-uniform
-  float x[PPL], y[PPL], z[PPL], th2[PPL], inv_theta[PPL];
+  float x[PPL], y[PPL], z[PPL], th2[PPL], k1[PPL], k2[PPL], inv_theta[PPL];
   double th2d[PPL];
   float ax[PPL], ay[PPL], az[PPL], cx[PPL], cy[PPL], cz[PPL];
   int depth[PPL];
   bool active[PPL];
-  uint32_t n_int = 0, n_vis = 0, n_pop = 0;
-  int stk_node = 0;
-  uint32_t stk_lo[PPL], stk_hi[PPL];
+  uint32_t w_int = 0, w_vis = 0, w_pop = 0;  // wave totals (uniform)
   const int root = root_idx[0];
 #pragma unroll
   for (int h = 0; h < PPL; ++h) {
@@ -174,163 +187,138 @@ __global__ __launch_bounds__(256) void k_traverse(
     th2[h] = p.th2;
     inv_theta[h] = p.inv_theta;
     th2d[h] = p.th2d;
+    float th = p.theta;
     if (GROUP_THETA) {
       const int64_t g = wave * PPL + h;
       const float t = (base + h * kWave < hi) ? theta_group[g] : 0.5f;
       th2[h] = t * t;
       inv_theta[h] = 1.0f / t;
       th2d[h] = static_cast<double>(t) * static_cast<double>(t);
+      th = t;
     }
+    // rounding band of the reference test as a function of the node's sx
+    k1[h] = (48.0f * kU) * th;
+    k2[h] = (0.08f * kU) * th2[h];
     ax[h] = ay[h] = az[h] = cx[h] = cy[h] = cz[h] = 0.0f;
     depth[h] = active[h] ? 1 : 0;
-    const uint64_t am = __ballot(active[h]);
-    stk_lo[h] = (lane == 0) ? static_cast<uint32_t>(am) : 0u;
-    stk_hi[h] = (lane == 0) ? static_cast<uint32_t>(am >> 32) : 0u;
   }
-  if (lane == 0) stk_node = root;
+  int stk = (lane == 0) ? root : 0;  // lane k holds stack slot k
   int top = 1;
 
   while (top > 0) {
     --top;
-    if (STATS) n_pop++;
-    const int node = __builtin_amdgcn_readlane(stk_node, top);
+    if (STATS) w_pop++;
+    const int node = __builtin_amdgcn_readlane(stk, top);
     bool in[PPL];
 #pragma unroll
     for (int h = 0; h < PPL; ++h) {
-      const uint32_t mlo = __builtin_amdgcn_readlane(stk_lo[h], top);
-      const uint32_t mhi = __builtin_amdgcn_readlane(stk_hi[h], top);
-      in[h] = ((lane < 32 ? mlo >> lane : mhi >> (lane - 32)) & 1u) != 0;
-      depth[h] -= in[h] ? 1 : 0;
+      in[h] = depth[h] > top;
+      depth[h] = min(depth[h], top);
     }
-    if (node < n_tree) {
-      // leaf: accepted unconditionally (physics_kernels.py:128,135)
-      const float4 L = leaves[node];
-      if (L.w <= 0.0f) continue;
-      const float gm = p.G * L.w;
-#pragma unroll
-      for (int h = 0; h < PPL; ++h) {
-        if (!in[h]) continue;
-        Disp d = displace(L.x, L.y, L.z, x[h], y[h], z[h]);
-        bool near_ok = d.d2 > 1e-7f;
-        // the reference's d^2 > 1e-7 cut in FP64 where FP32 rounding (at most
-        // 2e-10 near the cut) could decide otherwise
-        if (MAC == COSMO_MAC_REF && __builtin_expect(fabsf(d.d2 - 1e-7f) <= 2e-10f, 0))
-          near_ok = exact_d2(L.x, L.y, L.z, x[h], y[h], z[h]) > 1e-7;
-        if (near_ok) {
-          fix_ties(d, L.x, L.y, L.z, x[h], y[h], z[h]);
-          accumulate<COMP>(d, gm, p.eps2, ax[h], ay[h], az[h], cx[h], cy[h], cz[h]);
-          if (STATS) n_int++;
-        }
-        if (STATS) n_vis++;
+    // Leaf and internal records start with (x, y, z, m).  The loads are
+    // wave-uniform (scalar cache).  For a leaf the other fields copy A, and
+    // for the reference test B.x (sx^2) = -1, so that the test accepts it.
+    const bool leaf = node < n_tree;
+    float4 A, B, C, D;
+    if (leaf) {
+      A = leaves[node];
+      B = C = D = A;
+      if constexpr (MAC == COSMO_MAC_REF) B.x = -1.0f;
+    } else {
+      const float4 *rec =
+          reinterpret_cast<const float4 *>(ibase + int64_t(node - n_tree) * sizeof(Inode));
+      A = rec[0];
+      B = rec[1];
+      if constexpr (MAC == COSMO_MAC_AABB) {
+        C = rec[2];
+        D = rec[3];
+      } else {
+        C = D = A;
       }
-      continue;
     }
-    const Inode rec = inodes[node - n_tree];
-    if (rec.m <= 0.0f) continue;
-    const float gm = p.G * rec.m;
-    bool open[PPL];
+    if (A.w <= 0.0f) continue;
+    bool open[PPL], hit[PPL];
 #pragma unroll
     for (int h = 0; h < PPL; ++h) {
-      open[h] = false;
-      if (!in[h]) continue;
-      Disp d = displace(rec.cx, rec.cy, rec.cz, x[h], y[h], z[h]);
+      // every lane evaluates the node; only the entry's lanes (in) use it
+      Disp d = displace(A.x, A.y, A.z, x[h], y[h], z[h]);
       bool accept, near_ok = d.d2 > 1e-7f;
       if constexpr (MAC == COSMO_MAC_REF) {
         // The reference decides in FP64 (physics_kernels.py:122-136).  The
-        // FP32 decision is final unless rounding could flip it: with
+        // FP32 decision is final unless rounding could flip it.  With
         // |d_k - d_exact,k| <= u |c_k - p_k| <= u and a few u on the products,
-        // |error of sx^2 - theta^2 d^2| <= theta^2 u (10.1 d^2 + 3 |d|_1)
-        // <= 11.8 u theta^2 |d|_1 (|d| <= sqrt(3)/2), and |d^2 - d_exact^2|
-        // <= 2e-10 near the 1e-7 cut.  Lanes inside those bands redo both
-        // tests in FP64 exactly as the reference does.
-        const float sx = rec.sx;
+        // E = |error of sx^2 - theta^2 d^2| <= theta^2 u (10.1 d^2 + 3 |d|_1)
+        // <= 11.8 u theta^2 |d|_1 (|d| <= sqrt(3)/2, |d|_1 <= 3/2).  A flip
+        // needs |sx^2 - theta^2 d^2| <= 2E, so theta^2 d^2 <= sx^2 + 35.4 u,
+        // |d|_1 <= sqrt(3) d <= 1.7321 sx / theta + 2.52e-3 and
+        // 2E <= u (40.9 theta sx + 0.0595 theta^2), which the band below
+        // covers.  Near the 1e-7 cut |d^2 - d_exact^2| <=
+        // 2e-10.  Lanes inside either band redo both tests in FP64 exactly
+        // as the reference does.  A leaf has sx^2 = -1: always accepted,
+        // never inside the band.
+        const float sx2 = B.x;
         const float t = th2[h] * d.d2;
-        const float sx2 = sx * sx;
         accept = sx2 < t;
-        const float l1 = fabsf(d.dx) + fabsf(d.dy) + fabsf(d.dz);
-        const bool unsure = (fabsf(d.d2 - 1e-7f) <= 2e-10f) ||
-                            (fabsf(sx2 - t) <= (16.0f * kU) * th2[h] * l1);
+        const float bnd = __builtin_fmaf(k1[h], B.y, k2[h]);
+        const bool unsure = in[h] & ((fabsf(d.d2 - 1e-7f) <= 2e-10f) | (fabsf(sx2 - t) <= bnd));
         if (__builtin_expect(unsure, 0)) {
-          const double e2 = exact_d2(rec.cx, rec.cy, rec.cz, x[h], y[h], z[h]);
-          const double sxd = static_cast<double>(pin(sx));
-          accept = (sxd * sxd) < (th2d[h] * e2);
+          const double e2 = exact_d2(A.x, A.y, A.z, x[h], y[h], z[h]);
+          if (!leaf) {
+            const double sxd = static_cast<double>(pin(B.y));
+            accept = (sxd * sxd) < (th2d[h] * e2);
+          }
           near_ok = e2 > 1e-7;
         }
       } else {
         // (s/theta + dl)^2 < d^2, and every particle of the node sits on the
         // same side of the minimum-image boundary as its CoM in each axis
-        const cosmo_inode_aabb &ra = rec;
-        const float r = __builtin_fmaf(ra.s, inv_theta[h], ra.dl);
+        const float r = __builtin_fmaf(B.x, inv_theta[h], B.y);
         const float hb = p.half_box;
-        const bool inside = (d.dx + ra.hi[0] < hb) && (d.dx - ra.lo[0] > -hb) &&
-                            (d.dy + ra.hi[1] < hb) && (d.dy - ra.lo[1] > -hb) &&
-                            (d.dz + ra.hi[2] < hb) && (d.dz - ra.lo[2] > -hb);
-        accept = (r * r < d.d2) && inside;
+        const bool inside = (d.dx + C.w < hb) && (d.dx - C.x > -hb) &&
+                            (d.dy + D.x < hb) && (d.dy - C.y > -hb) &&
+                            (d.dz + D.y < hb) && (d.dz - C.z > -hb);
+        accept = leaf || ((r * r < d.d2) && inside);
       }
-      if (accept) {
-        if (near_ok) {
-          fix_ties(d, rec.cx, rec.cy, rec.cz, x[h], y[h], z[h]);
-          accumulate<COMP>(d, gm, p.eps2, ax[h], ay[h], az[h], cx[h], cy[h], cz[h]);
-          if (STATS) n_int++;
-        }
-      } else {
-        open[h] = depth[h] + 1 < 64;
+      hit[h] = in[h] && accept && near_ok;
+      open[h] = in[h] && !accept && (depth[h] + 1 < 64);
+      if (hit[h]) {
+        fix_ties(d, A.x, A.y, A.z, x[h], y[h], z[h]);
+        accumulate<COMP>(d, A.w, p.eps2, ax[h], ay[h], az[h], cx[h], cy[h], cz[h]);
       }
-      if (STATS) n_vis++;
     }
-    uint32_t olo[PPL], ohi[PPL];
-    bool any = false;
+    uint64_t om = 0;
 #pragma unroll
     for (int h = 0; h < PPL; ++h) {
-      const uint64_t om = __ballot(open[h]);
-      olo[h] = static_cast<uint32_t>(om);
-      ohi[h] = static_cast<uint32_t>(om >> 32);
-      any = any || (om != 0);
+      om |= __ballot(open[h]);
+      if (STATS) {
+        w_vis += __builtin_popcountll(__ballot(in[h]));
+        w_int += __builtin_popcountll(__ballot(hit[h]));
+      }
     }
-    if (any) {
+    if (om != 0) {
       // internal nodes have two children (build_tree): push c2, then c1
+      // (physics_kernels.py:143-149)
+      stk = write_lane(stk, __float_as_int(B.w), top);
+      stk = write_lane(stk, __float_as_int(B.z), top + 1);
 #pragma unroll
-      for (int h = 0; h < PPL; ++h)
-        if (open[h]) depth[h] += 2;
-      bool here = (lane == top);
-      stk_node = here ? rec.c2 : stk_node;
-#pragma unroll
-      for (int h = 0; h < PPL; ++h) {
-        stk_lo[h] = here ? olo[h] : stk_lo[h];
-        stk_hi[h] = here ? ohi[h] : stk_hi[h];
-      }
-      ++top;
-      here = (lane == top);
-      stk_node = here ? rec.c1 : stk_node;
-#pragma unroll
-      for (int h = 0; h < PPL; ++h) {
-        stk_lo[h] = here ? olo[h] : stk_lo[h];
-        stk_hi[h] = here ? ohi[h] : stk_hi[h];
-      }
-      ++top;
+      for (int h = 0; h < PPL; ++h) depth[h] = open[h] ? top + 2 : depth[h];
+      top += 2;
     }
   }
+  // G multiplies the finished sums (exact for G = 1)
 #pragma unroll
   for (int h = 0; h < PPL; ++h) {
     if (active[h]) {
       const int64_t i = base + h * kWave + lane;
-      force[3 * i + 0] = ax[h] + cx[h];
-      force[3 * i + 1] = ay[h] + cy[h];
-      force[3 * i + 2] = az[h] + cz[h];
+      force[3 * i + 0] = p.G * (ax[h] + cx[h]);
+      force[3 * i + 1] = p.G * (ay[h] + cy[h]);
+      force[3 * i + 2] = p.G * (az[h] + cz[h]);
     }
   }
-  if (STATS) {
-    uint32_t a = n_int, b = n_vis;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      a += __shfl_xor(a, off, kWave);
-      b += __shfl_xor(b, off, kWave);
-    }
-    if (lane == 0) {
-      atomicAdd(&stats[0], static_cast<unsigned long long>(a));
-      atomicAdd(&stats[1], static_cast<unsigned long long>(b));
-      atomicAdd(&stats[2], static_cast<unsigned long long>(n_pop));
-    }
+  if (STATS && lane == 0) {
+    atomicAdd(&stats[0], static_cast<unsigned long long>(w_int));
+    atomicAdd(&stats[1], static_cast<unsigned long long>(w_vis));
+    atomicAdd(&stats[2], static_cast<unsigned long long>(w_pop));
   }
 }
 
@@ -455,6 +443,7 @@ int cosmo_traverse(const float *px, const float *py, const float *pz, const void
   p.th2d = theta * theta;
   p.th2 = thf * thf;
   p.inv_theta = 1.0f / thf;
+  p.theta = thf;
   p.G = static_cast<float>(G);
   const float sf = static_cast<float>(soft);
   p.eps2 = sf * sf;

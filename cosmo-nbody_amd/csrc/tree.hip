@@ -262,7 +262,7 @@ __global__ __launch_bounds__(256) void k_pack_nodes(
     sx = (sx > boxf * 0.5f) ? boxf : sx;
     float4 *dst = reinterpret_cast<float4 *>(ibase + (k - n) * sizeof(cosmo_inode_ref));
     dst[0] = make_float4(fx, fy, fz, fm);
-    dst[1] = make_float4(sx, __int_as_float(ch.x), __int_as_float(ch.y), 0.0f);
+    dst[1] = make_float4(sx * sx, sx, __int_as_float(ch.x), __int_as_float(ch.y));
   } else {
     const float e0 = hi0 - lo0, e1 = hi1 - lo1, e2 = hi2 - lo2;
     float s = 0.0f;

@@ -63,7 +63,8 @@ extern "C" {
  *   n leaf records (cosmo_leaf, 16 B; node ids 0..n-1): CoM and mass;
  *   then n-1 internal records for node ids n..2n-2, of the test's type:
  *     REF  cosmo_inode_ref  (32 B): sx = x-extent, replaced by box when
- *          > box/2 (physics_kernels.py:129-130);
+ *          > box/2 (physics_kernels.py:129-130), and sx2 = sx*sx rounded
+ *          to FP32;
  *     AABB cosmo_inode_aabb (64 B): s = largest extent of the node's box,
  *          dl = |CoM - box centre|, lo[k] = CoM_k - min_k, hi[k] = max_k - CoM_k.
  * Leaves are the node ids below n, as build_tree produces them (a leaf is
@@ -74,9 +75,8 @@ typedef struct cosmo_leaf {
 
 typedef struct cosmo_inode_ref {
   float cx, cy, cz, m;
-  float sx;
+  float sx2, sx;
   int32_t c1, c2;
-  float pad;
 } cosmo_inode_ref;
 
 typedef struct cosmo_inode_aabb {
