@@ -19,7 +19,12 @@ for p in (ROOT, os.path.join(ROOT, "cosmo-nbody_amd")):
         sys.path.insert(0, p)
 
 PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 (vector, packed) / dense MFMA FP32 figure
+PEAK_HBM_GBS = 8000.0         # MI355X HBM3E
 FLOP_PER_INTERACTION = 20     # standard particle-node interaction count
+# HBM bytes per traversal launch measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (scaled by a 256 MiB copy: x2.0 and x1.0), per workload; the
+# summary with its commands is profiles/step_kernels_hbm_traffic_128.md.
+TRAVERSAL_TRAFFIC_BYTES = {"repo_128": 761994 * 1024 * 2 + 24576 * 1024}
 
 
 def parse():
@@ -161,6 +166,7 @@ def main():
     trav_s = stage["forces"] * 1e-3
     flops = FLOP_PER_INTERACTION * inter
     achieved = flops / trav_s / 1e12
+    traffic = TRAVERSAL_TRAFFIC_BYTES.get(c.NAME) if not eng.pairs else None
     out = {
         "metric": f"ms/step + particles/s at N={c.N_GRID}^3 (full step)",
         "value": n * args.steps / elapsed,
@@ -184,7 +190,9 @@ def main():
         "wave_pops_per_particle": pops / n,
         "traversal": "pairs (128 particles per wavefront)" if eng.pairs else "64 per wavefront",
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": traffic,
+                     "hbm_frac": (traffic / trav_s / 1e9 / PEAK_HBM_GBS) if traffic else None,
                      "kernel": "k_traverse (FP32 VALU; 20 flop per accepted interaction)"},
     }
     if not args.no_cpu_baseline:
