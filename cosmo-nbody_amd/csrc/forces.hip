@@ -28,6 +28,7 @@ struct TravParams {
   float G;
   float eps2;
   float half_box;
+  int wave_stats;   // STATS: also store each wavefront's counts
 };
 
 // Unit roundoff of FP32.
@@ -319,6 +320,12 @@ __global__ __launch_bounds__(256) void k_traverse(
     atomicAdd(&stats[0], static_cast<unsigned long long>(w_int));
     atomicAdd(&stats[1], static_cast<unsigned long long>(w_vis));
     atomicAdd(&stats[2], static_cast<unsigned long long>(w_pop));
+    if (p.wave_stats) {
+      unsigned long long *ws = stats + 3 + 3 * wave;
+      ws[0] = w_int;
+      ws[1] = w_vis;
+      ws[2] = w_pop;
+    }
   }
 }
 
@@ -432,7 +439,7 @@ int cosmo_traverse(const float *px, const float *py, const float *pz, const void
       n_tree > kMaxBodies)
     return COSMO_E_INVALID;
   if (mac != COSMO_MAC_REF && mac != COSMO_MAC_AABB) return COSMO_E_INVALID;
-  const bool stats_on = (flags & COSMO_TRAV_STATS) != 0;
+  const bool stats_on = (flags & (COSMO_TRAV_STATS | COSMO_TRAV_GROUP_STATS)) != 0;
   if (stats_on && !stats) return COSMO_E_INVALID;
   const int64_t lo = off;
   const int64_t hi = (off + count < n_bodies) ? off + count : n_bodies;
@@ -448,6 +455,7 @@ int cosmo_traverse(const float *px, const float *py, const float *pz, const void
   const float sf = static_cast<float>(soft);
   p.eps2 = sf * sf;
   p.half_box = static_cast<float>(box) * 0.5f;
+  p.wave_stats = (flags & COSMO_TRAV_GROUP_STATS) != 0;
   hipStream_t s = as_stream(stream);
   const bool comp = (flags & COSMO_TRAV_COMPENSATED) != 0;
   const bool pairs = (flags & COSMO_TRAV_PAIRS) != 0;

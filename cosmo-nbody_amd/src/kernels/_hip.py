@@ -28,6 +28,7 @@ MAC_AABB = 1
 TRAV_COMPENSATED = 1
 TRAV_STATS = 2
 TRAV_PAIRS = 4
+TRAV_GROUP_STATS = 8
 RENDER_FRAC_BITS = 20
 
 _p = C.c_void_p

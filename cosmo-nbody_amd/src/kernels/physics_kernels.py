@@ -16,8 +16,8 @@ from __future__ import annotations
 
 import torch
 
-from ._hip import (MAC_AABB, MAC_REF, TRAV_COMPENSATED, TRAV_PAIRS, TRAV_STATS, check, kernel,
-                   load, ptr, shared_context, stream_handle)
+from ._hip import (MAC_AABB, MAC_REF, TRAV_COMPENSATED, TRAV_GROUP_STATS, TRAV_PAIRS, TRAV_STATS,
+                   check, kernel, load, ptr, shared_context, stream_handle)
 
 __all__ = ["compute_forces", "integrate", "kick_drift", "traverse", "direct_sum", "MAC_REF",
            "MAC_AABB"]
@@ -54,21 +54,27 @@ def compute_forces(pos_x, pos_y, pos_z, children, n_mass, n_com, n_min, n_max, f
 
 def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0, count=None,
              mac=MAC_REF, theta_group=None, compensated=False, stats=None, n_nodes=None,
-             pairs=False):
+             pairs=False, group_stats=False):
     """Traversal over packed node records; stats: int64[3] tensor accumulating
-    (interactions, node visits, wavefront stack pops) when given.  The targets
-    (pos_*) need not be the tree's particles: n_nodes gives the tree size when
-    they differ.  pairs: two particles per lane (128-particle groups)."""
+    (interactions, node visits, wavefront stack pops) when given; with
+    group_stats it has 3 + 3*W entries and entries 3+3w.. hold wavefront group
+    w's own counts.  The targets (pos_*) need not be the tree's particles:
+    n_nodes gives the tree size when they differ.  pairs: two particles per
+    lane (128-particle groups)."""
     n = pos_x.shape[0]
     n_nodes = 2 * n - 1 if n_nodes is None else int(n_nodes)
     n_tree = (n_nodes + 1) // 2
     count = n - off if count is None else count
     flags = ((TRAV_COMPENSATED if compensated else 0) | (TRAV_STATS if stats is not None else 0) |
-             (TRAV_PAIRS if pairs else 0))
+             (TRAV_PAIRS if pairs else 0) | (TRAV_GROUP_STATS if group_stats else 0))
     groups = (min(n, off + count) - off + 63) // 64
+    waves = (min(n, off + count) - off + (127 if pairs else 63)) // (128 if pairs else 64)
+    if group_stats and stats is None:
+        raise ValueError("group_stats needs a stats tensor")
     tg = None if theta_group is None else ptr(theta_group, _F32, "theta_group",
                                               numel=max(groups, 0))
-    st = None if stats is None else ptr(stats, (torch.int64, torch.uint64), "stats", numel=3)
+    st = None if stats is None else ptr(stats, (torch.int64, torch.uint64), "stats",
+                                        numel=3 + (3 * max(waves, 0) if group_stats else 0))
     check(load().cosmo_traverse(
         *_xyz(pos_x, pos_y, pos_z, n), ptr(nodes, torch.uint8, "nodes"), n_tree,
         ptr(root, torch.int32, "root"), ptr(force, _F32, "force", numel=3 * n), float(theta), tg,

@@ -59,6 +59,7 @@ COMPENSATED = False        # Neumaier-compensated far-field accumulation
 MAC = "ref"                # acceptance test: "ref" (physics_kernels.py:128-133) or "aabb"
 TRAVERSAL_PAIRS = False    # two particles per lane in the traversal (same results, slower at 128^3)
 LEARNED_THETA = False
+ZA_AMPLITUDE = 0.3         # rms Zel'dovich displacement in grid spacings (cosmology.py:49)
 
 
 @dataclasses.dataclass
@@ -86,6 +87,7 @@ class Config:
     SEED: int = SEED
     MORTON_MASKS: str = MORTON_MASKS
     RUN_STEPS: int = RUN_STEPS
+    ZA_AMPLITUDE: float = ZA_AMPLITUDE
     RENDER_FRAC_BITS: int = RENDER_FRAC_BITS
     COMPENSATED: bool = COMPENSATED
     MAC: str = MAC

@@ -54,7 +54,7 @@ def generate_zeldovich_ics(config, seed=None):
     n_grid = config.N_GRID
     seed = config.SEED if seed is None else seed
     d = displacement_field(n_grid, seed)
-    scale = (1.0 / n_grid) * 0.3 / np.std(d[0])
+    scale = (1.0 / n_grid) * getattr(config, "ZA_AMPLITUDE", 0.3) / np.std(d[0])
     d *= scale
     lin = np.linspace(0, 1, n_grid, endpoint=False)
     gx, gy, gz = np.meshgrid(lin, lin, lin, indexing="ij")

@@ -54,6 +54,8 @@ extern "C" {
 #define COSMO_TRAV_STATS 2       /* count interactions / node visits      */
 #define COSMO_TRAV_PAIRS 4       /* two particles per lane (128-particle
                                     groups per wavefront)                 */
+#define COSMO_TRAV_GROUP_STATS 8 /* STATS, plus the counts of each
+                                    wavefront group                       */
 
 /* Fixed-point fraction bits of the density bins used by the drop-ins. */
 #define COSMO_RENDER_FRAC_BITS 20
@@ -179,7 +181,10 @@ int cosmo_normalize_pack(const int32_t *children, const double *n_mass,
  * float[ceil(count/64)]) gives each group's opening angle in place of
  * theta.  stats (nullable, device uint64[3]) accumulates interactions,
  * node visits (per particle) and stack pops (per wavefront) when flags has
- * COSMO_TRAV_STATS. */
+ * COSMO_TRAV_STATS.  With COSMO_TRAV_GROUP_STATS stats holds 3 + 3*W entries
+ * for W = ceil(count / (64 * particles per lane)) wavefront groups: the
+ * totals accumulate in stats[0..2] and group w's own counts are written to
+ * stats[3+3w .. 3+3w+2]. */
 int cosmo_traverse(const float *pos_x, const float *pos_y, const float *pos_z,
                    const void *nodes, int64_t n_tree, const int32_t *root_idx,
                    float *force, double theta, const float *theta_group,

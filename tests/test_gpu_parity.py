@@ -263,6 +263,18 @@ def test_traverse_interaction_counts(orc, torch_mod, ics32, mac):
     assert int(s[0]) == int(ni.sum())
     assert int(s[1]) == int(nv.sum())
     assert rel_rms(host(force), f32) <= 2e-6
+    # per-group counts: each wavefront group's interactions and visits are the
+    # oracle's per-particle counts summed over its 64 particles
+    W = (n + 63) // 64
+    gs = torch.zeros(3 + 3 * W, dtype=torch.int64, device="cuda")
+    traverse(*[dev(p, torch) for p in pos], nodes, root, force, 0.5, 1.0, soft, 1.0, mac=mac,
+             stats=gs, group_stats=True)
+    g = host(gs)
+    per = g[3:].reshape(W, 3)
+    pad = np.zeros(W * 64 - n, dtype=np.int64)
+    assert np.array_equal(per[:, 0], np.concatenate([ni, pad]).reshape(W, 64).sum(1))
+    assert np.array_equal(per[:, 1], np.concatenate([nv, pad]).reshape(W, 64).sum(1))
+    assert np.array_equal(g[:3], s) and int(per[:, 2].sum()) == int(s[2])
 
 
 @pytest.mark.parametrize("mac", [0, 1])
