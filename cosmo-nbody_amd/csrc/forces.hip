@@ -237,13 +237,17 @@ __global__ __launch_bounds__(256) void k_traverse(
         C = D = A;
       }
     }
-    if (A.w <= 0.0f) continue;
+    // massless nodes are skipped (integer test on the uniform bits: m <= 0)
+    if (__float_as_int(A.w) <= 0) continue;
+    // every lane of a popped entry has depth == top, so the reference's
+    // `stack_top + 1 < 64` (physics_kernels.py:143) is uniform here
+    const bool room = top + 1 < 64;
     bool open[PPL], hit[PPL];
 #pragma unroll
     for (int h = 0; h < PPL; ++h) {
       // every lane evaluates the node; only the entry's lanes (in) use it
       Disp d = displace(A.x, A.y, A.z, x[h], y[h], z[h]);
-      bool accept, near_ok = d.d2 > 1e-7f;
+      bool accept, near_ok;
       if constexpr (MAC == COSMO_MAC_REF) {
         // The reference decides in FP64 (physics_kernels.py:122-136).  The
         // FP32 decision is final unless rounding could flip it.  With
@@ -253,14 +257,15 @@ __global__ __launch_bounds__(256) void k_traverse(
         // needs |sx^2 - theta^2 d^2| <= 2E, so theta^2 d^2 <= sx^2 + 35.4 u,
         // |d|_1 <= sqrt(3) d <= 1.7321 sx / theta + 2.52e-3 and
         // 2E <= u (40.9 theta sx + 0.0595 theta^2), which the band below
-        // covers.  Near the 1e-7 cut |d^2 - d_exact^2| <=
-        // 2e-10.  Lanes inside either band redo both tests in FP64 exactly
-        // as the reference does.  A leaf has sx^2 = -1: always accepted,
-        // never inside the band.
+        // covers.  Near the 1e-7 cut (physics_kernels.py:136)
+        // |d^2 - d_exact^2| <= 2e-10.  Lanes inside either band redo both
+        // tests in FP64 exactly as the reference does.  A leaf has
+        // sx^2 = -1: always accepted, never inside the band.
         const float sx2 = B.x;
         const float t = th2[h] * d.d2;
         accept = sx2 < t;
         const float bnd = __builtin_fmaf(k1[h], B.y, k2[h]);
+        near_ok = d.d2 > 1e-7f;
         const bool unsure = in[h] & ((fabsf(d.d2 - 1e-7f) <= 2e-10f) | (fabsf(sx2 - t) <= bnd));
         if (__builtin_expect(unsure, 0)) {
           const double e2 = exact_d2(A.x, A.y, A.z, x[h], y[h], z[h]);
@@ -273,6 +278,7 @@ __global__ __launch_bounds__(256) void k_traverse(
       } else {
         // (s/theta + dl)^2 < d^2, and every particle of the node sits on the
         // same side of the minimum-image boundary as its CoM in each axis
+        near_ok = d.d2 > 1e-7f;
         const float r = __builtin_fmaf(B.x, inv_theta[h], B.y);
         const float hb = p.half_box;
         const bool inside = (d.dx + C.w < hb) && (d.dx - C.x > -hb) &&
@@ -280,8 +286,8 @@ __global__ __launch_bounds__(256) void k_traverse(
                             (d.dz + D.y < hb) && (d.dz - C.z > -hb);
         accept = leaf || ((r * r < d.d2) && inside);
       }
-      hit[h] = in[h] && accept && near_ok;
-      open[h] = in[h] && !accept && (depth[h] + 1 < 64);
+      hit[h] = in[h] & accept & near_ok;
+      open[h] = in[h] & !accept & room;
       if (hit[h]) {
         fix_ties(d, A.x, A.y, A.z, x[h], y[h], z[h]);
         accumulate<COMP>(d, A.w, p.eps2, ax[h], ay[h], az[h], cx[h], cy[h], cz[h]);

@@ -423,7 +423,7 @@ __global__ __launch_bounds__(256) void k_group_features(
     const int32_t *__restrict__ children, const int32_t *__restrict__ parents,
     const double *__restrict__ n_mass, const double *__restrict__ n_com,
     const float *__restrict__ n_min, const float *__restrict__ n_max, int64_t n, float a,
-    float n_grid, float *__restrict__ feat) {
+    const float *__restrict__ a_dev, float n_grid, float *__restrict__ feat) {
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t b0 = g * 64;
   if (b0 >= n) return;
@@ -450,7 +450,7 @@ __global__ __launch_bounds__(256) void k_group_features(
   float *f = feat + g * COSMO_MLP_GROUP_FEATURES;
   f[0] = log2f(fmaxf(ext * n_grid, 1e-6f));
   f[1] = log2f(static_cast<float>(cnt) / (vol * static_cast<float>(n)));
-  f[2] = a;
+  f[2] = a_dev ? a_dev[0] : a;
   // smallest ancestor of the group's first leaf holding >= cnt particles' mass
   const double need = (static_cast<double>(cnt) - 0.5) / static_cast<double>(n);
   int64_t node = b0;
@@ -587,6 +587,23 @@ int cosmo_group_features(const float *px, const float *py, const float *pz,
   const int64_t groups = (n + 63) / 64;
   k_group_features<<<blocks_for(groups, 256), 256, 0, as_stream(stream)>>>(
       px, py, pz, children, parents, n_mass, n_com, n_min, n_max, n, static_cast<float>(a),
+      nullptr, static_cast<float>(n_grid), feat);
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+int cosmo_group_features_dev(const float *px, const float *py, const float *pz,
+                             const int32_t *children, const int32_t *parents,
+                             const double *n_mass, const double *n_com, const float *n_min,
+                             const float *n_max, int64_t n, const float *a, double n_grid,
+                             float *feat, void *stream) {
+  if (n < 1 || n > kMaxBodies) return COSMO_E_INVALID;
+  if (!px || !py || !pz || !children || !parents || !n_mass || !n_com || !n_min || !n_max ||
+      !a || !feat)
+    return COSMO_E_INVALID;
+  const int64_t groups = (n + 63) / 64;
+  k_group_features<<<blocks_for(groups, 256), 256, 0, as_stream(stream)>>>(
+      px, py, pz, children, parents, n_mass, n_com, n_min, n_max, n, 0.0f, a,
       static_cast<float>(n_grid), feat);
   COSMO_CHECK_LAUNCH();
   return COSMO_OK;

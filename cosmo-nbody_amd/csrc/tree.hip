@@ -224,9 +224,57 @@ __global__ void k_top_tree_cache(const int32_t *__restrict__ child,
   while (write_idx < cache_size) top_nodes[write_idx++] = -1;
 }
 
-// Packs node k into its FP32 record (leaf or internal, see cosmo_abi.h);
-// NORMALIZE also divides the FP64 CoM sums by the mass first (normalize_com
-// fused) and writes them back.
+// Writes node k's FP32 traversal record (leaf or internal, see cosmo_abi.h)
+// from its FP64 mass, normalised FP64 CoM and FP32 box.
+template <int MAC>
+__device__ __forceinline__ void write_record(void *__restrict__ out, int64_t n, int64_t k,
+                                             double nm, double c0, double c1, double c2,
+                                             float lo0, float lo1, float lo2, float hi0,
+                                             float hi1, float hi2, int2 ch, float boxf) {
+  const float fx = static_cast<float>(c0), fy = static_cast<float>(c1), fz = static_cast<float>(c2);
+  const float fm = static_cast<float>(nm);
+  if (k < n) {
+    reinterpret_cast<float4 *>(out)[k] = make_float4(fx, fy, fz, fm);
+    return;
+  }
+  char *ibase = reinterpret_cast<char *>(out) + n * sizeof(cosmo_leaf);
+  if (MAC == COSMO_MAC_REF) {
+    // physics_kernels.py:129-130
+    float sx = hi0 - lo0;
+    sx = (sx > boxf * 0.5f) ? boxf : sx;
+    float4 *dst = reinterpret_cast<float4 *>(ibase + (k - n) * sizeof(cosmo_inode_ref));
+    dst[0] = make_float4(fx, fy, fz, fm);
+    dst[1] = make_float4(sx * sx, sx, __int_as_float(ch.x), __int_as_float(ch.y));
+  } else {
+    const float e0 = hi0 - lo0, e1 = hi1 - lo1, e2 = hi2 - lo2;
+    float sz = 0.0f;
+    sz = e0 > sz ? e0 : sz;
+    sz = e1 > sz ? e1 : sz;
+    sz = e2 > sz ? e2 : sz;
+    const float o0 = fx - 0.5f * (hi0 + lo0);
+    const float o1 = fy - 0.5f * (hi1 + lo1);
+    const float o2 = fz - 0.5f * (hi2 + lo2);
+    const float dl = sqrtf(__builtin_fmaf(o2, o2, __builtin_fmaf(o1, o1, o0 * o0)));
+    float4 *dst = reinterpret_cast<float4 *>(ibase + (k - n) * sizeof(cosmo_inode_aabb));
+    dst[0] = make_float4(fx, fy, fz, fm);
+    dst[1] = make_float4(sz, dl, __int_as_float(ch.x), __int_as_float(ch.y));
+    dst[2] = make_float4(fx - lo0, fy - lo1, fz - lo2, hi0 - fx);
+    dst[3] = make_float4(hi1 - fy, hi2 - fz, 0.0f, 0.0f);
+  }
+}
+
+// tree_kernels.py:193-214 on one node: c *= 1/m for m > 0.
+__device__ __forceinline__ void normalize_one(double nm, double &c0, double &c1, double &c2) {
+  if (nm > 0) {
+    const double inv = 1.0 / nm;
+    c0 *= inv;
+    c1 *= inv;
+    c2 *= inv;
+  }
+}
+
+// Packs node k into its FP32 record; NORMALIZE also divides the FP64 CoM
+// sums by the mass first (normalize_com fused) and writes them back.
 template <int MAC, bool NORMALIZE>
 __global__ __launch_bounds__(256) void k_pack_nodes(
     const int32_t *__restrict__ children, const double *__restrict__ n_mass,
@@ -238,47 +286,15 @@ __global__ __launch_bounds__(256) void k_pack_nodes(
   const double nm = n_mass[k];
   double c0 = n_com[3 * k + 0], c1 = n_com[3 * k + 1], c2 = n_com[3 * k + 2];
   if (NORMALIZE && nm > 0) {
-    const double inv = 1.0 / nm;
-    c0 *= inv;
-    c1 *= inv;
-    c2 *= inv;
+    normalize_one(nm, c0, c1, c2);
     n_com[3 * k + 0] = c0;
     n_com[3 * k + 1] = c1;
     n_com[3 * k + 2] = c2;
   }
-  const float fx = static_cast<float>(c0), fy = static_cast<float>(c1), fz = static_cast<float>(c2);
-  const float fm = static_cast<float>(nm);
-  if (k < n) {
-    reinterpret_cast<float4 *>(out)[k] = make_float4(fx, fy, fz, fm);
-    return;
-  }
-  const int2 ch = reinterpret_cast<const int2 *>(children)[k];
-  const float lo0 = n_min[3 * k + 0], lo1 = n_min[3 * k + 1], lo2 = n_min[3 * k + 2];
-  const float hi0 = n_max[3 * k + 0], hi1 = n_max[3 * k + 1], hi2 = n_max[3 * k + 2];
-  char *ibase = reinterpret_cast<char *>(out) + n * sizeof(cosmo_leaf);
-  if (MAC == COSMO_MAC_REF) {
-    // physics_kernels.py:129-130
-    float sx = hi0 - lo0;
-    sx = (sx > boxf * 0.5f) ? boxf : sx;
-    float4 *dst = reinterpret_cast<float4 *>(ibase + (k - n) * sizeof(cosmo_inode_ref));
-    dst[0] = make_float4(fx, fy, fz, fm);
-    dst[1] = make_float4(sx * sx, sx, __int_as_float(ch.x), __int_as_float(ch.y));
-  } else {
-    const float e0 = hi0 - lo0, e1 = hi1 - lo1, e2 = hi2 - lo2;
-    float s = 0.0f;
-    s = e0 > s ? e0 : s;
-    s = e1 > s ? e1 : s;
-    s = e2 > s ? e2 : s;
-    const float o0 = fx - 0.5f * (hi0 + lo0);
-    const float o1 = fy - 0.5f * (hi1 + lo1);
-    const float o2 = fz - 0.5f * (hi2 + lo2);
-    const float dl = sqrtf(__builtin_fmaf(o2, o2, __builtin_fmaf(o1, o1, o0 * o0)));
-    float4 *dst = reinterpret_cast<float4 *>(ibase + (k - n) * sizeof(cosmo_inode_aabb));
-    dst[0] = make_float4(fx, fy, fz, fm);
-    dst[1] = make_float4(s, dl, __int_as_float(ch.x), __int_as_float(ch.y));
-    dst[2] = make_float4(fx - lo0, fy - lo1, fz - lo2, hi0 - fx);
-    dst[3] = make_float4(hi1 - fy, hi2 - fz, 0.0f, 0.0f);
-  }
+  const int2 ch = k < n ? make_int2(-1, -1) : reinterpret_cast<const int2 *>(children)[k];
+  write_record<MAC>(out, n, k, nm, c0, c1, c2, n_min[3 * k + 0], n_min[3 * k + 1],
+                    n_min[3 * k + 2], n_max[3 * k + 0], n_max[3 * k + 1], n_max[3 * k + 2], ch,
+                    boxf);
 }
 
 // ------------------------------------------------------------ validator

@@ -44,6 +44,8 @@ class SimulationEngine:
                                         device=device)
         self.mac = mac_of(config)
         self.theta_group = None     # per-group opening angles (learned theta), or None
+        self.theta_selector = None  # LearnedTheta: recomputes theta_group after each build
+        self._a_dev = None          # device scale factor while a step is being captured
         self.stats = None           # int64[3] interaction / visit / pop counters, or None
         self.pairs = bool(getattr(config, "TRAVERSAL_PAIRS", False))
         self.steps_done = 0
@@ -57,6 +59,8 @@ class SimulationEngine:
         s = self.state
         s.sort_buffer_by_morton(self.config)
         self.tree_manager.build_packed(s.pos, s.mass, s.codes, self.config.BOX_SIZE, self.mac)
+        if self.theta_selector is not None:
+            self.theta_group = self.theta_selector.theta_groups(self, a=self._a_dev)
 
     def compute_forces(self):
         s, c = self.state, self.config
@@ -119,28 +123,34 @@ class SimulationEngine:
             dt = self.da / (a * H) * 0.8
             cd = np.float32(1.0 - H * dt)
             ck = np.float32(dt / a)
-            coefs.append([cd, ck, ck])
+            coefs.append([cd, ck, ck, np.float32(a)])
             mvps.append(get_mvp_matrix(self.steps_done + k, self.camera.box_center, c.BOX_SIZE,
                                        c).reshape(-1))
             a += self.da
         self._coef_table = torch.tensor(np.array(coefs, np.float32), device=dev)
         self._mvp_table = torch.tensor(np.array(mvps, np.float32), device=dev)
         self._step_dev = torch.zeros(1, dtype=torch.int64, device=dev)
-        self._coefs = torch.zeros(3, dtype=torch.float32, device=dev)
+        self._coefs = torch.zeros(4, dtype=torch.float32, device=dev)
         self._mvp = torch.zeros(16, dtype=torch.float32, device=dev)
         self._graph_steps = steps
         self._graph_done = 0
         st = self.state
         _ = st.ctx  # lazily created scratch must exist before capture
+        if self.theta_selector is not None:
+            self.theta_selector.theta_groups(self)  # its buffers too
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
         self._graphs, self._views = [], []
-        for _parity in range(2):
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g, stream=side):
-                self._graph_body()
-            self._graphs.append(g)
-            self._views.append((st.pos, st.vel, st.mass, st._alt))
+        self._a_dev = self._coefs[3:4]  # features read a from the step table
+        try:
+            for _parity in range(2):
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=side):
+                    self._graph_body()
+                self._graphs.append(g)
+                self._views.append((st.pos, st.vel, st.mass, st._alt))
+        finally:
+            self._a_dev = None
         torch.cuda.current_stream().wait_stream(side)
         self._step_dev.zero_()
         self._parity = 0

@@ -19,19 +19,24 @@ _F32, _I16 = torch.float32, torch.int16
 
 
 def group_features(pos, tree, a, n_grid, out=None):
-    """Nine features per group of 64 sorted particles (tree: OctreeManager)."""
+    """Nine features per group of 64 sorted particles (tree: OctreeManager).
+    a: the scale factor, a float or a device float tensor (read at run time,
+    as a captured step needs)."""
     n = pos[0].shape[0]
     G = (n + 63) // 64
     out = torch.empty((G, GROUP_FEATURES), dtype=_F32, device=pos[0].device) if out is None else out
     M = 2 * n - 1
-    check(load().cosmo_group_features(
+    on_dev = isinstance(a, torch.Tensor)
+    fn = load().cosmo_group_features_dev if on_dev else load().cosmo_group_features
+    a_arg = ptr(a, _F32, "a", numel=1) if on_dev else float(a)
+    check(fn(
         *[ptr(t, _F32, "pos", numel=n) for t in pos],
         ptr(tree.child, torch.int32, "children", numel=2 * M),
         ptr(tree.parent, torch.int32, "parents", numel=M),
         ptr(tree.mass, torch.float64, "n_mass", numel=M),
         ptr(tree.com, torch.float64, "n_com", numel=3 * M),
         ptr(tree.aabb_min, _F32, "n_min", numel=3 * M),
-        ptr(tree.aabb_max, _F32, "n_max", numel=3 * M), n, float(a), float(n_grid),
+        ptr(tree.aabb_max, _F32, "n_max", numel=3 * M), n, a_arg, float(n_grid),
         ptr(out, _F32, "feat", numel=G * GROUP_FEATURES), stream_handle()), "group_features")
     return out
 

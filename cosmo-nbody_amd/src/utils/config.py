@@ -12,6 +12,8 @@ a dataclass with the same attribute names plus the knobs this build adds;
   clustered_128       N = 128^3 evolved 300 steps, theta swept (precision study)
   learned_128         N = 128^3 with the learned per-group opening angle
   large_256           N = 256^3, 50 steps, 2048^2 splat, learned theta
+  halos_128           N = 128^3 heterogeneous test state (half in 64 Plummer
+                      spheres, half uniform), AABB acceptance test
 
 Time stepping follows the reference schedule: da = (A_END - A_START) /
 N_STEPS with N_STEPS = 2500 (src/core/engine.py:46); RUN_STEPS is how many of
@@ -60,6 +62,10 @@ MAC = "ref"                # acceptance test: "ref" (physics_kernels.py:128-133)
 TRAVERSAL_PAIRS = False    # two particles per lane in the traversal (same results, slower at 128^3)
 LEARNED_THETA = False
 ZA_AMPLITUDE = 0.3         # rms Zel'dovich displacement in grid spacings (cosmology.py:49)
+IC_KIND = "zeldovich"      # "zeldovich" (reference) or "halos" (heterogeneous test state)
+HALO_FRACTION = 0.5
+N_HALOS = 64
+HALO_RADII = (0.003, 0.03)
 
 
 @dataclasses.dataclass
@@ -88,6 +94,10 @@ class Config:
     MORTON_MASKS: str = MORTON_MASKS
     RUN_STEPS: int = RUN_STEPS
     ZA_AMPLITUDE: float = ZA_AMPLITUDE
+    IC_KIND: str = IC_KIND
+    HALO_FRACTION: float = HALO_FRACTION
+    N_HALOS: int = N_HALOS
+    HALO_RADII: tuple = HALO_RADII
     RENDER_FRAC_BITS: int = RENDER_FRAC_BITS
     COMPENSATED: bool = COMPENSATED
     MAC: str = MAC
@@ -125,6 +135,8 @@ PRESETS = {
     "learned_128": lambda: _plain(128, RUN_STEPS=20, LEARNED_THETA=True, NAME="learned_128"),
     "large_256": lambda: _plain(256, RUN_STEPS=50, RES=2048, LEARNED_THETA=True,
                                 NAME="large_256"),
+    "halos_128": lambda: _plain(128, RUN_STEPS=20, MAC="aabb", IC_KIND="halos",
+                                NAME="halos_128"),
 }
 
 

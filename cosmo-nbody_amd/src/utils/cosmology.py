@@ -63,3 +63,41 @@ def generate_zeldovich_ics(config, seed=None):
     pos = [((g + d[c]) % 1.0).astype(np.float32) for c, g in enumerate(grid)]
     vel = [(d[c] * coupling).astype(np.float32) for c in range(3)]
     return pos, vel
+
+
+def generate_halo_ics(config, seed=None):
+    """A heterogeneous test state with no reference counterpart: a fraction
+    HALO_FRACTION of the particles in N_HALOS Plummer spheres (scale radii
+    log-uniform in HALO_RADII, truncated at 10 scale radii, equal particle
+    counts) at uniform random centres, the rest uniform in the box; zero
+    velocities.  Used where the error/cost trade-off must differ between
+    groups (the reference's Zel'dovich states stay close to uniform)."""
+    n = config.N_GRID ** 3
+    seed = config.SEED if seed is None else seed
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n_halo = int(round(n * config.HALO_FRACTION))
+    k = max(1, int(config.N_HALOS))
+    lo, hi = config.HALO_RADII
+    radii = np.exp(rng.uniform(np.log(lo), np.log(hi), k))
+    centres = rng.uniform(0.0, 1.0, (k, 3))
+    which = np.arange(n_halo) % k
+    u = rng.uniform(1e-6, 1.0, n_halo)
+    r = radii[which] / np.sqrt(u ** (-2.0 / 3.0) - 1.0)
+    r = np.minimum(r, 10.0 * radii[which])
+    v = rng.normal(size=(n_halo, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    halo = centres[which] + v * r[:, None]
+    bg = rng.uniform(0.0, 1.0, (n - n_halo, 3))
+    allp = np.concatenate([halo, bg]) % 1.0
+    allp = allp[rng.permutation(n)]
+    pos = [allp[:, c].astype(np.float32) for c in range(3)]
+    pos = [np.where(p >= 1.0, np.float32(0.0), p) for p in pos]
+    vel = [np.zeros(n, np.float32) for _ in range(3)]
+    return pos, vel
+
+
+def generate_ics(config, seed=None):
+    """Initial conditions of the configured kind (IC_KIND)."""
+    if getattr(config, "IC_KIND", "zeldovich") == "halos":
+        return generate_halo_ics(config, seed)
+    return generate_zeldovich_ics(config, seed)

@@ -46,6 +46,15 @@ int cosmo_group_features(const float *pos_x, const float *pos_y,
                          const float *n_max, int64_t n, double a,
                          double n_grid, float *feat, void *stream);
 
+/* cosmo_group_features with the scale factor read from device memory
+ * (a[0]), for callers that replay a captured step. */
+int cosmo_group_features_dev(const float *pos_x, const float *pos_y,
+                             const float *pos_z, const int32_t *children,
+                             const int32_t *parents, const double *n_mass,
+                             const double *n_com, const float *n_min,
+                             const float *n_max, int64_t n, const float *a,
+                             double n_grid, float *feat, void *stream);
+
 /* X[g*K + k][0..15] = BF16 [thetas[k], feat[g][0..8], 0...]. */
 int cosmo_assemble_rows(const float *feat, int64_t groups,
                         const float *thetas, int K, uint16_t *X, void *stream);

@@ -44,3 +44,34 @@ def test_graph_replays_match_eager(mac):
         assert eager.a == graph.a
     with pytest.raises(RuntimeError):
         graph.step_graph()
+
+
+def test_graph_with_learned_theta_matches_eager():
+    """The learned per-group theta (features from the step's tree and scale
+    factor, MLP, selection) inside the captured step: replays equal eager
+    steps bit for bit, theta_group included."""
+    import torch
+
+    from src.core.learned_theta import DEFAULT_THETAS, LearnedTheta, ThetaModel
+    cost = np.linspace(4000.0, 500.0, len(DEFAULT_THETAS))
+    engines = []
+    for _ in range(2):
+        e = _engine(mac="aabb")
+        e.theta_selector = LearnedTheta(ThetaModel(seed=3), DEFAULT_THETAS, cost, 50.0, 0.5)
+        engines.append(e)
+    eager, graph = engines
+    steps = 4
+    graph.capture_graphs(steps)
+    seen = []
+    for k in range(steps):
+        eager.step()
+        eager.camera.capture(eager.steps_done - 1, *eager.state.pos)
+        graph.step_graph()
+        torch.cuda.synchronize()
+        assert torch.equal(eager.theta_group, graph.theta_group), f"step {k}"
+        for a, b in zip(eager.state.pos + eager.state.vel, graph.state.pos + graph.state.vel):
+            assert torch.equal(a, b), f"step {k}"
+        assert torch.equal(eager.state.force, graph.state.force), f"step {k}"
+        assert torch.equal(eager.camera.grid_d, graph.camera.grid_d), f"step {k}"
+        seen.append(eager.theta_group.clone())
+    assert all(bool(torch.isfinite(t).all()) for t in seen)
