@@ -29,6 +29,8 @@ struct TravParams {
   float eps2;
   float half_box;
   int wave_stats;   // STATS: also store each wavefront's counts
+  const int32_t *order;  // launch order of the groups (nullable)
+  int64_t n_waves;       // number of groups when order is given
 };
 
 // Unit roundoff of FP32.
@@ -165,7 +167,15 @@ __global__ __launch_bounds__(256) void k_traverse(
   const char *__restrict__ ibase =
       reinterpret_cast<const char *>(tree) + int64_t(n_tree) * sizeof(cosmo_leaf);
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  // slot = launch position of this wavefront; wave = the group it takes
+  const int slot = __builtin_amdgcn_readfirstlane(
+      static_cast<int>((int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave));
+  int64_t wave = slot;
+  if (p.order) {
+    if (slot >= p.n_waves) return;
+    wave = p.order[slot];
+    if (wave < 0 || wave >= p.n_waves) return;
+  }
   const int64_t base = lo + wave * (kWave * PPL);
   if (base >= hi) return;  // wave-uniform
   float x[PPL], y[PPL], z[PPL], th2[PPL], k1[PPL], k2[PPL], inv_theta[PPL];
@@ -430,17 +440,52 @@ static void dispatch_traverse2(bool stats_on, bool gt, bool pairs, hipStream_t s
   }
 }
 
+// Launch order for per-group theta: groups by increasing theta (smaller
+// theta, more interactions, earlier), so the longest wavefronts start first
+// and the tail of the launch holds short ones.  64 bins over log2(theta) in
+// [-6, 1); order within a bin is arbitrary (results do not depend on it).
+constexpr int kOrderThreads = 1024;
+constexpr int kOrderBins = 64;
+
+__device__ __forceinline__ int order_bin(float t) {
+  const float b = (log2f(fmaxf(t, 1e-30f)) + 6.0f) * (kOrderBins / 7.0f);
+  return b < 0.0f ? 0 : (b >= kOrderBins - 1 ? kOrderBins - 1 : static_cast<int>(b));
+}
+
+__global__ __launch_bounds__(kOrderThreads) void k_order_groups(const float *__restrict__ tg,
+                                                                int64_t groups,
+                                                                int32_t *__restrict__ order) {
+  __shared__ int hist[kOrderBins];
+  if (threadIdx.x < kOrderBins) hist[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t g = threadIdx.x; g < groups; g += kOrderThreads) atomicAdd(&hist[order_bin(tg[g])], 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int b = 0; b < kOrderBins; ++b) {
+      const int c = hist[b];
+      hist[b] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  for (int64_t g = threadIdx.x; g < groups; g += kOrderThreads)
+    order[atomicAdd(&hist[order_bin(tg[g])], 1)] = static_cast<int32_t>(g);
+}
+
+__host__ __device__ constexpr bool pairs_of(int flags) { return (flags & COSMO_TRAV_PAIRS) != 0; }
+
 }  // namespace cosmo
 
 using namespace cosmo;
 
 extern "C" {
 
-int cosmo_traverse(const float *px, const float *py, const float *pz, const void *nodes,
-                   int64_t n_tree, const int32_t *root_idx, float *force, double theta,
-                   const float *theta_group, double G, double soft, double box, int64_t off,
-                   int64_t count, int64_t n_bodies, int mac, int flags, uint64_t *stats,
-                   void *stream) {
+static int traverse_impl(const float *px, const float *py, const float *pz, const void *nodes,
+                         int64_t n_tree, const int32_t *root_idx, float *force, double theta,
+                         const float *theta_group, double G, double soft, double box,
+                         int64_t off, int64_t count, int64_t n_bodies, int mac, int flags,
+                         uint64_t *stats, const int32_t *order, void *stream) {
   if (n_bodies < 0 || n_bodies > kMaxBodies || off < 0 || count < 0 || n_tree < 1 ||
       n_tree > kMaxBodies)
     return COSMO_E_INVALID;
@@ -462,6 +507,8 @@ int cosmo_traverse(const float *px, const float *py, const float *pz, const void
   p.eps2 = sf * sf;
   p.half_box = static_cast<float>(box) * 0.5f;
   p.wave_stats = (flags & COSMO_TRAV_GROUP_STATS) != 0;
+  p.order = order;
+  p.n_waves = (hi - lo + kWave * (pairs_of(flags) ? 2 : 1) - 1) / (kWave * (pairs_of(flags) ? 2 : 1));
   hipStream_t s = as_stream(stream);
   const bool comp = (flags & COSMO_TRAV_COMPENSATED) != 0;
   const bool pairs = (flags & COSMO_TRAV_PAIRS) != 0;
@@ -477,6 +524,34 @@ int cosmo_traverse(const float *px, const float *py, const float *pz, const void
     else
       dispatch_traverse2<COSMO_MAC_AABB, false>(stats_on, gt, pairs, s, px, py, pz, nodes, static_cast<int>(n_tree), root_idx, force, p, theta_group, lo, hi, stats);
   }
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+int cosmo_traverse(const float *px, const float *py, const float *pz, const void *nodes,
+                   int64_t n_tree, const int32_t *root_idx, float *force, double theta,
+                   const float *theta_group, double G, double soft, double box, int64_t off,
+                   int64_t count, int64_t n_bodies, int mac, int flags, uint64_t *stats,
+                   void *stream) {
+  return traverse_impl(px, py, pz, nodes, n_tree, root_idx, force, theta, theta_group, G, soft,
+                       box, off, count, n_bodies, mac, flags, stats, nullptr, stream);
+}
+
+int cosmo_traverse_ordered(const float *px, const float *py, const float *pz, const void *nodes,
+                           int64_t n_tree, const int32_t *root_idx, float *force, double theta,
+                           const float *theta_group, double G, double soft, double box,
+                           int64_t off, int64_t count, int64_t n_bodies, int mac, int flags,
+                           uint64_t *stats, const int32_t *group_order, void *stream) {
+  if (!group_order || (flags & COSMO_TRAV_PAIRS)) return COSMO_E_INVALID;
+  return traverse_impl(px, py, pz, nodes, n_tree, root_idx, force, theta, theta_group, G, soft,
+                       box, off, count, n_bodies, mac, flags, stats, group_order, stream);
+}
+
+int cosmo_order_groups(const float *theta_group, int64_t groups, int32_t *order, void *stream) {
+  if (groups < 0 || groups > kMaxBodies) return COSMO_E_INVALID;
+  if (groups == 0) return COSMO_OK;
+  if (!theta_group || !order) return COSMO_E_INVALID;
+  k_order_groups<<<1, kOrderThreads, 0, as_stream(stream)>>>(theta_group, groups, order);
   COSMO_CHECK_LAUNCH();
   return COSMO_OK;
 }

@@ -45,6 +45,7 @@ class SimulationEngine:
         self.mac = mac_of(config)
         self.theta_group = None     # per-group opening angles (learned theta), or None
         self.theta_selector = None  # LearnedTheta: recomputes theta_group after each build
+        self.group_order = None     # launch order of the groups under per-group theta
         self._a_dev = None          # device scale factor while a step is being captured
         self.stats = None           # int64[3] interaction / visit / pop counters, or None
         self.pairs = bool(getattr(config, "TRAVERSAL_PAIRS", False))
@@ -61,13 +62,16 @@ class SimulationEngine:
         self.tree_manager.build_packed(s.pos, s.mass, s.codes, self.config.BOX_SIZE, self.mac)
         if self.theta_selector is not None:
             self.theta_group = self.theta_selector.theta_groups(self, a=self._a_dev)
+            self.group_order = self.theta_selector.launch_order(self.theta_group)
 
     def compute_forces(self):
         s, c = self.state, self.config
         traverse(*s.pos, self.tree_manager.nodes, self.tree_manager.root_idx, s.force, c.THETA,
                  c.G, c.SOFTENING, c.BOX_SIZE, mac=self.mac, theta_group=self.theta_group,
                  compensated=bool(getattr(c, "COMPENSATED", False)), stats=self.stats,
-                 pairs=self.pairs)
+                 pairs=self.pairs,
+                 group_order=(self.group_order if self.theta_group is not None and not self.pairs
+                              else None))
 
     def step(self):
         H, dt = self.timestep()

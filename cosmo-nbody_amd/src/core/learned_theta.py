@@ -26,7 +26,7 @@ import numpy as np
 import torch
 
 from src.kernels import mlp_kernels as K
-from src.kernels.physics_kernels import direct_sum, traverse
+from src.kernels.physics_kernels import direct_sum, order_groups, traverse
 
 DEFAULT_THETAS = (0.04, 0.05, 0.06, 0.07, 0.08, 0.1, 0.12, 0.15, 0.2, 0.25, 0.3, 0.4, 0.5, 0.6,
                   0.8, 1.0)
@@ -191,7 +191,8 @@ class LearnedTheta:
             tg = torch.empty(G, dtype=torch.float32, device=device)
             nf = torch.zeros(1, dtype=torch.int64, device=device)
             feat = torch.empty((G, K.GROUP_FEATURES), dtype=torch.float32, device=device)
-            self._dev = (G, th, cost, X, Y, tg, nf, feat)
+            order = torch.empty(G, dtype=torch.int32, device=device)
+            self._dev = (G, th, cost, X, Y, tg, nf, feat, order)
         return self._dev
 
     def theta_groups(self, engine, lam=None, a=None):
@@ -200,12 +201,17 @@ class LearnedTheta:
         captured step).  Allocates nothing after the first call."""
         st, tm, c = engine.state, engine.tree_manager, engine.config
         G = (st.n + 63) // 64
-        _, th, cost, X, Y, tg, nf, feat = self._ensure(st.pos[0].device, G)
+        _, th, cost, X, Y, tg, nf, feat, _ = self._ensure(st.pos[0].device, G)
         K.group_features(st.pos, tm, engine.a if a is None else a, c.N_GRID, feat)
         K.assemble_rows(feat, th, X)
         self.model.predict(X, Y)
         K.select_theta(Y, G, th, cost, self.lam if lam is None else lam, self.fallback, tg, nf)
         return tg
+
+    def launch_order(self, theta_group):
+        """Groups by increasing theta: the traversal's launch order."""
+        order = self._ensure(theta_group.device, theta_group.shape[0])[8]
+        return order_groups(theta_group, order)
 
     @property
     def n_fallback(self):
@@ -222,16 +228,16 @@ class LearnedTheta:
         return cls(model, d["thetas"].tolist(), d["cost"], float(d["lam"]), float(d["fallback"]))
 
 
-def _time_traversal(engine, theta, theta_group, reps=5):
+def _time_traversal(engine, theta, theta_group, reps=5, group_order=None):
     """Average HIP-event time (ms) of the full traversal (all particles)."""
     st, tm, c = engine.state, engine.tree_manager, engine.config
     traverse(*st.pos, tm.nodes, tm.root_idx, st.force, theta, c.G, c.SOFTENING, c.BOX_SIZE,
-             mac=engine.mac, theta_group=theta_group)
+             mac=engine.mac, theta_group=theta_group, group_order=group_order)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(reps):
         traverse(*st.pos, tm.nodes, tm.root_idx, st.force, theta, c.G, c.SOFTENING, c.BOX_SIZE,
-                 mac=engine.mac, theta_group=theta_group)
+                 mac=engine.mac, theta_group=theta_group, group_order=group_order)
     e1.record()
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps
@@ -340,7 +346,8 @@ def experiment(preset="repo_128", target=1e-3, mac="aabb", states=None, n_groups
     tg = lt.theta_groups(eng).clone()
     e_learned, i_learned = probe.rel_rms(0.5, tg.index_select(0, vg).contiguous())
     t_fixed = _time_traversal(eng, theta_star, None)
-    t_learned = _time_traversal(eng, 0.5, tg)
+    t_learned_unordered = _time_traversal(eng, 0.5, tg)
+    t_learned = _time_traversal(eng, 0.5, tg, group_order=lt.launch_order(tg).clone())
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(5):
@@ -354,7 +361,8 @@ def experiment(preset="repo_128", target=1e-3, mac="aabb", states=None, n_groups
            "loss_last": hist[-1][1], "theta_fixed": theta_star, "err_fixed": e_fixed,
            "inter_fixed": i_fixed, "lambda": lt.lam, "err_learned": e_learned,
            "inter_learned": i_learned, "traversal_ms_fixed": t_fixed,
-           "traversal_ms_learned": t_learned, "select_ms": t_select,
+           "traversal_ms_learned": t_learned,
+           "traversal_ms_learned_unordered": t_learned_unordered, "select_ms": t_select,
            "speedup_traversal": t_fixed / t_learned,
            "speedup_with_select": t_fixed / (t_learned + t_select),
            "interaction_ratio": i_fixed / i_learned,

@@ -19,8 +19,8 @@ import torch
 from ._hip import (MAC_AABB, MAC_REF, TRAV_COMPENSATED, TRAV_GROUP_STATS, TRAV_PAIRS, TRAV_STATS,
                    check, kernel, load, ptr, shared_context, stream_handle)
 
-__all__ = ["compute_forces", "integrate", "kick_drift", "traverse", "direct_sum", "MAC_REF",
-           "MAC_AABB"]
+__all__ = ["compute_forces", "integrate", "kick_drift", "traverse", "order_groups", "direct_sum",
+           "MAC_REF", "MAC_AABB"]
 
 _F32 = torch.float32
 
@@ -54,13 +54,14 @@ def compute_forces(pos_x, pos_y, pos_z, children, n_mass, n_com, n_min, n_max, f
 
 def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0, count=None,
              mac=MAC_REF, theta_group=None, compensated=False, stats=None, n_nodes=None,
-             pairs=False, group_stats=False):
+             pairs=False, group_stats=False, group_order=None):
     """Traversal over packed node records; stats: int64[3] tensor accumulating
     (interactions, node visits, wavefront stack pops) when given; with
     group_stats it has 3 + 3*W entries and entries 3+3w.. hold wavefront group
     w's own counts.  The targets (pos_*) need not be the tree's particles:
     n_nodes gives the tree size when they differ.  pairs: two particles per
-    lane (128-particle groups)."""
+    lane (128-particle groups).  group_order: int32 launch order of the
+    64-particle groups (a permutation, e.g. from order_groups)."""
     n = pos_x.shape[0]
     n_nodes = 2 * n - 1 if n_nodes is None else int(n_nodes)
     n_tree = (n_nodes + 1) // 2
@@ -75,11 +76,25 @@ def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0
                                               numel=max(groups, 0))
     st = None if stats is None else ptr(stats, (torch.int64, torch.uint64), "stats",
                                         numel=3 + (3 * max(waves, 0) if group_stats else 0))
-    check(load().cosmo_traverse(
-        *_xyz(pos_x, pos_y, pos_z, n), ptr(nodes, torch.uint8, "nodes"), n_tree,
-        ptr(root, torch.int32, "root"), ptr(force, _F32, "force", numel=3 * n), float(theta), tg,
-        float(G), float(soft), float(box), int(off), int(count), n, int(mac), flags, st,
-        stream_handle()), "traverse")
+    args = (*_xyz(pos_x, pos_y, pos_z, n), ptr(nodes, torch.uint8, "nodes"), n_tree,
+            ptr(root, torch.int32, "root"), ptr(force, _F32, "force", numel=3 * n),
+            float(theta), tg, float(G), float(soft), float(box), int(off), int(count), n,
+            int(mac), flags, st)
+    if group_order is None:
+        check(load().cosmo_traverse(*args, stream_handle()), "traverse")
+    else:
+        order = ptr(group_order, torch.int32, "group_order", numel=max(groups, 0))
+        check(load().cosmo_traverse_ordered(*args, order, stream_handle()), "traverse_ordered")
+
+
+def order_groups(theta_group, out=None):
+    """Launch order of the groups by increasing theta (longest first)."""
+    g = theta_group.shape[0]
+    out = torch.empty(g, dtype=torch.int32, device=theta_group.device) if out is None else out
+    check(load().cosmo_order_groups(ptr(theta_group, _F32, "theta_group"), g,
+                                    ptr(out, torch.int32, "order", numel=g), stream_handle()),
+          "order_groups")
+    return out
 
 
 @kernel

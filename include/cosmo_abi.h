@@ -192,6 +192,25 @@ int cosmo_traverse(const float *pos_x, const float *pos_y, const float *pos_z,
                    int64_t count, int64_t n_bodies, int mac, int flags,
                    uint64_t *stats, void *stream);
 
+/* cosmo_traverse with the wavefront groups launched in the order
+ * group_order[0..W) (a permutation of the W = ceil(count/64) groups; no
+ * reference counterpart).  Results do not depend on the order; a good
+ * order starts the most expensive groups first.  Not with
+ * COSMO_TRAV_PAIRS. */
+int cosmo_traverse_ordered(const float *pos_x, const float *pos_y,
+                           const float *pos_z, const void *nodes, int64_t n_tree,
+                           const int32_t *root_idx, float *force, double theta,
+                           const float *theta_group, double G, double soft,
+                           double box, int64_t off, int64_t count,
+                           int64_t n_bodies, int mac, int flags, uint64_t *stats,
+                           const int32_t *group_order, void *stream);
+
+/* order[0..groups) = the groups sorted by increasing theta_group (64
+ * logarithmic bins; arbitrary order inside a bin): the launch order for
+ * cosmo_traverse_ordered with per-group theta. */
+int cosmo_order_groups(const float *theta_group, int64_t groups, int32_t *order,
+                       void *stream);
+
 /* src/kernels/physics_kernels.py:14-153 compute_forces(pos_x, pos_y, pos_z,
  * children, n_mass, n_com, n_min, n_max, force, theta, G, soft, box, root,
  * top_nodes, node_to_cache, off) for particles [off, off+count): packs the

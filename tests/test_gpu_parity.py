@@ -307,6 +307,37 @@ def test_traverse_pairs_identical(orc, torch_mod, ics32, mac):
         assert res[1][1][2] < res[0][1][2]  # fewer wavefront pops with pairs
 
 
+def test_traverse_launch_order(orc, torch_mod, ics32):
+    """Launching the groups in theta order changes nothing in the results:
+    forces bit for bit, per-group counts; order_groups is a permutation with
+    non-decreasing theta bins."""
+    torch = torch_mod
+    from src.kernels.physics_kernels import order_groups, traverse
+    from src.kernels.tree_kernels import pack_nodes
+    pos, mass, ch, nm, com, lo, hi = _case(orc, ics32)
+    n = len(mass)
+    W = (n + 63) // 64
+    nodes = pack_nodes(dev(ch, torch), dev(nm, torch), dev(com, torch), dev(lo, torch),
+                       dev(hi, torch), 1.0, 1)
+    root = torch.tensor([n], dtype=torch.int32, device="cuda")
+    P = [dev(p, torch) for p in pos]
+    tg = torch.from_numpy(np.random.default_rng(5).choice([0.1, 0.2, 0.3, 0.5, 0.8], W)
+                          .astype(np.float32)).cuda()
+    order = order_groups(tg)
+    o = host(order)
+    assert np.array_equal(np.sort(o), np.arange(W))
+    assert np.all(np.diff(host(tg)[o]) >= 0)
+    res = []
+    for go in (None, order):
+        f = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+        st = torch.zeros(3 + 3 * W, dtype=torch.int64, device="cuda")
+        traverse(*P, nodes, root, f, 0.5, 1.0, 0.02 / 32, 1.0, mac=1, theta_group=tg, stats=st,
+                 group_stats=True, group_order=go)
+        res.append((host(f), host(st)))
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1], res[1][1])
+
+
 def test_traverse_group_theta_and_compensated(orc, torch_mod, ics32):
     torch = torch_mod
     from src.kernels.physics_kernels import traverse
