@@ -102,7 +102,7 @@ def main():
         if ev: ev[0].record()
         st.sort_buffer_by_morton(c)
         if ev: ev[1].record()
-        eng.tree_manager.build_packed(st.pos, st.mass, st.codes, c.BOX_SIZE, eng.mac)
+        eng.tree_manager.build_packed(st.pos, st.mass, st.codes, c.BOX_SIZE, eng.mac, eng.bucket)
         if ev: ev[2].record()
         eng.compute_forces()
         if ev: ev[3].record()

@@ -252,7 +252,7 @@ __global__ __launch_bounds__(256) void k_traverse(
     // every lane of a popped entry has depth == top, so the reference's
     // `stack_top + 1 < 64` (physics_kernels.py:143) is uniform here
     const bool room = top + 1 < 64;
-    bool open[PPL], hit[PPL];
+    bool open[PPL], hit[PPL], rej[PPL];
 #pragma unroll
     for (int h = 0; h < PPL; ++h) {
       // every lane evaluates the node; only the entry's lanes (in) use it
@@ -297,7 +297,8 @@ __global__ __launch_bounds__(256) void k_traverse(
         accept = leaf || ((r * r < d.d2) && inside);
       }
       hit[h] = in[h] & accept & near_ok;
-      open[h] = in[h] & !accept & room;
+      rej[h] = in[h] & !accept;
+      open[h] = rej[h] & room;
       if (hit[h]) {
         fix_ties(d, A.x, A.y, A.z, x[h], y[h], z[h]);
         accumulate<COMP>(d, A.w, p.eps2, ax[h], ay[h], az[h], cx[h], cy[h], cz[h]);
@@ -312,7 +313,43 @@ __global__ __launch_bounds__(256) void k_traverse(
         w_int += __builtin_popcountll(__ballot(hit[h]));
       }
     }
-    if (om != 0) {
+    // AABB records of small nodes carry their leaf range (count > 0): the
+    // lanes that reject such a node sum its leaves directly, in leaf order,
+    // which is the order in which the walk would reach them.  The wavefront
+    // loads the leaf records with one vector load and broadcasts them.
+    int bcount = 0;
+    if constexpr (MAC == COSMO_MAC_AABB) bcount = leaf ? 0 : __float_as_int(D.w);
+    if (bcount > 0) {
+      uint64_t bm = 0;  // lanes that reject the node (stack room is not needed)
+#pragma unroll
+      for (int h = 0; h < PPL; ++h) bm |= __ballot(rej[h]);
+      if (bm != 0) {
+        const int first = __float_as_int(D.z);
+        float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (lane < bcount) q = leaves[first + lane];
+        for (int k = 0; k < bcount; ++k) {
+          const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.x), k));
+          const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.y), k));
+          const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.z), k));
+          const int qm = __builtin_amdgcn_readlane(__float_as_int(q.w), k);
+          if (qm <= 0) continue;  // massless leaves are skipped, as in the walk
+#pragma unroll
+          for (int h = 0; h < PPL; ++h) {
+            Disp d = displace(qx, qy, qz, x[h], y[h], z[h]);
+            const bool hb = rej[h] & (d.d2 > 1e-7f);
+            if (hb) {
+              fix_ties(d, qx, qy, qz, x[h], y[h], z[h]);
+              accumulate<COMP>(d, __int_as_float(qm), p.eps2, ax[h], ay[h], az[h], cx[h],
+                               cy[h], cz[h]);
+            }
+            if (STATS) {
+              w_vis += __builtin_popcountll(__ballot(rej[h]));
+              w_int += __builtin_popcountll(__ballot(hb));
+            }
+          }
+        }
+      }
+    } else if (om != 0) {
       // internal nodes have two children (build_tree): push c2, then c1
       // (physics_kernels.py:143-149)
       stk = write_lane(stk, __float_as_int(B.w), top);

@@ -46,10 +46,14 @@ __global__ __launch_bounds__(256) void k_reset_tree(int32_t *__restrict__ child,
   }
 }
 
-// tree_kernels.py:60-129, one thread per internal node i in [0, n-2].
+// tree_kernels.py:60-129, one thread per internal node i in [0, n-2].  far
+// (nullable) receives j, the other end of node i's leaf range [min(i,j),
+// max(i,j)] (no reference counterpart; the leaf buckets of the packed
+// records are read from it).
 __global__ __launch_bounds__(256) void k_build_tree(const uint64_t *__restrict__ codes,
                                                     int32_t *__restrict__ children,
-                                                    int32_t *__restrict__ parents, int n) {
+                                                    int32_t *__restrict__ parents,
+                                                    int32_t *__restrict__ far, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   const uint64_t ci = codes[i];
@@ -80,6 +84,7 @@ __global__ __launch_bounds__(256) void k_build_tree(const uint64_t *__restrict__
   reinterpret_cast<int2 *>(children)[idx] = make_int2(c0, c1);
   parents[c0] = idx;
   parents[c1] = idx;
+  if (far) far[i] = j;
 }
 
 // Device-coherent (agent-scope) relaxed accesses for data exchanged between
@@ -230,7 +235,8 @@ template <int MAC>
 __device__ __forceinline__ void write_record(void *__restrict__ out, int64_t n, int64_t k,
                                              double nm, double c0, double c1, double c2,
                                              float lo0, float lo1, float lo2, float hi0,
-                                             float hi1, float hi2, int2 ch, float boxf) {
+                                             float hi1, float hi2, int2 ch, float boxf,
+                                             int first, int count) {
   const float fx = static_cast<float>(c0), fy = static_cast<float>(c1), fz = static_cast<float>(c2);
   const float fm = static_cast<float>(nm);
   if (k < n) {
@@ -259,7 +265,7 @@ __device__ __forceinline__ void write_record(void *__restrict__ out, int64_t n, 
     dst[0] = make_float4(fx, fy, fz, fm);
     dst[1] = make_float4(sz, dl, __int_as_float(ch.x), __int_as_float(ch.y));
     dst[2] = make_float4(fx - lo0, fy - lo1, fz - lo2, hi0 - fx);
-    dst[3] = make_float4(hi1 - fy, hi2 - fz, 0.0f, 0.0f);
+    dst[3] = make_float4(hi1 - fy, hi2 - fz, __int_as_float(first), __int_as_float(count));
   }
 }
 
@@ -274,12 +280,16 @@ __device__ __forceinline__ void normalize_one(double nm, double &c0, double &c1,
 }
 
 // Packs node k into its FP32 record; NORMALIZE also divides the FP64 CoM
-// sums by the mass first (normalize_com fused) and writes them back.
+// sums by the mass first (normalize_com fused) and writes them back.  With
+// far (the other end of each internal node's leaf range, from the build)
+// an AABB record of a node with at most `bucket` leaves also holds its leaf
+// range (first, count); count 0 marks every other node.
 template <int MAC, bool NORMALIZE>
 __global__ __launch_bounds__(256) void k_pack_nodes(
     const int32_t *__restrict__ children, const double *__restrict__ n_mass,
     double *__restrict__ n_com, const float *__restrict__ n_min,
-    const float *__restrict__ n_max, int64_t m, float boxf, void *__restrict__ out) {
+    const float *__restrict__ n_max, int64_t m, float boxf, void *__restrict__ out,
+    const int32_t *__restrict__ far, int bucket) {
   const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k >= m) return;
   const int64_t n = (m + 1) / 2;
@@ -292,9 +302,18 @@ __global__ __launch_bounds__(256) void k_pack_nodes(
     n_com[3 * k + 2] = c2;
   }
   const int2 ch = k < n ? make_int2(-1, -1) : reinterpret_cast<const int2 *>(children)[k];
+  int first = 0, count = 0;
+  if (far && k >= n) {
+    const int i = static_cast<int>(k - n), j = far[k - n];
+    const int c = (i < j ? j - i : i - j) + 1;
+    if (c <= bucket) {
+      first = i < j ? i : j;
+      count = c;
+    }
+  }
   write_record<MAC>(out, n, k, nm, c0, c1, c2, n_min[3 * k + 0], n_min[3 * k + 1],
                     n_min[3 * k + 2], n_max[3 * k + 0], n_max[3 * k + 1], n_max[3 * k + 2], ch,
-                    boxf);
+                    boxf, first, count);
 }
 
 // ------------------------------------------------------------ validator
@@ -383,15 +402,26 @@ int cosmo_reset_tree_buffers(int32_t *child, int32_t *parent, int32_t *counters,
   return COSMO_OK;
 }
 
-int cosmo_build_tree(const uint64_t *codes, int32_t *children, int32_t *parents, int64_t n,
-                     void *stream) {
+static int build_tree_impl(const uint64_t *codes, int32_t *children, int32_t *parents,
+                           int32_t *far, int64_t n, void *stream) {
   if (n < 0 || n > kMaxBodies) return COSMO_E_INVALID;
   if (n < 2) return COSMO_OK;
   if (!codes || !children || !parents) return COSMO_E_INVALID;
   k_build_tree<<<blocks_for(n - 1, 256), 256, 0, as_stream(stream)>>>(codes, children, parents,
-                                                                       static_cast<int>(n));
+                                                                       far, static_cast<int>(n));
   COSMO_CHECK_LAUNCH();
   return COSMO_OK;
+}
+
+int cosmo_build_tree(const uint64_t *codes, int32_t *children, int32_t *parents, int64_t n,
+                     void *stream) {
+  return build_tree_impl(codes, children, parents, nullptr, n, stream);
+}
+
+int cosmo_build_tree_spans(const uint64_t *codes, int32_t *children, int32_t *parents,
+                           int32_t *far, int64_t n, void *stream) {
+  if (n >= 2 && !far) return COSMO_E_INVALID;
+  return build_tree_impl(codes, children, parents, far, n, stream);
 }
 
 int cosmo_compute_multipoles(const float *px, const float *py, const float *pz,
@@ -445,21 +475,24 @@ int cosmo_build_top_tree_cache(const int32_t *child, const int32_t *root_idx,
 
 static int pack_impl(const int32_t *children, const double *n_mass, double *n_com,
                      const float *n_min, const float *n_max, int64_t m, double box, int mac,
-                     bool normalize, void *nodes, void *stream) {
+                     bool normalize, const int32_t *far, int bucket, void *nodes, void *stream) {
   if (m < 0 || m > 2 * kMaxBodies || (m > 0 && (m & 1) == 0)) return COSMO_E_INVALID;
+  if (bucket < 0 || bucket > kMaxBucket || (bucket > 0 && (mac != COSMO_MAC_AABB || !far)))
+    return COSMO_E_INVALID;
   if (m == 0) return COSMO_OK;
   if (!children || !n_mass || !n_com || !n_min || !n_max || !nodes) return COSMO_E_INVALID;
   const unsigned g = blocks_for(m, 256);
   hipStream_t s = as_stream(stream);
   const float boxf = static_cast<float>(box);
+  if (bucket == 0) far = nullptr;
   if (mac == COSMO_MAC_REF && normalize)
-    k_pack_nodes<COSMO_MAC_REF, true><<<g, 256, 0, s>>>(children, n_mass, n_com, n_min, n_max, m, boxf, nodes);
+    k_pack_nodes<COSMO_MAC_REF, true><<<g, 256, 0, s>>>(children, n_mass, n_com, n_min, n_max, m, boxf, nodes, far, bucket);
   else if (mac == COSMO_MAC_REF)
-    k_pack_nodes<COSMO_MAC_REF, false><<<g, 256, 0, s>>>(children, n_mass, n_com, n_min, n_max, m, boxf, nodes);
+    k_pack_nodes<COSMO_MAC_REF, false><<<g, 256, 0, s>>>(children, n_mass, n_com, n_min, n_max, m, boxf, nodes, far, bucket);
   else if (mac == COSMO_MAC_AABB && normalize)
-    k_pack_nodes<COSMO_MAC_AABB, true><<<g, 256, 0, s>>>(children, n_mass, n_com, n_min, n_max, m, boxf, nodes);
+    k_pack_nodes<COSMO_MAC_AABB, true><<<g, 256, 0, s>>>(children, n_mass, n_com, n_min, n_max, m, boxf, nodes, far, bucket);
   else if (mac == COSMO_MAC_AABB)
-    k_pack_nodes<COSMO_MAC_AABB, false><<<g, 256, 0, s>>>(children, n_mass, n_com, n_min, n_max, m, boxf, nodes);
+    k_pack_nodes<COSMO_MAC_AABB, false><<<g, 256, 0, s>>>(children, n_mass, n_com, n_min, n_max, m, boxf, nodes, far, bucket);
   else
     return COSMO_E_INVALID;
   COSMO_CHECK_LAUNCH();
@@ -470,14 +503,22 @@ int cosmo_pack_nodes(const int32_t *children, const double *n_mass, const double
                      const float *n_min, const float *n_max, int64_t n_nodes, double box,
                      int mac, void *nodes, void *stream) {
   return pack_impl(children, n_mass, const_cast<double *>(n_com), n_min, n_max, n_nodes, box,
-                   mac, false, nodes, stream);
+                   mac, false, nullptr, 0, nodes, stream);
 }
 
 int cosmo_normalize_pack(const int32_t *children, const double *n_mass, double *n_com,
                          const float *n_min, const float *n_max, int64_t n_nodes, double box,
                          int mac, void *nodes, void *stream) {
-  return pack_impl(children, n_mass, n_com, n_min, n_max, n_nodes, box, mac, true, nodes,
-                   stream);
+  return pack_impl(children, n_mass, n_com, n_min, n_max, n_nodes, box, mac, true, nullptr, 0,
+                   nodes, stream);
+}
+
+int cosmo_normalize_pack_buckets(const int32_t *children, const int32_t *far,
+                                 const double *n_mass, double *n_com, const float *n_min,
+                                 const float *n_max, int64_t n_nodes, double box, int mac,
+                                 int bucket, void *nodes, void *stream) {
+  return pack_impl(children, n_mass, n_com, n_min, n_max, n_nodes, box, mac, true, far, bucket,
+                   nodes, stream);
 }
 
 int cosmo_validate_tree(cosmo_ctx *ctx, const int32_t *children, const int32_t *parents,

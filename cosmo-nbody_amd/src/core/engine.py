@@ -43,6 +43,8 @@ class SimulationEngine:
         self.node_to_cache = torch.full((self.tree_manager.n_nodes,), -1, dtype=torch.int32,
                                         device=device)
         self.mac = mac_of(config)
+        # leaf buckets of the AABB records (0 = walk every leaf)
+        self.bucket = int(getattr(config, "BUCKET", 0)) if self.mac == MAC_AABB else 0
         self.theta_group = None     # per-group opening angles (learned theta), or None
         self.theta_selector = None  # LearnedTheta: recomputes theta_group after each build
         self.group_order = None     # launch order of the groups under per-group theta
@@ -59,7 +61,8 @@ class SimulationEngine:
     def build_tree(self):
         s = self.state
         s.sort_buffer_by_morton(self.config)
-        self.tree_manager.build_packed(s.pos, s.mass, s.codes, self.config.BOX_SIZE, self.mac)
+        self.tree_manager.build_packed(s.pos, s.mass, s.codes, self.config.BOX_SIZE, self.mac,
+                                       self.bucket)
         if self.theta_selector is not None:
             self.theta_group = self.theta_selector.theta_groups(self, a=self._a_dev)
             self.group_order = self.theta_selector.launch_order(self.theta_group)

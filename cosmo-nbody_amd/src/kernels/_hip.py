@@ -50,12 +50,14 @@ SIGNATURES = {
     "cosmo_permute": ([_p, _int, C.POINTER(_p), C.POINTER(_p), _i64, _p], _int),
     "cosmo_reset_tree_buffers": ([_p, _p, _p, _p, _i64, _p], _int),
     "cosmo_build_tree": ([_p, _p, _p, _i64, _p], _int),
+    "cosmo_build_tree_spans": ([_p, _p, _p, _p, _i64, _p], _int),
     "cosmo_compute_multipoles": ([_p] * 11 + [_i64, _p], _int),
     "cosmo_normalize_com": ([_p, _p, _i64, _p], _int),
     "cosmo_find_root": ([_p, _p, _i64, _p], _int),
     "cosmo_build_top_tree_cache": ([_p, _p, _p, _p, _i64, _p], _int),
     "cosmo_pack_nodes": ([_p] * 5 + [_i64, _f64, _int, _p, _p], _int),
     "cosmo_normalize_pack": ([_p] * 5 + [_i64, _f64, _int, _p, _p], _int),
+    "cosmo_normalize_pack_buckets": ([_p] * 6 + [_i64, _f64, _int, _int, _p, _p], _int),
     "cosmo_traverse": ([_p] * 4 + [_i64, _p, _p, _f64, _p, _f64, _f64, _f64, _i64, _i64, _i64,
                                     _int, _int, _p, _p], _int),
     "cosmo_traverse_ordered": ([_p] * 4 + [_i64, _p, _p, _f64, _p, _f64, _f64, _f64, _i64, _i64,

@@ -18,7 +18,9 @@ launch that covers every node, as all of the reference's callers use.
 
 Additions without a reference counterpart: pack_nodes / normalize_pack (the
 packed FP32 leaf and internal-node records the traversal reads, a uint8
-buffer of tree_bytes(n_nodes, mac) bytes) and validate_tree.
+buffer of tree_bytes(n_nodes, mac) bytes), build_tree_spans and
+normalize_pack_buckets (AABB records of small nodes carry their leaf range,
+which the traversal sums directly when it opens them), and validate_tree.
 """
 from __future__ import annotations
 
@@ -27,9 +29,9 @@ import torch
 from ._hip import (MAC_AABB, MAC_REF, U32, U64, check, kernel, launch_extent, load, ptr,
                    shared_context, stream_handle)
 
-__all__ = ["reset_tree_buffers", "build_tree", "compute_multipoles", "normalize_com",
-           "find_root", "build_top_tree_cache", "pack_nodes", "normalize_pack",
-           "validate_tree", "node_buffer", "MAC_REF", "MAC_AABB"]
+__all__ = ["reset_tree_buffers", "build_tree", "build_tree_spans", "compute_multipoles",
+           "normalize_com", "find_root", "build_top_tree_cache", "pack_nodes", "normalize_pack",
+           "normalize_pack_buckets", "validate_tree", "node_buffer", "MAC_REF", "MAC_AABB"]
 
 
 def _full(launch, need, name):
@@ -55,6 +57,18 @@ def build_tree(codes, children, parents, _launch=None):
     check(load().cosmo_build_tree(
         ptr(codes, U64, "codes"), ptr(children, torch.int32, "children", numel=2 * m),
         ptr(parents, torch.int32, "parents", numel=m), n, stream_handle()), "build_tree")
+
+
+def build_tree_spans(codes, children, parents, far):
+    """build_tree that also writes far[i] (int32[n-1]): the other end of
+    internal node n+i's leaf range."""
+    n = codes.shape[0]
+    m = 2 * n - 1
+    check(load().cosmo_build_tree_spans(
+        ptr(codes, U64, "codes"), ptr(children, torch.int32, "children", numel=2 * m),
+        ptr(parents, torch.int32, "parents", numel=m),
+        ptr(far, torch.int32, "far", numel=max(n - 1, 0)), n, stream_handle()),
+        "build_tree_spans")
 
 
 @kernel
@@ -134,6 +148,24 @@ def normalize_pack(children, n_mass, n_com, n_min, n_max, box, mac=MAC_REF, node
     nodes = node_buffer(children.shape[0], children.device, mac) if nodes is None else nodes
     return _pack(load().cosmo_normalize_pack, children, n_mass, n_com, n_min, n_max, box, mac,
                  nodes, "normalize_pack")
+
+
+def normalize_pack_buckets(children, far, n_mass, n_com, n_min, n_max, box, bucket, nodes=None):
+    """normalize_pack for the AABB test; nodes of at most `bucket` leaves
+    (1..64) also carry their leaf range (from build_tree_spans' far)."""
+    m = children.shape[0]
+    n = (m + 1) // 2
+    nodes = node_buffer(m, children.device, MAC_AABB) if nodes is None else nodes
+    check(load().cosmo_normalize_pack_buckets(
+        ptr(children, torch.int32, "children", numel=2 * m),
+        ptr(far, torch.int32, "far", numel=max(n - 1, 0)),
+        ptr(n_mass, torch.float64, "n_mass", numel=m),
+        ptr(n_com, torch.float64, "n_com", numel=3 * m),
+        ptr(n_min, torch.float32, "n_min", numel=3 * m),
+        ptr(n_max, torch.float32, "n_max", numel=3 * m), m, float(box), MAC_AABB, int(bucket),
+        ptr(nodes, torch.uint8, "nodes", numel=tree_bytes(m, MAC_AABB)), stream_handle()),
+        "normalize_pack_buckets")
+    return nodes
 
 
 def validate_tree(children, parents, n: int, ctx=None) -> int:

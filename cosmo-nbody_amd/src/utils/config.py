@@ -59,6 +59,7 @@ RUN_STEPS = N_STEPS
 RENDER_FRAC_BITS = 20      # fixed-point fraction bits of the density bins
 COMPENSATED = False        # Neumaier-compensated far-field accumulation
 MAC = "ref"                # acceptance test: "ref" (physics_kernels.py:128-133) or "aabb"
+BUCKET = 0                 # AABB test: nodes of <= BUCKET leaves are summed leaf by leaf when opened
 TRAVERSAL_PAIRS = False    # two particles per lane in the traversal (same results, slower at 128^3)
 LEARNED_THETA = False
 ZA_AMPLITUDE = 0.3         # rms Zel'dovich displacement in grid spacings (cosmology.py:49)
@@ -101,6 +102,7 @@ class Config:
     RENDER_FRAC_BITS: int = RENDER_FRAC_BITS
     COMPENSATED: bool = COMPENSATED
     MAC: str = MAC
+    BUCKET: int = BUCKET
     TRAVERSAL_PAIRS: bool = TRAVERSAL_PAIRS
     LEARNED_THETA: bool = LEARNED_THETA
     NAME: str = "reference_defaults"

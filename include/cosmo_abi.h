@@ -68,7 +68,12 @@ extern "C" {
  *          > box/2 (physics_kernels.py:129-130), and sx2 = sx*sx rounded
  *          to FP32;
  *     AABB cosmo_inode_aabb (64 B): s = largest extent of the node's box,
- *          dl = |CoM - box centre|, lo[k] = CoM_k - min_k, hi[k] = max_k - CoM_k.
+ *          dl = |CoM - box centre|, lo[k] = CoM_k - min_k, hi[k] = max_k - CoM_k,
+ *          and, for a node of at most `bucket` leaves packed by
+ *          cosmo_normalize_pack_buckets, its leaf range [first,
+ *          first + count) (count = 0 otherwise).  A lane that opens such a
+ *          node sums the leaves of the range directly, in leaf order, instead
+ *          of walking the node's subtree.
  * Leaves are the node ids below n, as build_tree produces them (a leaf is
  * what the reference recognises by children == -1). */
 typedef struct cosmo_leaf {
@@ -86,7 +91,7 @@ typedef struct cosmo_inode_aabb {
   float s, dl;
   int32_t c1, c2;
   float lo[3], hi[3];
-  float pad[2];
+  int32_t first, count;
 } cosmo_inode_aabb;
 
 /* Size in bytes of the packed records of an n-leaf tree for test `mac`. */
@@ -133,6 +138,13 @@ int cosmo_reset_tree_buffers(int32_t *child, int32_t *parent,
 int cosmo_build_tree(const uint64_t *codes, int32_t *children,
                      int32_t *parents, int64_t n, void *stream);
 
+/* cosmo_build_tree that also writes far[i] (int32[n-1]) = j, the other end
+ * of internal node n+i's leaf range [min(i,j), max(i,j)] (no reference
+ * counterpart: the leaf buckets of cosmo_normalize_pack_buckets read it). */
+int cosmo_build_tree_spans(const uint64_t *codes, int32_t *children,
+                           int32_t *parents, int32_t *far, int64_t n,
+                           void *stream);
+
 /* src/kernels/tree_kernels.py:131-191 compute_multipoles(pos_x, pos_y, pos_z,
  * mass, children, parents, n_mass, n_com, n_min, n_max, counts); FP64 mass
  * and mass-weighted CoM sums.  counts must be zero (reset_tree_buffers). */
@@ -172,6 +184,15 @@ int cosmo_normalize_pack(const int32_t *children, const double *n_mass,
                          double *n_com, const float *n_min, const float *n_max,
                          int64_t n_nodes, double box, int mac,
                          void *nodes, void *stream);
+
+/* cosmo_normalize_pack for the AABB test with leaf buckets (no reference
+ * counterpart): internal nodes of at most `bucket` (1..64; 0 = none) leaves
+ * carry their leaf range, read from far (cosmo_build_tree_spans). */
+int cosmo_normalize_pack_buckets(const int32_t *children, const int32_t *far,
+                                 const double *n_mass, double *n_com,
+                                 const float *n_min, const float *n_max,
+                                 int64_t n_nodes, double box, int mac,
+                                 int bucket, void *nodes, void *stream);
 
 /* Barnes-Hut traversal over the packed records of an n_tree-leaf tree for
  * the particles [off, off+count) ∩ [0, n_bodies) of pos_* (the tree's own

@@ -327,6 +327,50 @@ int orc_validate_tree(const int32_t *children, const int32_t *parents,
  *   sits across the minimum-image boundary from its CoM.
  * Leaves are always accepted; interactions with d^2 <= 1e-7 are dropped. */
 
+/* Leaf range [first, first + count) of every node, from the children
+ * array alone: pre-order from the root, then parents from children in
+ * reverse pre-order.  Returns 0 on success, -1 if the walk leaves the tree. */
+static int leaf_spans(const int32_t *children, int64_t n, int32_t root,
+                      int32_t *first, int32_t *count) {
+  const int64_t M = 2 * n - 1;
+  int32_t *order = (int32_t *)malloc(sizeof(int32_t) * (size_t)M);
+  int32_t *stk = (int32_t *)malloc(sizeof(int32_t) * (size_t)(M + 1));
+  int64_t k = 0, top = 0;
+  int bad = 0;
+  for (int64_t i = 0; i < M; ++i) {
+    first[i] = (i < n) ? (int32_t)i : 0;
+    count[i] = (i < n) ? 1 : 0;
+  }
+  stk[top++] = root;
+  while (top > 0 && !bad) {
+    const int32_t v = stk[--top];
+    if (v < 0 || v >= M || k >= M) {
+      bad = 1;
+      break;
+    }
+    order[k++] = v;
+    if (v >= n) {
+      stk[top++] = children[2 * v];
+      stk[top++] = children[2 * v + 1];
+    }
+  }
+  for (int64_t idx = k - 1; idx >= 0 && !bad; --idx) {
+    const int32_t v = order[idx];
+    if (v < n) continue;
+    const int32_t a = children[2 * v], b = children[2 * v + 1];
+    first[v] = first[a] < first[b] ? first[a] : first[b];
+    count[v] = count[a] + count[b];
+  }
+  free(order);
+  free(stk);
+  return bad ? -1 : 0;
+}
+
+/* Bucket rule (AABB test, bucket > 0; no reference counterpart): a rejected
+ * internal node of at most `bucket` leaves is not opened; its leaves
+ * first..first+count-1 are visited in that order as if each had been
+ * popped (massless ones skipped), which is the order the walk reaches them. */
+
 /* physics_kernels.py:79-153, one particle, FP64 arithmetic on FP32 data
  * (positions, node CoM/mass/AABB), as Numba promotes them. */
 static void bh_one_f64(const float *px, const float *py, const float *pz,
@@ -334,6 +378,7 @@ static void bh_one_f64(const float *px, const float *py, const float *pz,
                        const float *n_com, const float *n_min,
                        const float *n_max, int mac, double theta, double G,
                        double soft, double box, int32_t root, int64_t i,
+                       const int32_t *bfirst, const int32_t *bcount, int bucket,
                        double out[3], int64_t *n_int, int64_t *n_vis) {
   const double x = px[i], y = py[i], z = pz[i];
   double ax = 0.0, ay = 0.0, az = 0.0;
@@ -389,6 +434,27 @@ static void bh_one_f64(const float *px, const float *py, const float *pz,
         az += f * dz;
         ni++;
       }
+    } else if (bucket > 0 && bcount[node] <= bucket) {
+      for (int32_t j = bfirst[node]; j < bfirst[node] + bcount[node]; ++j) {
+        const double qm = n_mass[j];
+        if (qm <= 0) continue;
+        double qx = (double)n_com[3 * j + 0] - x;
+        double qy = (double)n_com[3 * j + 1] - y;
+        double qz = (double)n_com[3 * j + 2] - z;
+        qx -= nearbyint(qx);
+        qy -= nearbyint(qy);
+        qz -= nearbyint(qz);
+        const double q2 = qx * qx + qy * qy + qz * qz;
+        nv++;
+        if (q2 > 1e-7) {
+          const double d_inv = 1.0 / sqrt(q2 + soft * soft);
+          const double f = G * qm * (d_inv * d_inv * d_inv);
+          ax += f * qx;
+          ay += f * qy;
+          az += f * qz;
+          ni++;
+        }
+      }
     } else if (top + 1 < 64) {
       if (c1 != -1) stack[top++] = c2;
       if (c2 != -1) stack[top++] = c1;
@@ -440,7 +506,8 @@ static void bh_one_f32(const float *px, const float *py, const float *pz,
                        const float *n_com, const float *n_min,
                        const float *n_max, int mac, float thf, double th2d,
                        float Gf, float softf, float boxf, int32_t root,
-                       int64_t i, int compensated, double out[3],
+                       int64_t i, int compensated, const int32_t *bfirst,
+                       const int32_t *bcount, int bucket, double out[3],
                        int64_t *n_int, int64_t *n_vis) {
   const float x = px[i], y = py[i], z = pz[i];
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
@@ -509,6 +576,30 @@ static void bh_one_f32(const float *px, const float *py, const float *pz,
         }
         ni++;
       }
+    } else if (bucket > 0 && bcount[node] <= bucket) {
+      for (int32_t j = bfirst[node]; j < bfirst[node] + bcount[node]; ++j) {
+        const float qm = n_mass[j];
+        if (qm <= 0.0f) continue;
+        const float qx = min_image_f32(n_com[3 * j + 0], x);
+        const float qy = min_image_f32(n_com[3 * j + 1], y);
+        const float qz = min_image_f32(n_com[3 * j + 2], z);
+        const float q2 = fmaf(qz, qz, fmaf(qy, qy, qx * qx));
+        nv++;
+        if (q2 > 1e-7f) {
+          const float inv = 1.0f / sqrtf(q2 + eps2);
+          const float f = (Gf * qm) * ((inv * inv) * inv);
+          if (compensated) {
+            neumaier_add(&ax, &cx, f * qx);
+            neumaier_add(&ay, &cy, f * qy);
+            neumaier_add(&az, &cz, f * qz);
+          } else {
+            ax = fmaf(f, qx, ax);
+            ay = fmaf(f, qy, ay);
+            az = fmaf(f, qz, az);
+          }
+          ni++;
+        }
+      }
     } else if (top + 1 < 64) {
       if (c1 != -1) stack[top++] = c2;
       if (c2 != -1) stack[top++] = c1;
@@ -527,10 +618,17 @@ void orc_compute_forces(const float *px, const float *py, const float *pz,
                         const float *n_max, float *force, double theta,
                         double G, double soft, double box, int32_t root,
                         int64_t off, int64_t count, int64_t n, int precision,
-                        int mac, const float *theta_g, int64_t *n_inter,
-                        int64_t *n_visit, double *force64) {
+                        int mac, const float *theta_g, int bucket,
+                        int64_t *n_inter, int64_t *n_visit, double *force64) {
   int64_t hi = off + count;
   if (hi > n) hi = n;
+  int32_t *bfirst = NULL, *bcount = NULL;
+  if (mac != ORC_MAC_AABB) bucket = 0;
+  if (bucket > 0) {
+    bfirst = (int32_t *)malloc(sizeof(int32_t) * (size_t)(2 * n - 1));
+    bcount = (int32_t *)malloc(sizeof(int32_t) * (size_t)(2 * n - 1));
+    if (leaf_spans(children, n, root, bfirst, bcount) != 0) bucket = 0;
+  }
 #pragma omp parallel for schedule(dynamic, 64)
   for (int64_t i = off; i < hi; ++i) {
     double a[3];
@@ -538,13 +636,13 @@ void orc_compute_forces(const float *px, const float *py, const float *pz,
     if (precision == 64) {
       const double th = theta_g ? (double)theta_g[i] : theta;
       bh_one_f64(px, py, pz, children, n_mass, n_com, n_min, n_max, mac, th, G,
-                 soft, box, root, i, a, &ni, &nv);
+                 soft, box, root, i, bfirst, bcount, bucket, a, &ni, &nv);
     } else {
       const float th = theta_g ? theta_g[i] : (float)theta;
       const double th2d = theta_g ? (double)th * (double)th : theta * theta;
       bh_one_f32(px, py, pz, children, n_mass, n_com, n_min, n_max, mac, th,
                  th2d, (float)G, (float)soft, (float)box, root, i,
-                 precision == 31, a, &ni, &nv);
+                 precision == 31, bfirst, bcount, bucket, a, &ni, &nv);
     }
     force[3 * i + 0] = (float)a[0];
     force[3 * i + 1] = (float)a[1];
@@ -557,6 +655,8 @@ void orc_compute_forces(const float *px, const float *py, const float *pz,
     if (n_inter) n_inter[i] = ni;
     if (n_visit) n_visit[i] = nv;
   }
+  free(bfirst);
+  free(bcount);
 }
 
 void orc_direct_sum(const float *px, const float *py, const float *pz,

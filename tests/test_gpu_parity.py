@@ -277,6 +277,73 @@ def test_traverse_interaction_counts(orc, torch_mod, ics32, mac):
     assert np.array_equal(g[:3], s) and int(per[:, 2].sum()) == int(s[2])
 
 
+def _leaf_ranges(ch, n):
+    """(first, last) leaf of every internal node from the children array."""
+    first = np.arange(2 * n - 1, dtype=np.int64)
+    last = first.copy()
+    order, stack = [], [n]
+    while stack:
+        v = stack.pop()
+        order.append(v)
+        if v >= n:
+            stack.extend(ch[v])
+    for v in reversed(order):
+        if v >= n:
+            a, b = ch[v]
+            first[v], last[v] = min(first[a], first[b]), max(last[a], last[b])
+    return first[n:], last[n:]
+
+
+@pytest.mark.parametrize("bucket", [2, 16, 64])
+def test_traverse_buckets(orc, torch_mod, ics32, bucket):
+    """AABB records with leaf buckets: the build's leaf ranges match the
+    tree's; the traversal's per-particle interaction and visit counts equal
+    the FP32 oracle's bucket rule (totals and per group, fixed and per-group
+    theta) and the forces agree to FP32 rounding."""
+    torch = torch_mod
+    from src.kernels.physics_kernels import traverse
+    from src.kernels.tree_kernels import build_tree_spans, normalize_pack_buckets
+    pos, mass, ch, nm, com, lo, hi = _case(orc, ics32)
+    _, codes = sorted_case(orc, ics32)
+    n = len(mass)
+    soft = 0.02 / 32
+    m = 2 * n - 1
+    children = torch.full((m, 2), -1, dtype=torch.int32, device="cuda")
+    parents = torch.full((m,), -1, dtype=torch.int32, device="cuda")
+    far = torch.zeros(n - 1, dtype=torch.int32, device="cuda")
+    build_tree_spans(dev(codes, torch), children, parents, far)
+    assert np.array_equal(host(children), ch)
+    f_ = host(far).astype(np.int64)
+    i = np.arange(n - 1)
+    first, last = _leaf_ranges(ch, n)
+    assert np.array_equal(np.minimum(i, f_), first) and np.array_equal(np.maximum(i, f_), last)
+    # the bucket pack normalises the CoM sums itself
+    _, csum, _, _ = orc.compute_multipoles(*pos, mass, ch, orc.build_tree(codes)[1])
+    nodes = normalize_pack_buckets(children, far, dev(nm, torch), dev(csum, torch),
+                                   dev(lo, torch), dev(hi, torch), 1.0, bucket)
+    root = torch.tensor([n], dtype=torch.int32, device="cuda")
+    P = [dev(p, torch) for p in pos]
+    W = (n + 63) // 64
+    tg = np.random.default_rng(6).uniform(0.1, 0.9, W).astype(np.float32)
+    for theta, group_theta in ((0.3, None), (0.7, None), (0.5, tg)):
+        force = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+        gs = torch.zeros(3 + 3 * W, dtype=torch.int64, device="cuda")
+        traverse(*P, nodes, root, force, theta, 1.0, soft, 1.0, mac=1, stats=gs,
+                 group_stats=True,
+                 theta_group=None if group_theta is None else dev(group_theta, torch))
+        per_particle = None if group_theta is None else np.repeat(group_theta, 64)[:n]
+        f32, ni, nv = orc.compute_forces(*pos, ch, nm, com, lo, hi, theta, 1.0, soft, 1.0, n,
+                                         precision=32, mac=1, theta_g=per_particle,
+                                         counts=True, bucket=bucket)
+        g = host(gs)
+        assert int(g[0]) == int(ni.sum()) and int(g[1]) == int(nv.sum())
+        per = g[3:].reshape(W, 3)
+        pad = np.zeros(W * 64 - n, dtype=np.int64)
+        assert np.array_equal(per[:, 0], np.concatenate([ni, pad]).reshape(W, 64).sum(1))
+        assert np.array_equal(per[:, 1], np.concatenate([nv, pad]).reshape(W, 64).sum(1))
+        assert rel_rms(host(force), f32) <= 2e-6
+
+
 @pytest.mark.parametrize("mac", [0, 1])
 def test_traverse_pairs_identical(orc, torch_mod, ics32, mac):
     """Two particles per lane (128-particle groups) gives the same forces bit

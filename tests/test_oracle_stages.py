@@ -268,6 +268,48 @@ def test_bh_error_versus_theta(orc, ics16):
         assert rel_rms(f, direct) < 1e-6
 
 
+def test_bucket_rule(orc, ics16):
+    """Leaf buckets (AABB test): when every node is rejected the walk reaches
+    every leaf in leaf order, so buckets change no bit of the sums, in either
+    precision, and no interaction count (only the visits of the internal
+    nodes they skip); a whole-tree bucket is the direct sum; and at a real
+    theta the buckets can only add exact leaf terms, so the error against
+    the direct sum does not grow."""
+    p, m, ch, nm, com, nmin, nmax = _tree(orc, ics16)
+    n = len(m)
+    soft = 0.02 / 16
+    for prec in (64, 32):
+        base = orc.compute_forces(*p, ch, nm, com, nmin, nmax, 1e-6, 1.0, soft, 1.0, n,
+                                  precision=prec, mac=orc.MAC_AABB, counts=True)
+        for b in (2, 16, 64):
+            got = orc.compute_forces(*p, ch, nm, com, nmin, nmax, 1e-6, 1.0, soft, 1.0, n,
+                                     precision=prec, mac=orc.MAC_AABB, counts=True, bucket=b)
+            assert np.array_equal(got[0], base[0])
+            assert np.array_equal(got[1], base[1])
+            # a bucket of two leaves holds no internal node to skip
+            assert np.all(got[2] <= base[2]) and (b == 2 or np.all(got[2] < base[2]))
+    # the reference test ignores buckets
+    ref = orc.compute_forces(*p, ch, nm, com, nmin, nmax, 0.5, 1.0, soft, 1.0, n, precision=32)
+    assert np.array_equal(ref, orc.compute_forces(*p, ch, nm, com, nmin, nmax, 0.5, 1.0, soft,
+                                                  1.0, n, precision=32, bucket=16))
+    direct = orc.direct_sum(*p, m, np.arange(n), 1.0, soft)
+    f0 = orc.compute_forces(*p, ch, nm, com, nmin, nmax, 0.5, 1.0, soft, 1.0, n,
+                            mac=orc.MAC_AABB)
+    f16 = orc.compute_forces(*p, ch, nm, com, nmin, nmax, 0.5, 1.0, soft, 1.0, n,
+                             mac=orc.MAC_AABB, bucket=16)
+    assert rel_rms(f16, direct) <= rel_rms(f0, direct)
+    # 50 particles, one bucket holding the whole tree: the root straddles the
+    # minimum-image boundary of every particle, so it is rejected and summed
+    rng = np.random.default_rng(8)
+    pos = [rng.uniform(0, 1, 50).astype(np.float32) for _ in range(3)]
+    p, m, ch, nm, com, nmin, nmax = _tree(orc, pos)
+    f, ni, nv, f64 = orc.compute_forces(*p, ch, nm, com, nmin, nmax, 0.5, 1.0, 1e-3, 1.0, 50,
+                                        precision=64, mac=orc.MAC_AABB, counts=True,
+                                        want_f64=True, bucket=64)
+    assert rel_rms(f64, orc.direct_sum(*p, m, np.arange(50), 1.0, 1e-3)) < 1e-12
+    assert np.all(ni == 49) and np.all(nv == 51)
+
+
 @pytest.mark.parametrize("mac", [0, 1])
 def test_fp32_mode_tracks_fp64(orc, ics32, mac):
     p, m, ch, nm, com, nmin, nmax = _tree(orc, ics32)
