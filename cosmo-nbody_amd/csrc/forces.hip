@@ -28,6 +28,7 @@ struct TravParams {
   float G;
   float eps2;
   float half_box;
+  float straddle_mu;  // AABB: nodes lighter than this pass the minimum-image guard
   int wave_stats;   // STATS: also store each wavefront's counts
   const int32_t *order;  // launch order of the groups (nullable)
   int64_t n_waves;       // number of groups when order is given
@@ -287,14 +288,16 @@ __global__ __launch_bounds__(256) void k_traverse(
         }
       } else {
         // (s/theta + dl)^2 < d^2, and every particle of the node sits on the
-        // same side of the minimum-image boundary as its CoM in each axis
+        // same side of the minimum-image boundary as its CoM in each axis,
+        // unless the node is lighter than straddle_mu (a node across the
+        // boundary puts at most its mass on the wrong image)
         near_ok = d.d2 > 1e-7f;
         const float r = __builtin_fmaf(B.x, inv_theta[h], B.y);
         const float hb = p.half_box;
         const bool inside = (d.dx + C.w < hb) && (d.dx - C.x > -hb) &&
                             (d.dy + D.x < hb) && (d.dy - C.y > -hb) &&
                             (d.dz + D.y < hb) && (d.dz - C.z > -hb);
-        accept = leaf || ((r * r < d.d2) && inside);
+        accept = leaf || ((r * r < d.d2) && (inside || A.w < p.straddle_mu));
       }
       hit[h] = in[h] & accept & near_ok;
       rej[h] = in[h] & !accept;
@@ -327,6 +330,11 @@ __global__ __launch_bounds__(256) void k_traverse(
         const int first = __float_as_int(D.z);
         float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (lane < bcount) q = leaves[first + lane];
+        // compensated mode: the bucket's terms form a short FP32 partial sum
+        // that enters the running sum as one compensated term
+        float bx[PPL], by[PPL], bz[PPL];
+#pragma unroll
+        for (int h = 0; h < PPL; ++h) bx[h] = by[h] = bz[h] = 0.0f;
         for (int k = 0; k < bcount; ++k) {
           const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.x), k));
           const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.y), k));
@@ -339,12 +347,26 @@ __global__ __launch_bounds__(256) void k_traverse(
             const bool hb = rej[h] & (d.d2 > 1e-7f);
             if (hb) {
               fix_ties(d, qx, qy, qz, x[h], y[h], z[h]);
-              accumulate<COMP>(d, __int_as_float(qm), p.eps2, ax[h], ay[h], az[h], cx[h],
-                               cy[h], cz[h]);
+              if (COMP)
+                accumulate<false>(d, __int_as_float(qm), p.eps2, bx[h], by[h], bz[h], cx[h],
+                                  cy[h], cz[h]);
+              else
+                accumulate<false>(d, __int_as_float(qm), p.eps2, ax[h], ay[h], az[h], cx[h],
+                                  cy[h], cz[h]);
             }
             if (STATS) {
               w_vis += __builtin_popcountll(__ballot(rej[h]));
               w_int += __builtin_popcountll(__ballot(hb));
+            }
+          }
+        }
+        if (COMP) {
+#pragma unroll
+          for (int h = 0; h < PPL; ++h) {
+            if (rej[h]) {
+              neumaier_add(ax[h], cx[h], bx[h]);
+              neumaier_add(ay[h], cy[h], by[h]);
+              neumaier_add(az[h], cz[h], bz[h]);
             }
           }
         }
@@ -522,7 +544,10 @@ static int traverse_impl(const float *px, const float *py, const float *pz, cons
                          int64_t n_tree, const int32_t *root_idx, float *force, double theta,
                          const float *theta_group, double G, double soft, double box,
                          int64_t off, int64_t count, int64_t n_bodies, int mac, int flags,
-                         uint64_t *stats, const int32_t *order, void *stream) {
+                         uint64_t *stats, const int32_t *order, double straddle_mass,
+                         void *stream) {
+  if (!(straddle_mass >= 0.0) || (straddle_mass > 0.0 && mac != COSMO_MAC_AABB))
+    return COSMO_E_INVALID;
   if (n_bodies < 0 || n_bodies > kMaxBodies || off < 0 || count < 0 || n_tree < 1 ||
       n_tree > kMaxBodies)
     return COSMO_E_INVALID;
@@ -543,6 +568,7 @@ static int traverse_impl(const float *px, const float *py, const float *pz, cons
   const float sf = static_cast<float>(soft);
   p.eps2 = sf * sf;
   p.half_box = static_cast<float>(box) * 0.5f;
+  p.straddle_mu = static_cast<float>(straddle_mass);
   p.wave_stats = (flags & COSMO_TRAV_GROUP_STATS) != 0;
   p.order = order;
   p.n_waves = (hi - lo + kWave * (pairs_of(flags) ? 2 : 1) - 1) / (kWave * (pairs_of(flags) ? 2 : 1));
@@ -571,7 +597,7 @@ int cosmo_traverse(const float *px, const float *py, const float *pz, const void
                    int64_t count, int64_t n_bodies, int mac, int flags, uint64_t *stats,
                    void *stream) {
   return traverse_impl(px, py, pz, nodes, n_tree, root_idx, force, theta, theta_group, G, soft,
-                       box, off, count, n_bodies, mac, flags, stats, nullptr, stream);
+                       box, off, count, n_bodies, mac, flags, stats, nullptr, 0.0, stream);
 }
 
 int cosmo_traverse_ordered(const float *px, const float *py, const float *pz, const void *nodes,
@@ -581,7 +607,18 @@ int cosmo_traverse_ordered(const float *px, const float *py, const float *pz, co
                            uint64_t *stats, const int32_t *group_order, void *stream) {
   if (!group_order || (flags & COSMO_TRAV_PAIRS)) return COSMO_E_INVALID;
   return traverse_impl(px, py, pz, nodes, n_tree, root_idx, force, theta, theta_group, G, soft,
-                       box, off, count, n_bodies, mac, flags, stats, group_order, stream);
+                       box, off, count, n_bodies, mac, flags, stats, group_order, 0.0, stream);
+}
+
+int cosmo_traverse_ex(const float *px, const float *py, const float *pz, const void *nodes,
+                      int64_t n_tree, const int32_t *root_idx, float *force, double theta,
+                      const float *theta_group, double G, double soft, double box, int64_t off,
+                      int64_t count, int64_t n_bodies, int mac, int flags, uint64_t *stats,
+                      const int32_t *group_order, double straddle_mass, void *stream) {
+  if (group_order && (flags & COSMO_TRAV_PAIRS)) return COSMO_E_INVALID;
+  return traverse_impl(px, py, pz, nodes, n_tree, root_idx, force, theta, theta_group, G, soft,
+                       box, off, count, n_bodies, mac, flags, stats, group_order, straddle_mass,
+                       stream);
 }
 
 int cosmo_order_groups(const float *theta_group, int64_t groups, int32_t *order, void *stream) {

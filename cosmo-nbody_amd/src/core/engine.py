@@ -45,6 +45,9 @@ class SimulationEngine:
         self.mac = mac_of(config)
         # leaf buckets of the AABB records (0 = walk every leaf)
         self.bucket = int(getattr(config, "BUCKET", 0)) if self.mac == MAC_AABB else 0
+        # AABB test: nodes lighter than straddle_mass pass the minimum-image guard
+        self.straddle_mass = (float(getattr(config, "STRADDLE_MASS", 0.0))
+                              if self.mac == MAC_AABB else 0.0)
         self.theta_group = None     # per-group opening angles (learned theta), or None
         self.theta_selector = None  # LearnedTheta: recomputes theta_group after each build
         self.group_order = None     # launch order of the groups under per-group theta
@@ -72,7 +75,7 @@ class SimulationEngine:
         traverse(*s.pos, self.tree_manager.nodes, self.tree_manager.root_idx, s.force, c.THETA,
                  c.G, c.SOFTENING, c.BOX_SIZE, mac=self.mac, theta_group=self.theta_group,
                  compensated=bool(getattr(c, "COMPENSATED", False)), stats=self.stats,
-                 pairs=self.pairs,
+                 pairs=self.pairs, straddle_mass=self.straddle_mass,
                  group_order=(self.group_order if self.theta_group is not None and not self.pairs
                               else None))
 

@@ -6,10 +6,12 @@ traversal.  For each group and each candidate theta_k the MLP predicts
 y = log2(mean over the group of |a_BH - a_direct|^2 / mean over all sampled
 targets of |a_direct|^2), the group's share of the squared relative RMS
 error.  The step then picks, per group, the theta_k that minimises
-cost_k + lambda * 2^y, where cost_k is the measured number of interactions
-per particle at theta_k; lambda sets the error budget and is calibrated
-against the FP64 direct sum.  Non-finite predictions fall back to the fixed
-theta, so with poisoned weights the step is the plain method.
+cost_k + lambda * 2^y, where cost_k is the measured cost of theta_k: the
+traversal time of one group at theta_k (the whole-state traversal at fixed
+theta_k divided by the number of groups), or the interactions per particle;
+lambda sets the error budget and is calibrated against the FP64 direct sum.
+Non-finite predictions fall back to the fixed theta, so with poisoned
+weights the step is the plain method.
 
 Everything runs on the GPU with the project's own kernels: features
 (cosmo_group_features), rows (cosmo_assemble_rows), the MFMA MLP forward
@@ -30,6 +32,7 @@ from src.kernels.physics_kernels import direct_sum, order_groups, traverse
 
 DEFAULT_THETAS = (0.04, 0.05, 0.06, 0.07, 0.08, 0.1, 0.12, 0.15, 0.2, 0.25, 0.3, 0.4, 0.5, 0.6,
                   0.8, 1.0)
+LABEL_FLOOR = -40.0   # log2 of the smallest squared-error share a label keeps
 
 
 def init_params(seed=0, device="cuda"):
@@ -91,7 +94,7 @@ class Samples:
     X: torch.Tensor        # int16 [groups*K, 16] BF16 rows
     T: torch.Tensor        # float32 [groups*K] labels log2(relative squared error)
     err2: torch.Tensor     # float64 [groups, K] group squared-error sums
-    cost: np.ndarray       # interactions per particle at each theta
+    cost: np.ndarray       # cost of each theta (ms per group, or interactions per particle)
     norm: float            # sum over targets of |a_direct|^2
     groups: np.ndarray     # sampled group ids
 
@@ -137,7 +140,8 @@ class GroupProbe:
         self._stats.zero_()
         traverse(*self.tpos, tm.nodes, tm.root_idx, self._force, theta, c.G, c.SOFTENING,
                  c.BOX_SIZE, mac=e.mac, theta_group=theta_group, stats=self._stats,
-                 n_nodes=tm.n_nodes)
+                 n_nodes=tm.n_nodes, compensated=bool(getattr(c, "COMPENSATED", False)),
+                 straddle_mass=e.straddle_mass)
         d = torch.zeros((self.T, 3), dtype=torch.float64, device=self.a_ref.device)
         d[self.vmask] = self._force[self.vmask].double() - self.a_ref
         err2 = (d ** 2).sum(1).reshape(len(self.groups), 64).sum(1)
@@ -148,8 +152,11 @@ class GroupProbe:
         return math.sqrt(float(err2.sum().item()) / self.norm), inter
 
 
-def sample(engine, n_groups, thetas=DEFAULT_THETAS, seed=0):
-    """Training samples from the engine's current tree (built)."""
+def sample(engine, n_groups, thetas=DEFAULT_THETAS, seed=0, cost_kind="time", cost=None):
+    """Training samples from the engine's current tree (built).  cost_kind:
+    "time" (ms of the whole-state traversal at each fixed theta per group)
+    or "interactions" (per particle, on the sampled groups); a given `cost`
+    array is used as is."""
     st, tm, c = engine.state, engine.tree_manager, engine.config
     G = (st.n + 63) // 64
     rng = np.random.default_rng(seed)
@@ -158,16 +165,23 @@ def sample(engine, n_groups, thetas=DEFAULT_THETAS, seed=0):
     th = torch.tensor(thetas, dtype=torch.float32, device=feat.device)
     X = K.assemble_rows(feat[torch.from_numpy(groups).to(feat.device)].contiguous(), th)
     probe = GroupProbe(engine, groups)
-    cols, cost = [], []
+    cols, measured = [], []
     for t in thetas:
         e2, inter = probe.errors(t)
         cols.append(e2)
-        cost.append(inter)
+        if cost is None:
+            measured.append(inter if cost_kind == "interactions"
+                            else _time_traversal(engine, t, None, reps=2) / G)
+    cost = np.asarray(measured if cost is None else cost, np.float64)
     err2 = torch.stack(cols, 1)
     mean_norm = probe.norm / probe.counts.sum()
     cnt = torch.from_numpy(probe.counts).to(err2.device)[:, None]
-    T = torch.log2(err2 / cnt / mean_norm + 1e-30).float().reshape(-1).contiguous()
-    return Samples(X, T, err2, np.array(cost), probe.norm, groups)
+    # log2 of the group's share of the squared error, floored at LABEL_FLOOR:
+    # shares that far below the budget of any useful target never decide a
+    # selection, and unfloored they would dominate the squared loss
+    T = torch.log2(err2 / cnt / mean_norm + 1e-30).clamp_min(LABEL_FLOOR).float()
+    T = T.reshape(-1).contiguous()
+    return Samples(X, T, err2, cost, probe.norm, groups)
 
 
 class LearnedTheta:
@@ -231,16 +245,35 @@ class LearnedTheta:
 def _time_traversal(engine, theta, theta_group, reps=5, group_order=None):
     """Average HIP-event time (ms) of the full traversal (all particles)."""
     st, tm, c = engine.state, engine.tree_manager, engine.config
+    comp = bool(getattr(c, "COMPENSATED", False))
     traverse(*st.pos, tm.nodes, tm.root_idx, st.force, theta, c.G, c.SOFTENING, c.BOX_SIZE,
-             mac=engine.mac, theta_group=theta_group, group_order=group_order)
+             mac=engine.mac, theta_group=theta_group, group_order=group_order,
+             compensated=comp, straddle_mass=engine.straddle_mass)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(reps):
         traverse(*st.pos, tm.nodes, tm.root_idx, st.force, theta, c.G, c.SOFTENING, c.BOX_SIZE,
-                 mac=engine.mac, theta_group=theta_group, group_order=group_order)
+                 mac=engine.mac, theta_group=theta_group, group_order=group_order,
+                 compensated=comp, straddle_mass=engine.straddle_mass)
     e1.record()
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps
+
+
+def _time_steps(c, state, selector, steps=3):
+    """Average HIP-event time (ms) of full engine steps (sort, tree, per-group
+    theta selection when a selector is given, traversal, update) starting
+    from `state`; one untimed warm-up step."""
+    eng = _engine_at(c, *state)
+    eng.theta_selector = selector
+    eng.step()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        eng.step()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
 
 
 def _bisect(f, lo, hi, target, iters=20, log=False, increasing=True):
@@ -282,8 +315,8 @@ def default_states(c):
 
 
 def experiment(preset="repo_128", target=1e-3, mac="aabb", states=None, n_groups=2048,
-               n_val_groups=1024, fit_steps=3000, seed=0, thetas=DEFAULT_THETAS,
-               target_theta=None, lambdas=None, log=print):
+               n_val_groups=1024, fit_steps=20000, seed=0, thetas=DEFAULT_THETAS,
+               target_theta=None, lambdas=None, log=print, overrides=None):
     """Trains the model on the GPU from samples the method generates, then
     compares fixed theta and learned per-group theta at equal relative RMS
     error (FP64 direct sum on held-out groups of a held-out state) by
@@ -291,12 +324,13 @@ def experiment(preset="repo_128", target=1e-3, mac="aabb", states=None, n_groups
     the cheapest setting that meets the target is kept.  With target_theta
     the error target is the fixed-theta error at that theta."""
     from src.utils import config as cfg
-    c = cfg.preset(preset, MAC=mac)
+    c = cfg.preset(preset, MAC=mac, **(overrides or {}))
     train, val = default_states(c) if states is None else states
     Xs, Ts, costs = [], [], []
     for k, (sd, steps) in enumerate(train):
         eng = _engine_at(c, sd, steps)
-        smp = sample(eng, n_groups, thetas, seed=seed + k)
+        # the cost of each theta is timed on the first training state
+        smp = sample(eng, n_groups, thetas, seed=seed + k, cost=costs[0] if costs else None)
         Xs.append(smp.X)
         Ts.append(smp.T)
         costs.append(smp.cost)
@@ -317,7 +351,11 @@ def experiment(preset="repo_128", target=1e-3, mac="aabb", states=None, n_groups
     rng = np.random.default_rng(seed + 12345)
     vgroups = np.sort(rng.choice(G, size=min(n_val_groups, G), replace=False))
     probe = GroupProbe(eng, vgroups)
-    fixed_curve = [(t, *probe.rel_rms(t)) for t in thetas]
+    err2_k, fixed_curve = [], []
+    for t in thetas:
+        e2, inter = probe.errors(t)
+        err2_k.append(e2.cpu().numpy())
+        fixed_curve.append((t, math.sqrt(float(e2.sum().item()) / probe.norm), inter))
     log("fixed theta curve (theta, rel_rms, interactions): " +
         ", ".join(f"({t:.2f}, {e:.2e}, {i:.0f})" for t, e, i in fixed_curve))
     if target_theta is not None:
@@ -328,24 +366,46 @@ def experiment(preset="repo_128", target=1e-3, mac="aabb", states=None, n_groups
     if theta_star is None:
         return {"reachable": False, "target": target, "fixed_curve": fixed_curve}
     e_fixed, i_fixed = probe.rel_rms(theta_star)
+    t_fixed = _time_traversal(eng, theta_star, None)
+    # hindsight bound: per validation group the theta of least cost + lambda *
+    # measured error (what a perfect predictor could pick), against the fixed
+    # theta's time
+    E = np.stack(err2_k, 1)
+    cost_arr = np.asarray(cost, np.float64)
+    bound = None
+    for lam in np.logspace(-6, 16, 221):
+        k = np.argmin(cost_arr[None, :] + lam * E / probe.norm, axis=1)
+        e = math.sqrt(E[np.arange(len(k)), k].sum() / probe.norm)
+        mc = float(cost_arr[k].mean())
+        if e <= target and (bound is None or mc < bound[1]):
+            bound = (float(lam), mc)
     lt = LearnedTheta(model, thetas, cost, 1.0, theta_star)
     vg = torch.from_numpy(vgroups).to(eng.state.pos[0].device)
+    th_np = np.asarray(thetas, np.float32)
+
+    def model_cost(tg):
+        k = np.clip(np.searchsorted(th_np, tg.cpu().numpy()), 0, len(th_np) - 1)
+        return float(cost_arr[k].mean())
+
     lambdas = np.logspace(-2, 14, 65) if lambdas is None else lambdas
-    best = None
     sweep = []
     for lam in lambdas:
         tg = lt.theta_groups(eng, float(lam))
         e, inter = probe.rel_rms(0.5, tg.index_select(0, vg).contiguous())
-        sweep.append((float(lam), e, inter))
-        if e <= target and (best is None or inter < best[2]):
-            best = (float(lam), e, inter)
-    if best is None:
+        sweep.append((float(lam), e, inter, model_cost(tg)))
+    ok = sorted((sw for sw in sweep if sw[1] <= target), key=lambda sw: sw[3])
+    if not ok:
         return {"reachable": False, "target": target, "theta_fixed": theta_star,
                 "sweep": sweep}
+    best = None
+    for lam, _, _, _ in ok[:3]:  # the cheapest settings by the cost model, timed
+        tg = lt.theta_groups(eng, lam).clone()
+        t = _time_traversal(eng, 0.5, tg, group_order=lt.launch_order(tg).clone())
+        if best is None or t < best[1]:
+            best = (lam, t)
     lt.lam = best[0]
     tg = lt.theta_groups(eng).clone()
     e_learned, i_learned = probe.rel_rms(0.5, tg.index_select(0, vg).contiguous())
-    t_fixed = _time_traversal(eng, theta_star, None)
     t_learned_unordered = _time_traversal(eng, 0.5, tg)
     t_learned = _time_traversal(eng, 0.5, tg, group_order=lt.launch_order(tg).clone())
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -355,7 +415,12 @@ def experiment(preset="repo_128", target=1e-3, mac="aabb", states=None, n_groups
     e1.record()
     torch.cuda.synchronize()
     t_select = e0.elapsed_time(e1) / 5
+    step_fixed = _time_steps(c.replace(THETA=theta_star), val, None)
+    step_learned = _time_steps(c, val, lt)
     res = {"reachable": True, "preset": preset, "mac": mac, "target": target,
+           "bucket": int(getattr(c, "BUCKET", 0)),
+           "compensated": bool(getattr(c, "COMPENSATED", False)),
+           "straddle_mass": float(getattr(c, "STRADDLE_MASS", 0.0)),
            "n": eng.state.n, "train_states": train, "val_state": val,
            "train_rows": int(X.shape[0]), "fit_s": t_fit, "loss_first": hist[0][1],
            "loss_last": hist[-1][1], "theta_fixed": theta_star, "err_fixed": e_fixed,
@@ -365,6 +430,9 @@ def experiment(preset="repo_128", target=1e-3, mac="aabb", states=None, n_groups
            "traversal_ms_learned_unordered": t_learned_unordered, "select_ms": t_select,
            "speedup_traversal": t_fixed / t_learned,
            "speedup_with_select": t_fixed / (t_learned + t_select),
+           "step_ms_fixed": step_fixed, "step_ms_learned": step_learned,
+           "speedup_step": step_fixed / step_learned,
+           "hindsight_bound_speedup": ((t_fixed / G) / bound[1] if bound else None),
            "interaction_ratio": i_fixed / i_learned,
            "theta_group_mean": float(tg.mean().item()), "fallbacks": lt.n_fallback}
     return res, lt
@@ -377,15 +445,26 @@ def main():
     ap.add_argument("--preset", default="repo_128")
     ap.add_argument("--mac", default="aabb")
     ap.add_argument("--target", type=float, default=1e-3)
-    ap.add_argument("--fit-steps", type=int, default=3000)
+    ap.add_argument("--fit-steps", type=int, default=20000)
     ap.add_argument("--groups", type=int, default=2048)
     ap.add_argument("--target-theta", type=float, default=None)
     ap.add_argument("--val-groups", type=int, default=1024)
+    ap.add_argument("--bucket", type=int, default=None)
+    ap.add_argument("--compensated", type=int, default=None)
+    ap.add_argument("--straddle", type=float, default=None)
     ap.add_argument("--save", default="")
     args = ap.parse_args()
+    over = {}
+    if args.bucket is not None:
+        over["BUCKET"] = args.bucket
+    if args.compensated is not None:
+        over["COMPENSATED"] = bool(args.compensated)
+    if args.straddle is not None:
+        over["STRADDLE_MASS"] = args.straddle
     out = experiment(args.preset, args.target, args.mac, fit_steps=args.fit_steps,
                      n_groups=args.groups, n_val_groups=args.val_groups,
-                     target_theta=args.target_theta, log=lambda m: print(m, flush=True))
+                     target_theta=args.target_theta, log=lambda m: print(m, flush=True),
+                     overrides=over)
     if isinstance(out, tuple):
         res, lt = out
         if args.save:

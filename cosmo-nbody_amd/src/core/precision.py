@@ -7,7 +7,7 @@ Error metrics over the targets s:
   rel_rms   = sqrt(sum_s |a_s - d_s|^2 / sum_s |d_s|^2)   (primary)
   mean_rel  = sqrt(mean_s |a_s - d_s|^2 / |d_s|^2)
 
-Run: python -m src.core.precision --preset repo_128 --evolve 300
+Run: python -m src.core.precision --preset clustered_128 --evolve 300
 """
 from __future__ import annotations
 
@@ -19,7 +19,6 @@ import numpy as np
 import torch
 
 from src.kernels.physics_kernels import MAC_AABB, MAC_REF, direct_sum, traverse
-from src.kernels.tree_kernels import normalize_pack, pack_nodes
 
 
 def sample_targets(n, n_targets, seed=1234):
@@ -35,9 +34,12 @@ def errors(force, ref):
             float(torch.sqrt((num / den.clamp_min(1e-300)).mean())))
 
 
-def sweep(engine, thetas, n_targets=16384, seed=1234, macs=(MAC_REF, MAC_AABB),
+def sweep(engine, thetas, n_targets=16384, seed=1234,
+          variants=(("ref", 0, 0.0), ("aabb", 0, 0.0), ("aabb", 32, 0.0)),
           compensated=(False, True)):
-    """engine: a SimulationEngine whose tree is built for the current state."""
+    """engine: a SimulationEngine whose tree is built for the current state.
+    variants: (acceptance test, leaf bucket, straddle mass) triples; the
+    engine's tree is re-packed for each and restored at the end."""
     st, tm, c = engine.state, engine.tree_manager, engine.config
     idx = sample_targets(st.n, n_targets, seed)
     it = torch.from_numpy(idx).to(st.pos[0].device)
@@ -49,21 +51,25 @@ def sweep(engine, thetas, n_targets=16384, seed=1234, macs=(MAC_REF, MAC_AABB),
     out = []
     force = torch.empty((len(idx), 3), dtype=torch.float32, device=ref.device)
     stats = torch.zeros(3, dtype=torch.int64, device=ref.device)
-    for mac in macs:
-        pack_nodes(tm.child, tm.mass, tm.com, tm.aabb_min, tm.aabb_max, c.BOX_SIZE, mac, tm.nodes)
+    own = (engine.mac, engine.bucket)
+    for name, bucket, mu in variants:
+        mac = MAC_REF if name == "ref" else MAC_AABB
+        # the particles are already in Morton order, so the rebuild keeps them
+        engine.mac, engine.bucket = mac, (bucket if mac == MAC_AABB else 0)
+        engine.build_tree()
         for th in thetas:
             for comp in compensated:
                 stats.zero_()
                 traverse(*tpos, tm.nodes, tm.root_idx, force, th, c.G, c.SOFTENING, c.BOX_SIZE,
-                         mac=mac, compensated=comp, stats=stats, n_nodes=tm.n_nodes)
+                         mac=mac, compensated=comp, stats=stats, n_nodes=tm.n_nodes,
+                         straddle_mass=mu if mac == MAC_AABB else 0.0)
                 r, m = errors(force, ref)
-                out.append({"mac": "ref" if mac == MAC_REF else "aabb", "theta": th,
-                            "compensated": comp, "rel_rms": r, "mean_rel": m,
+                out.append({"mac": name, "bucket": engine.bucket, "straddle_mass": mu,
+                            "theta": th, "compensated": comp, "rel_rms": r, "mean_rel": m,
                             "interactions": float(stats[0].item()) / len(idx),
                             "visits": float(stats[1].item()) / len(idx)})
-    # restore the engine's own packing
-    pack_nodes(tm.child, tm.mass, tm.com, tm.aabb_min, tm.aabb_max, c.BOX_SIZE, engine.mac,
-               tm.nodes)
+    engine.mac, engine.bucket = own
+    engine.build_tree()
     return out, t_direct
 
 
@@ -83,7 +89,7 @@ def main():
     from src.core.engine import SimulationEngine
     from src.core.state import SimulationState
     from src.utils import config as cfg
-    from src.utils.cosmology import generate_zeldovich_ics
+    from src.utils.cosmology import generate_ics
     ap = argparse.ArgumentParser()
     ap.add_argument("--preset", default="repo_128")
     ap.add_argument("--evolve", type=int, default=0)
@@ -91,7 +97,7 @@ def main():
     ap.add_argument("--thetas", default="0.1,0.2,0.3,0.5,0.7,0.9")
     args = ap.parse_args()
     c = cfg.preset(args.preset)
-    pos, vel = generate_zeldovich_ics(c)
+    pos, vel = generate_ics(c)
     st = SimulationState(len(pos[0]), pos, vel)
     eng = SimulationEngine(c, st)
     info = {"preset": c.NAME, "n": st.n, "evolve_steps": args.evolve,

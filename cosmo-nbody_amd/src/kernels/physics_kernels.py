@@ -54,14 +54,16 @@ def compute_forces(pos_x, pos_y, pos_z, children, n_mass, n_com, n_min, n_max, f
 
 def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0, count=None,
              mac=MAC_REF, theta_group=None, compensated=False, stats=None, n_nodes=None,
-             pairs=False, group_stats=False, group_order=None):
+             pairs=False, group_stats=False, group_order=None, straddle_mass=0.0):
     """Traversal over packed node records; stats: int64[3] tensor accumulating
     (interactions, node visits, wavefront stack pops) when given; with
     group_stats it has 3 + 3*W entries and entries 3+3w.. hold wavefront group
     w's own counts.  The targets (pos_*) need not be the tree's particles:
     n_nodes gives the tree size when they differ.  pairs: two particles per
     lane (128-particle groups).  group_order: int32 launch order of the
-    64-particle groups (a permutation, e.g. from order_groups)."""
+    64-particle groups (a permutation, e.g. from order_groups).
+    straddle_mass (AABB test): nodes lighter than straddle_mass pass the
+    minimum-image guard (0 = exact guard)."""
     n = pos_x.shape[0]
     n_nodes = 2 * n - 1 if n_nodes is None else int(n_nodes)
     n_tree = (n_nodes + 1) // 2
@@ -80,11 +82,10 @@ def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0
             ptr(root, torch.int32, "root"), ptr(force, _F32, "force", numel=3 * n),
             float(theta), tg, float(G), float(soft), float(box), int(off), int(count), n,
             int(mac), flags, st)
-    if group_order is None:
-        check(load().cosmo_traverse(*args, stream_handle()), "traverse")
-    else:
-        order = ptr(group_order, torch.int32, "group_order", numel=max(groups, 0))
-        check(load().cosmo_traverse_ordered(*args, order, stream_handle()), "traverse_ordered")
+    order = None if group_order is None else ptr(group_order, torch.int32, "group_order",
+                                                 numel=max(groups, 0))
+    check(load().cosmo_traverse_ex(*args, order, float(straddle_mass), stream_handle()),
+          "traverse")
 
 
 def order_groups(theta_group, out=None):

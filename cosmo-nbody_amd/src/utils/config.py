@@ -60,6 +60,8 @@ RENDER_FRAC_BITS = 20      # fixed-point fraction bits of the density bins
 COMPENSATED = False        # Neumaier-compensated far-field accumulation
 MAC = "ref"                # acceptance test: "ref" (physics_kernels.py:128-133) or "aabb"
 BUCKET = 0                 # AABB test: nodes of <= BUCKET leaves are summed leaf by leaf when opened
+STRADDLE_MASS = 0.0        # AABB test: nodes lighter than STRADDLE_MASS pass the
+                           # minimum-image guard (0 = exact guard)
 TRAVERSAL_PAIRS = False    # two particles per lane in the traversal (same results, slower at 128^3)
 LEARNED_THETA = False
 ZA_AMPLITUDE = 0.3         # rms Zel'dovich displacement in grid spacings (cosmology.py:49)
@@ -103,6 +105,7 @@ class Config:
     COMPENSATED: bool = COMPENSATED
     MAC: str = MAC
     BUCKET: int = BUCKET
+    STRADDLE_MASS: float = STRADDLE_MASS
     TRAVERSAL_PAIRS: bool = TRAVERSAL_PAIRS
     LEARNED_THETA: bool = LEARNED_THETA
     NAME: str = "reference_defaults"

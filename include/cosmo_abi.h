@@ -226,6 +226,21 @@ int cosmo_traverse_ordered(const float *pos_x, const float *pos_y,
                            int64_t n_bodies, int mac, int flags, uint64_t *stats,
                            const int32_t *group_order, void *stream);
 
+/* cosmo_traverse with every option (no reference counterpart):
+ * group_order (nullable) as for cosmo_traverse_ordered, and, for the AABB
+ * test, straddle_mass >= 0: a node whose box crosses a particle's
+ * minimum-image boundary still passes the test when its FP32 mass is below
+ * fl(straddle_mass) (0 = the exact guard of cosmo_traverse).  Such a node
+ * puts at most its mass on the wrong image, so the limit bounds the error
+ * it adds. */
+int cosmo_traverse_ex(const float *pos_x, const float *pos_y, const float *pos_z,
+                      const void *nodes, int64_t n_tree, const int32_t *root_idx,
+                      float *force, double theta, const float *theta_group,
+                      double G, double soft, double box, int64_t off,
+                      int64_t count, int64_t n_bodies, int mac, int flags,
+                      uint64_t *stats, const int32_t *group_order,
+                      double straddle_mass, void *stream);
+
 /* order[0..groups) = the groups sorted by increasing theta_group (64
  * logarithmic bins; arbitrary order inside a bin): the launch order for
  * cosmo_traverse_ordered with per-group theta. */

@@ -324,7 +324,10 @@ int orc_validate_tree(const int32_t *children, const int32_t *parents,
  *   AABB, dl = |CoM - AABB centre|; accept iff (s/theta + dl)^2 < d^2 and,
  *   in every axis, d_k + (max_k - CoM_k) < box/2 and
  *   d_k - (CoM_k - min_k) > -box/2, so that no particle of an accepted node
- *   sits across the minimum-image boundary from its CoM.
+ *   sits across the minimum-image boundary from its CoM, unless the node's
+ *   FP32 mass is below the straddle limit fl(straddle_mass) (0 = the exact
+ *   guard): a node that straddles the boundary puts at most its mass on the
+ *   wrong image, so the limit bounds that error per node.
  * Leaves are always accepted; interactions with d^2 <= 1e-7 are dropped. */
 
 /* Leaf range [first, first + count) of every node, from the children
@@ -379,7 +382,8 @@ static void bh_one_f64(const float *px, const float *py, const float *pz,
                        const float *n_max, int mac, double theta, double G,
                        double soft, double box, int32_t root, int64_t i,
                        const int32_t *bfirst, const int32_t *bcount, int bucket,
-                       double out[3], int64_t *n_int, int64_t *n_vis) {
+                       float straddle_lim, double out[3], int64_t *n_int,
+                       int64_t *n_vis) {
   const double x = px[i], y = py[i], z = pz[i];
   double ax = 0.0, ay = 0.0, az = 0.0;
   int32_t stack[64];
@@ -423,7 +427,7 @@ static void bh_one_f64(const float *px, const float *py, const float *pz,
         const double dn = dd[k] - (c - (double)n_min[3 * node + k]);
         inside = inside && (up < 0.5 * box) && (dn > -0.5 * box);
       }
-      accept = (r_open * r_open < d2) && inside;
+      accept = (r_open * r_open < d2) && (inside || n_mass[node] < straddle_lim);
     }
     if (c1 == -1 || accept) {
       if (d2 > 1e-7) {
@@ -507,8 +511,8 @@ static void bh_one_f32(const float *px, const float *py, const float *pz,
                        const float *n_max, int mac, float thf, double th2d,
                        float Gf, float softf, float boxf, int32_t root,
                        int64_t i, int compensated, const int32_t *bfirst,
-                       const int32_t *bcount, int bucket, double out[3],
-                       int64_t *n_int, int64_t *n_vis) {
+                       const int32_t *bcount, int bucket, float straddle_lim,
+                       double out[3], int64_t *n_int, int64_t *n_vis) {
   const float x = px[i], y = py[i], z = pz[i];
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   float cx = 0.0f, cy = 0.0f, cz = 0.0f; /* Neumaier compensation terms */
@@ -558,7 +562,7 @@ static void bh_one_f32(const float *px, const float *py, const float *pz,
       }
       const float dl = sqrtf(fmaf(o[2], o[2], fmaf(o[1], o[1], o[0] * o[0])));
       const float r = fmaf(s, inv_theta, dl);
-      accept = (r * r < d2) && inside;
+      accept = (r * r < d2) && (inside || nm < straddle_lim);
     }
     if (c1 == -1 || accept) {
       if (near_ok) {
@@ -577,6 +581,11 @@ static void bh_one_f32(const float *px, const float *py, const float *pz,
         ni++;
       }
     } else if (bucket > 0 && bcount[node] <= bucket) {
+      /* compensated mode: the bucket's terms form a plain FP32 partial sum
+       * that enters the running sum as one compensated term */
+      float bx = 0.0f, by = 0.0f, bz = 0.0f;
+      float *sx = compensated ? &bx : &ax, *sy = compensated ? &by : &ay,
+            *sz = compensated ? &bz : &az;
       for (int32_t j = bfirst[node]; j < bfirst[node] + bcount[node]; ++j) {
         const float qm = n_mass[j];
         if (qm <= 0.0f) continue;
@@ -588,17 +597,16 @@ static void bh_one_f32(const float *px, const float *py, const float *pz,
         if (q2 > 1e-7f) {
           const float inv = 1.0f / sqrtf(q2 + eps2);
           const float f = (Gf * qm) * ((inv * inv) * inv);
-          if (compensated) {
-            neumaier_add(&ax, &cx, f * qx);
-            neumaier_add(&ay, &cy, f * qy);
-            neumaier_add(&az, &cz, f * qz);
-          } else {
-            ax = fmaf(f, qx, ax);
-            ay = fmaf(f, qy, ay);
-            az = fmaf(f, qz, az);
-          }
+          *sx = fmaf(f, qx, *sx);
+          *sy = fmaf(f, qy, *sy);
+          *sz = fmaf(f, qz, *sz);
           ni++;
         }
+      }
+      if (compensated) {
+        neumaier_add(&ax, &cx, bx);
+        neumaier_add(&ay, &cy, by);
+        neumaier_add(&az, &cz, bz);
       }
     } else if (top + 1 < 64) {
       if (c1 != -1) stack[top++] = c2;
@@ -619,7 +627,8 @@ void orc_compute_forces(const float *px, const float *py, const float *pz,
                         double G, double soft, double box, int32_t root,
                         int64_t off, int64_t count, int64_t n, int precision,
                         int mac, const float *theta_g, int bucket,
-                        int64_t *n_inter, int64_t *n_visit, double *force64) {
+                        double straddle_mass, int64_t *n_inter, int64_t *n_visit,
+                        double *force64) {
   int64_t hi = off + count;
   if (hi > n) hi = n;
   int32_t *bfirst = NULL, *bcount = NULL;
@@ -636,13 +645,15 @@ void orc_compute_forces(const float *px, const float *py, const float *pz,
     if (precision == 64) {
       const double th = theta_g ? (double)theta_g[i] : theta;
       bh_one_f64(px, py, pz, children, n_mass, n_com, n_min, n_max, mac, th, G,
-                 soft, box, root, i, bfirst, bcount, bucket, a, &ni, &nv);
+                 soft, box, root, i, bfirst, bcount, bucket, (float)straddle_mass,
+                 a, &ni, &nv);
     } else {
       const float th = theta_g ? theta_g[i] : (float)theta;
       const double th2d = theta_g ? (double)th * (double)th : theta * theta;
       bh_one_f32(px, py, pz, children, n_mass, n_com, n_min, n_max, mac, th,
                  th2d, (float)G, (float)soft, (float)box, root, i,
-                 precision == 31, bfirst, bcount, bucket, a, &ni, &nv);
+                 precision == 31, bfirst, bcount, bucket, (float)straddle_mass,
+                 a, &ni, &nv);
     }
     force[3 * i + 0] = (float)a[0];
     force[3 * i + 1] = (float)a[1];

@@ -89,6 +89,9 @@ int orc_validate_tree(const int32_t *children, const int32_t *parents,
  *                 for d2 and the accumulation, IEEE sqrt and divide).
  * mac: ORC_MAC_REF (the reference's test) or ORC_MAC_AABB (corrected).
  * theta_g: optional per-particle opening angle (NULL = use theta).
+ * straddle_mass: ORC_MAC_AABB only: a node whose box crosses the particle's
+ *   minimum-image boundary is still accepted (if the size test passes) when
+ *   its FP32 mass is below fl(straddle_mass); 0 = never.
  * bucket: ORC_MAC_AABB only (0 = off): a rejected internal node of at most
  *   `bucket` leaves is not opened; its leaves are visited in leaf order
  *   (no reference counterpart).
@@ -101,7 +104,8 @@ void orc_compute_forces(const float *px, const float *py, const float *pz,
                         double G, double soft, double box, int32_t root,
                         int64_t off, int64_t count, int64_t n, int precision,
                         int mac, const float *theta_g, int bucket,
-                        int64_t *n_inter, int64_t *n_visit, double *force64);
+                        double straddle_mass, int64_t *n_inter,
+                        int64_t *n_visit, double *force64);
 
 /* FP64 direct sum over all n sources for the listed targets, with the same
  * minimum image, d2 > 1e-7 cut and Plummer law as compute_forces.  Output

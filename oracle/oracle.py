@@ -52,8 +52,8 @@ def lib():
             "orc_build_top_tree_cache": [_p, _p, _p, _p, _i64],
             "orc_validate_tree": [_p, _p, _i64],
             "orc_compute_forces": [_p] * 9 + [_f64] * 4 + [_i32, _i64, _i64, _i64,
-                                                           C.c_int, C.c_int, _p, C.c_int, _p,
-                                                           _p, _p],
+                                                           C.c_int, C.c_int, _p, C.c_int, _f64,
+                                                           _p, _p, _p],
             "orc_direct_sum": [_p, _p, _p, _p, _i64, _p, _i64, _f64, _f64, _p],
             "orc_kick_drift": [_p] * 7 + [_f32] * 3 + [_i64],
             "orc_integrate_coeffs": [_f64, _f64, _f64, _p, _p, _p],
@@ -165,7 +165,7 @@ def validate_tree(children, parents, n):
 
 def compute_forces(px, py, pz, children, n_mass, n_com, n_min, n_max, theta, G, soft,
                    box, root, off=0, count=None, precision=64, mac=MAC_REF, theta_g=None,
-                   counts=False, want_f64=False, bucket=0):
+                   counts=False, want_f64=False, bucket=0, straddle_mass=0.0):
     """src/kernels/physics_kernels.py:14-153.
 
     Node arrays may be FP64 (as produced by the FP64 upward pass); they are
@@ -175,6 +175,8 @@ def compute_forces(px, py, pz, children, n_mass, n_com, n_min, n_max, theta, G, 
     if counts, (n_inter, n_visit) int64[n]; if want_f64 also the FP64 sums.
     bucket (AABB test only): rejected internal nodes of at most `bucket`
     leaves are summed leaf by leaf in leaf order instead of being opened.
+    straddle_mass (AABB test only): nodes lighter than straddle_mass pass the
+    minimum-image guard.
     """
     px, py, pz = (_c(a, np.float32) for a in (px, py, pz))
     n = px.shape[0]
@@ -193,8 +195,8 @@ def compute_forces(px, py, pz, children, n_mass, n_com, n_min, n_max, theta, G, 
     lib().orc_compute_forces(_ptr(px), _ptr(py), _ptr(pz), _ptr(children), _ptr(nm), _ptr(nc),
                              _ptr(nmin), _ptr(nmax), _ptr(force), float(theta), float(G),
                              float(soft), float(box), int(root), int(off), int(count), n,
-                             int(precision), int(mac), _ptr(tg), int(bucket), _ptr(ni),
-                             _ptr(nv), _ptr(f64))
+                             int(precision), int(mac), _ptr(tg), int(bucket),
+                             float(straddle_mass), _ptr(ni), _ptr(nv), _ptr(f64))
     out = [force]
     if counts:
         out += [ni, nv]

@@ -299,7 +299,8 @@ def test_traverse_buckets(orc, torch_mod, ics32, bucket):
     """AABB records with leaf buckets: the build's leaf ranges match the
     tree's; the traversal's per-particle interaction and visit counts equal
     the FP32 oracle's bucket rule (totals and per group, fixed and per-group
-    theta) and the forces agree to FP32 rounding."""
+    theta, plain and compensated sums, exact and tolerant minimum-image
+    guard) and the forces agree to FP32 rounding."""
     torch = torch_mod
     from src.kernels.physics_kernels import traverse
     from src.kernels.tree_kernels import build_tree_spans, normalize_pack_buckets
@@ -325,16 +326,20 @@ def test_traverse_buckets(orc, torch_mod, ics32, bucket):
     P = [dev(p, torch) for p in pos]
     W = (n + 63) // 64
     tg = np.random.default_rng(6).uniform(0.1, 0.9, W).astype(np.float32)
-    for theta, group_theta in ((0.3, None), (0.7, None), (0.5, tg)):
+    # (theta, per-group theta, compensated, straddle mass)
+    for theta, group_theta, comp, mu in ((0.3, None, False, 0.0), (0.7, None, False, 0.0),
+                                         (0.5, tg, False, 0.0), (0.3, None, True, 0.0),
+                                         (0.6, None, True, 4e-3), (0.5, tg, False, 1e-3)):
         force = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
         gs = torch.zeros(3 + 3 * W, dtype=torch.int64, device="cuda")
         traverse(*P, nodes, root, force, theta, 1.0, soft, 1.0, mac=1, stats=gs,
-                 group_stats=True,
+                 group_stats=True, compensated=comp, straddle_mass=mu,
                  theta_group=None if group_theta is None else dev(group_theta, torch))
         per_particle = None if group_theta is None else np.repeat(group_theta, 64)[:n]
         f32, ni, nv = orc.compute_forces(*pos, ch, nm, com, lo, hi, theta, 1.0, soft, 1.0, n,
-                                         precision=32, mac=1, theta_g=per_particle,
-                                         counts=True, bucket=bucket)
+                                         precision=31 if comp else 32, mac=1,
+                                         theta_g=per_particle, counts=True, bucket=bucket,
+                                         straddle_mass=mu)
         g = host(gs)
         assert int(g[0]) == int(ni.sum()) and int(g[1]) == int(nv.sum())
         per = g[3:].reshape(W, 3)

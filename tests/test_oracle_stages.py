@@ -310,6 +310,30 @@ def test_bucket_rule(orc, ics16):
     assert np.all(ni == 49) and np.all(nv == 51)
 
 
+def test_straddle_tolerance(orc, ics16):
+    """The straddle limit of the AABB test: 0 is the exact guard; a limit
+    below the lightest internal node changes nothing; raising it lets heavier
+    nodes across the minimum-image boundary through (fewer interactions,
+    larger error against the direct sum)."""
+    p, m, ch, nm, com, nmin, nmax = _tree(orc, ics16)
+    n = len(m)
+    soft = 0.02 / 16
+    direct = orc.direct_sum(*p, m, np.arange(n), 1.0, soft)
+    base = orc.compute_forces(*p, ch, nm, com, nmin, nmax, 0.7, 1.0, soft, 1.0, n, mac=1,
+                              counts=True, precision=32)
+    tiny = orc.compute_forces(*p, ch, nm, com, nmin, nmax, 0.7, 1.0, soft, 1.0, n, mac=1,
+                              counts=True, precision=32, straddle_mass=1.9 / n)
+    assert np.array_equal(base[0], tiny[0]) and np.array_equal(base[1], tiny[1])
+    errs, inter = [], []
+    for mu in (0.0, 1e-3, 1e-2, 1e-1):
+        f, ni, nv = orc.compute_forces(*p, ch, nm, com, nmin, nmax, 0.7, 1.0, soft, 1.0, n,
+                                       mac=1, counts=True, precision=32, straddle_mass=mu)
+        errs.append(rel_rms(f, direct))
+        inter.append(int(ni.sum()))
+    assert all(a <= b for a, b in zip(errs, errs[1:])) and errs[-1] > 2 * errs[0]
+    assert all(a >= b for a, b in zip(inter, inter[1:])) and inter[-1] < inter[0]
+
+
 @pytest.mark.parametrize("mac", [0, 1])
 def test_fp32_mode_tracks_fp64(orc, ics32, mac):
     p, m, ch, nm, com, nmin, nmax = _tree(orc, ics32)
