@@ -42,7 +42,36 @@ def parse():
                          "state plus a 256 MiB device copy (counter scale)")
     ap.add_argument("--graph", action="store_true",
                     help="replay the whole step from HIP graphs (per-stage times not split)")
+    ap.add_argument("--learned", action="store_true",
+                    help="learned-theta vs fixed-theta step rate at equal 1e-3 relative RMS "
+                         "force error (FP64 direct sum) on the 128^3 halo state: trains the "
+                         "network on three training states, evaluates on a held-out one")
     return ap.parse_args()
+
+
+def learned_line(args):
+    """One JSON line: full-step rate of the learned per-group theta against the
+    fixed theta that gives the same 1e-3 relative RMS error."""
+    from src.core import learned_theta as L
+    from src.utils import config as cfg
+    c = cfg.preset("halos_128")
+    out = L.experiment("halos_128", 1e-3, "aabb", n_groups=10240,
+                       log=lambda m: print(m, file=sys.stderr, flush=True))
+    res = out[0][0]
+    if not res.get("reachable"):
+        print(json.dumps({"metric": "learned-theta step rate at 1e-3", "value": None,
+                          "detail": res}, default=float))
+        return
+    print(json.dumps({
+        "metric": "learned-theta / fixed-theta step rate at equal 1e-3 relative RMS force error",
+        "value": res["speedup_step"], "unit": "x", "n_gpus": 1,
+        "ms_per_step": res["step_ms_learned"], "ms_per_step_fixed": res["step_ms_fixed"],
+        "higher_is_better": True, "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"halos_128: N={res['n']} (half in 64 Plummer spheres, half "
+                               f"uniform), AABB test, bucket {c.BUCKET}, compensated sums, "
+                               f"straddle mass {c.STRADDLE_MASS}, held-out state "
+                               f"{res['val_state']}", "n_bodies": res["n"]},
+        "detail": res}, default=float))
 
 
 def cpu_baseline(pos, vel, c, threads):
@@ -90,6 +119,9 @@ def main():
     from src.utils import config as cfg
     from src.utils.cosmology import generate_zeldovich_ics
 
+    if args.learned:
+        learned_line(args)
+        return
     c = cfg.preset(args.preset, TRAVERSAL_PAIRS=args.pairs)
     pos, vel = generate_zeldovich_ics(c)
     n = len(pos[0])

@@ -281,7 +281,9 @@ int64_t cosmo_tree_bytes(int64_t n, int mac) {
   if (n < 1) return 0;
   const int64_t inode = (mac == COSMO_MAC_AABB) ? int64_t(sizeof(cosmo_inode_aabb))
                                                 : int64_t(sizeof(cosmo_inode_ref));
-  return n * int64_t(sizeof(cosmo_leaf)) + (n - 1) * inode;
+  // the traversal reads a whole internal-record size at every node, leaves
+  // included, so the buffer carries one internal record of tail room
+  return n * int64_t(sizeof(cosmo_leaf)) + n * inode;
 }
 
 int cosmo_compute_codes(const float *px, const float *py, const float *pz,

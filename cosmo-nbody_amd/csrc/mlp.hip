@@ -416,6 +416,61 @@ __device__ __forceinline__ float node_extent(const float *__restrict__ lo,
   return e > e2 ? e : e2;
 }
 
+// Straddle profile: the nodes that the AABB test's straddle limit lets
+// across a minimum-image boundary are (down to the size test) the
+// straddle-level nodes, lighter than the limit with a parent at or above it.
+// Along each axis, bin b of the profile holds the sums of (M/limit)^2 and
+// M/limit over the straddle-level nodes whose box spans the bin, in 2^-32
+// fixed point (order-independent integer sums).  prof layout: [2][3][nbins],
+// first the squared masses, then the masses; diff has the same layout.
+constexpr double kProfScale = 4294967296.0;
+
+__global__ __launch_bounds__(256) void k_profile_clear(unsigned long long *__restrict__ d,
+                                                       int len) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < len) d[i] = 0ull;
+}
+
+__global__ __launch_bounds__(256) void k_profile_nodes(
+    const int32_t *__restrict__ parents, const double *__restrict__ n_mass,
+    const float *__restrict__ n_min, const float *__restrict__ n_max, int64_t n, double limit,
+    int nbins, unsigned long long *__restrict__ diff) {
+  const int64_t k = n + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= 2 * n - 1) return;
+  const double m = n_mass[k];
+  if (!(m < limit)) return;
+  const int p = parents[k];
+  if (p >= 0 && !(n_mass[p] >= limit)) return;
+  const double r = m / limit;
+  const unsigned long long w[2] = {static_cast<unsigned long long>(r * r * kProfScale),
+                                   static_cast<unsigned long long>(r * kProfScale)};
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    int b0 = static_cast<int>(n_min[3 * k + ax] * static_cast<float>(nbins));
+    int b1 = static_cast<int>(n_max[3 * k + ax] * static_cast<float>(nbins));
+    b0 = b0 < 0 ? 0 : (b0 > nbins - 1 ? nbins - 1 : b0);
+    b1 = b1 < 0 ? 0 : (b1 > nbins - 1 ? nbins - 1 : b1);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      unsigned long long *row = diff + (q * 3 + ax) * nbins;
+      atomicAdd(&row[b0], w[q]);
+      if (b1 + 1 < nbins) atomicAdd(&row[b1 + 1], 0ull - w[q]);
+    }
+  }
+}
+
+// Prefix sums of the six difference rows (one thread per row).
+__global__ void k_profile_scan(const unsigned long long *__restrict__ diff, int nbins,
+                               unsigned long long *__restrict__ prof) {
+  const int row = threadIdx.x;
+  if (row >= 6) return;
+  unsigned long long run = 0ull;
+  for (int b = 0; b < nbins; ++b) {
+    run += diff[row * nbins + b];
+    prof[row * nbins + b] = run;
+  }
+}
+
 // One thread per group of 64 Morton-consecutive particles: the nine
 // per-group features (the tenth input is the candidate theta).
 __global__ __launch_bounds__(256) void k_group_features(
@@ -423,7 +478,8 @@ __global__ __launch_bounds__(256) void k_group_features(
     const int32_t *__restrict__ children, const int32_t *__restrict__ parents,
     const double *__restrict__ n_mass, const double *__restrict__ n_com,
     const float *__restrict__ n_min, const float *__restrict__ n_max, int64_t n, float a,
-    const float *__restrict__ a_dev, float n_grid, float *__restrict__ feat) {
+    const float *__restrict__ a_dev, float n_grid, const unsigned long long *__restrict__ prof,
+    int nbins, float *__restrict__ feat) {
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t b0 = g * 64;
   if (b0 >= n) return;
@@ -459,7 +515,7 @@ __global__ __launch_bounds__(256) void k_group_features(
     if (p < 0) break;
     node = p;
   }
-  for (int lvl = 0; lvl < 3; ++lvl) {
+  for (int lvl = 0; lvl < 2; ++lvl) {
     const int p = parents[node];
     float r = 0.f, mlog = 0.f;
     if (p >= 0) {
@@ -477,6 +533,22 @@ __global__ __launch_bounds__(256) void k_group_features(
     f[3 + 2 * lvl] = r;
     f[4 + 2 * lvl] = mlog;
   }
+  // the straddle profile where the group's minimum-image boundaries lie,
+  // half a box from its centre in each axis
+  float s2 = 0.f, s1 = 0.f;
+  if (prof) {
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      float x = cen[ax] + 0.5f;
+      x -= floorf(x);
+      int b = static_cast<int>(x * static_cast<float>(nbins));
+      b = b < 0 ? 0 : (b > nbins - 1 ? nbins - 1 : b);
+      s2 += static_cast<float>(prof[ax * nbins + b]);
+      s1 += static_cast<float>(prof[(3 + ax) * nbins + b]);
+    }
+  }
+  f[7] = log2f(1.0f + s2 * 2.3283064365386963e-10f);
+  f[8] = log2f(1.0f + s1 * 2.3283064365386963e-10f);
 }
 
 // Rows g*K + k of the MLP input: [theta_k, features of g..., 0 pad] in BF16.
@@ -576,37 +648,63 @@ int cosmo_mlp_to_bf16(const float *params, uint16_t *wbf, void *stream) {
   return COSMO_OK;
 }
 
-int cosmo_group_features(const float *px, const float *py, const float *pz,
-                         const int32_t *children, const int32_t *parents, const double *n_mass,
-                         const double *n_com, const float *n_min, const float *n_max, int64_t n,
-                         double a, double n_grid, float *feat, void *stream) {
+int cosmo_straddle_profile(const int32_t *parents, const double *n_mass, const float *n_min,
+                           const float *n_max, int64_t n, double straddle_mass,
+                           uint64_t *profile, uint64_t *scratch, int nbins, void *stream) {
+  if (n < 1 || n > kMaxBodies || nbins < 1 || nbins > COSMO_MLP_PROFILE_MAX_BINS ||
+      !(straddle_mass > 0.0))
+    return COSMO_E_INVALID;
+  if (!parents || !n_mass || !n_min || !n_max || !profile || !scratch) return COSMO_E_INVALID;
+  hipStream_t s = as_stream(stream);
+  unsigned long long *diff = reinterpret_cast<unsigned long long *>(scratch);
+  const int len = 6 * nbins;
+  k_profile_clear<<<blocks_for(len, 256), 256, 0, s>>>(diff, len);
+  if (n > 1)
+    k_profile_nodes<<<blocks_for(n - 1, 256), 256, 0, s>>>(parents, n_mass, n_min, n_max, n,
+                                                           straddle_mass, nbins, diff);
+  k_profile_scan<<<1, 64, 0, s>>>(diff, nbins,
+                                  reinterpret_cast<unsigned long long *>(profile));
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+static int group_features_impl(const float *px, const float *py, const float *pz,
+                               const int32_t *children, const int32_t *parents,
+                               const double *n_mass, const double *n_com, const float *n_min,
+                               const float *n_max, int64_t n, double a, const float *a_dev,
+                               double n_grid, const uint64_t *profile, int nbins, float *feat,
+                               void *stream) {
   if (n < 1 || n > kMaxBodies) return COSMO_E_INVALID;
   if (!px || !py || !pz || !children || !parents || !n_mass || !n_com || !n_min || !n_max ||
       !feat)
     return COSMO_E_INVALID;
+  if (profile && (nbins < 1 || nbins > COSMO_MLP_PROFILE_MAX_BINS)) return COSMO_E_INVALID;
   const int64_t groups = (n + 63) / 64;
   k_group_features<<<blocks_for(groups, 256), 256, 0, as_stream(stream)>>>(
       px, py, pz, children, parents, n_mass, n_com, n_min, n_max, n, static_cast<float>(a),
-      nullptr, static_cast<float>(n_grid), feat);
+      a_dev, static_cast<float>(n_grid),
+      reinterpret_cast<const unsigned long long *>(profile), nbins, feat);
   COSMO_CHECK_LAUNCH();
   return COSMO_OK;
+}
+
+int cosmo_group_features(const float *px, const float *py, const float *pz,
+                         const int32_t *children, const int32_t *parents, const double *n_mass,
+                         const double *n_com, const float *n_min, const float *n_max, int64_t n,
+                         double a, double n_grid, const uint64_t *profile, int nbins,
+                         float *feat, void *stream) {
+  return group_features_impl(px, py, pz, children, parents, n_mass, n_com, n_min, n_max, n, a,
+                             nullptr, n_grid, profile, nbins, feat, stream);
 }
 
 int cosmo_group_features_dev(const float *px, const float *py, const float *pz,
                              const int32_t *children, const int32_t *parents,
                              const double *n_mass, const double *n_com, const float *n_min,
                              const float *n_max, int64_t n, const float *a, double n_grid,
-                             float *feat, void *stream) {
-  if (n < 1 || n > kMaxBodies) return COSMO_E_INVALID;
-  if (!px || !py || !pz || !children || !parents || !n_mass || !n_com || !n_min || !n_max ||
-      !a || !feat)
-    return COSMO_E_INVALID;
-  const int64_t groups = (n + 63) / 64;
-  k_group_features<<<blocks_for(groups, 256), 256, 0, as_stream(stream)>>>(
-      px, py, pz, children, parents, n_mass, n_com, n_min, n_max, n, 0.0f, a,
-      static_cast<float>(n_grid), feat);
-  COSMO_CHECK_LAUNCH();
-  return COSMO_OK;
+                             const uint64_t *profile, int nbins, float *feat, void *stream) {
+  if (!a) return COSMO_E_INVALID;
+  return group_features_impl(px, py, pz, children, parents, n_mass, n_com, n_min, n_max, n,
+                             0.0, a, n_grid, profile, nbins, feat, stream);
 }
 
 int cosmo_assemble_rows(const float *feat, int64_t groups, const float *thetas, int K,

@@ -31,7 +31,7 @@ from src.kernels import mlp_kernels as K
 from src.kernels.physics_kernels import direct_sum, order_groups, traverse
 
 DEFAULT_THETAS = (0.04, 0.05, 0.06, 0.07, 0.08, 0.1, 0.12, 0.15, 0.2, 0.25, 0.3, 0.4, 0.5, 0.6,
-                  0.8, 1.0)
+                  0.8, 1.0, 1.4, 2.0)
 LABEL_FLOOR = -40.0   # log2 of the smallest squared-error share a label keeps
 
 
@@ -161,7 +161,9 @@ def sample(engine, n_groups, thetas=DEFAULT_THETAS, seed=0, cost_kind="time", co
     G = (st.n + 63) // 64
     rng = np.random.default_rng(seed)
     groups = np.sort(rng.choice(G, size=min(n_groups, G), replace=False))
-    feat = K.group_features(st.pos, tm, engine.a, c.N_GRID)
+    prof = (K.straddle_profile(tm, engine.straddle_mass) if engine.straddle_mass > 0
+            else None)
+    feat = K.group_features(st.pos, tm, engine.a, c.N_GRID, profile=prof)
     th = torch.tensor(thetas, dtype=torch.float32, device=feat.device)
     X = K.assemble_rows(feat[torch.from_numpy(groups).to(feat.device)].contiguous(), th)
     probe = GroupProbe(engine, groups)
@@ -206,7 +208,8 @@ class LearnedTheta:
             nf = torch.zeros(1, dtype=torch.int64, device=device)
             feat = torch.empty((G, K.GROUP_FEATURES), dtype=torch.float32, device=device)
             order = torch.empty(G, dtype=torch.int32, device=device)
-            self._dev = (G, th, cost, X, Y, tg, nf, feat, order)
+            prof = torch.empty((2, 6 * K.PROFILE_BINS), dtype=torch.int64, device=device)
+            self._dev = (G, th, cost, X, Y, tg, nf, feat, order, prof)
         return self._dev
 
     def theta_groups(self, engine, lam=None, a=None):
@@ -215,8 +218,12 @@ class LearnedTheta:
         captured step).  Allocates nothing after the first call."""
         st, tm, c = engine.state, engine.tree_manager, engine.config
         G = (st.n + 63) // 64
-        _, th, cost, X, Y, tg, nf, feat, _ = self._ensure(st.pos[0].device, G)
-        K.group_features(st.pos, tm, engine.a if a is None else a, c.N_GRID, feat)
+        _, th, cost, X, Y, tg, nf, feat, _, prof = self._ensure(st.pos[0].device, G)
+        profile = None
+        if engine.straddle_mass > 0:
+            profile = K.straddle_profile(tm, engine.straddle_mass, prof[0], prof[1])
+        K.group_features(st.pos, tm, engine.a if a is None else a, c.N_GRID, feat,
+                         profile=profile)
         K.assemble_rows(feat, th, X)
         self.model.predict(X, Y)
         K.select_theta(Y, G, th, cost, self.lam if lam is None else lam, self.fallback, tg, nf)
@@ -314,38 +321,41 @@ def default_states(c):
     return [(c.SEED, 0), (c.SEED, 10), (c.SEED, 20)], (c.SEED, 30)
 
 
-def experiment(preset="repo_128", target=1e-3, mac="aabb", states=None, n_groups=2048,
-               n_val_groups=1024, fit_steps=20000, seed=0, thetas=DEFAULT_THETAS,
-               target_theta=None, lambdas=None, log=print, overrides=None):
-    """Trains the model on the GPU from samples the method generates, then
-    compares fixed theta and learned per-group theta at equal relative RMS
-    error (FP64 direct sum on held-out groups of a held-out state) by
-    interactions and by traversal time.  lambda is swept on a log grid and
-    the cheapest setting that meets the target is kept.  With target_theta
-    the error target is the fixed-theta error at that theta."""
-    from src.utils import config as cfg
-    c = cfg.preset(preset, MAC=mac, **(overrides or {}))
-    train, val = default_states(c) if states is None else states
+def train(c, states, n_groups, thetas, fit_steps, seed=0, log=print):
+    """Samples from the training states (cost of each theta timed on the
+    first), trains the network; returns (model, cost, info)."""
     Xs, Ts, costs = [], [], []
-    for k, (sd, steps) in enumerate(train):
+    for k, (sd, steps) in enumerate(states):
         eng = _engine_at(c, sd, steps)
-        # the cost of each theta is timed on the first training state
         smp = sample(eng, n_groups, thetas, seed=seed + k, cost=costs[0] if costs else None)
         Xs.append(smp.X)
         Ts.append(smp.T)
         costs.append(smp.cost)
         log(f"samples from (seed {sd}, step {steps}): {smp.X.shape[0]} rows, "
-            f"cost {np.round(smp.cost, 1)}")
+            f"cost {np.round(smp.cost, 5)}")
         del eng
     X = torch.cat(Xs)
     T = torch.cat(Ts)
-    cost = np.mean(np.stack(costs), 0)
     model = ThetaModel(seed=seed)
     t0 = time.perf_counter()
     hist = model.fit(X, T, steps=fit_steps, log=max(1, fit_steps // 10))
     torch.cuda.synchronize()
     t_fit = time.perf_counter() - t0
     log(f"fit {fit_steps} steps in {t_fit:.1f}s: loss {hist[0][1]:.3f} -> {hist[-1][1]:.3f}")
+    info = {"train_states": list(states), "train_rows": int(X.shape[0]), "fit_s": t_fit,
+            "loss_first": hist[0][1], "loss_last": hist[-1][1]}
+    return model, np.mean(np.stack(costs), 0), info
+
+
+def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0,
+             target_theta=None, lambdas=None, log=print):
+    """Fixed theta against learned per-group theta on the held-out state
+    `val` at equal relative RMS error (FP64 direct sum on sampled groups):
+    the fixed theta meeting the target is found by bisection; lambda is swept
+    on a log grid, the cheapest settings meeting the target by the cost model
+    are timed and the fastest is kept.  Reports traversal and full-step
+    times, interactions and the hindsight bound (per group the theta of least
+    cost + lambda * measured error)."""
     eng = _engine_at(c, *val)
     G = (eng.state.n + 63) // 64
     rng = np.random.default_rng(seed + 12345)
@@ -364,12 +374,10 @@ def experiment(preset="repo_128", target=1e-3, mac="aabb", states=None, n_groups
     theta_star = _bisect(lambda t: probe.rel_rms(t)[0], min(thetas) * 0.5, max(thetas), target,
                          iters=18)
     if theta_star is None:
-        return {"reachable": False, "target": target, "fixed_curve": fixed_curve}
+        return {"reachable": False, "val_state": val, "target": target,
+                "fixed_curve": fixed_curve}
     e_fixed, i_fixed = probe.rel_rms(theta_star)
     t_fixed = _time_traversal(eng, theta_star, None)
-    # hindsight bound: per validation group the theta of least cost + lambda *
-    # measured error (what a perfect predictor could pick), against the fixed
-    # theta's time
     E = np.stack(err2_k, 1)
     cost_arr = np.asarray(cost, np.float64)
     bound = None
@@ -387,7 +395,7 @@ def experiment(preset="repo_128", target=1e-3, mac="aabb", states=None, n_groups
         k = np.clip(np.searchsorted(th_np, tg.cpu().numpy()), 0, len(th_np) - 1)
         return float(cost_arr[k].mean())
 
-    lambdas = np.logspace(-2, 14, 65) if lambdas is None else lambdas
+    lambdas = np.logspace(-2, 14, 129) if lambdas is None else lambdas
     sweep = []
     for lam in lambdas:
         tg = lt.theta_groups(eng, float(lam))
@@ -395,10 +403,10 @@ def experiment(preset="repo_128", target=1e-3, mac="aabb", states=None, n_groups
         sweep.append((float(lam), e, inter, model_cost(tg)))
     ok = sorted((sw for sw in sweep if sw[1] <= target), key=lambda sw: sw[3])
     if not ok:
-        return {"reachable": False, "target": target, "theta_fixed": theta_star,
-                "sweep": sweep}
+        return {"reachable": False, "val_state": val, "target": target,
+                "theta_fixed": theta_star, "sweep": sweep}
     best = None
-    for lam, _, _, _ in ok[:3]:  # the cheapest settings by the cost model, timed
+    for lam, _, _, _ in ok[:5]:  # the cheapest settings by the cost model, timed
         tg = lt.theta_groups(eng, lam).clone()
         t = _time_traversal(eng, 0.5, tg, group_order=lt.launch_order(tg).clone())
         if best is None or t < best[1]:
@@ -417,13 +425,13 @@ def experiment(preset="repo_128", target=1e-3, mac="aabb", states=None, n_groups
     t_select = e0.elapsed_time(e1) / 5
     step_fixed = _time_steps(c.replace(THETA=theta_star), val, None)
     step_learned = _time_steps(c, val, lt)
-    res = {"reachable": True, "preset": preset, "mac": mac, "target": target,
+    hist_t = np.histogram(tg.cpu().numpy(), bins=np.concatenate([th_np - 1e-6, [np.inf]]))[0]
+    res = {"reachable": True, "preset": c.NAME, "mac": c.MAC, "target": target,
            "bucket": int(getattr(c, "BUCKET", 0)),
            "compensated": bool(getattr(c, "COMPENSATED", False)),
            "straddle_mass": float(getattr(c, "STRADDLE_MASS", 0.0)),
-           "n": eng.state.n, "train_states": train, "val_state": val,
-           "train_rows": int(X.shape[0]), "fit_s": t_fit, "loss_first": hist[0][1],
-           "loss_last": hist[-1][1], "theta_fixed": theta_star, "err_fixed": e_fixed,
+           "n": eng.state.n, "val_state": val, "val_groups": int(len(vgroups)),
+           "theta_fixed": theta_star, "err_fixed": e_fixed,
            "inter_fixed": i_fixed, "lambda": lt.lam, "err_learned": e_learned,
            "inter_learned": i_learned, "traversal_ms_fixed": t_fixed,
            "traversal_ms_learned": t_learned,
@@ -434,8 +442,32 @@ def experiment(preset="repo_128", target=1e-3, mac="aabb", states=None, n_groups
            "speedup_step": step_fixed / step_learned,
            "hindsight_bound_speedup": ((t_fixed / G) / bound[1] if bound else None),
            "interaction_ratio": i_fixed / i_learned,
-           "theta_group_mean": float(tg.mean().item()), "fallbacks": lt.n_fallback}
+           "theta_group_mean": float(tg.mean().item()),
+           "theta_group_histogram": dict(zip([float(t) for t in thetas], hist_t.tolist())),
+           "fallbacks": lt.n_fallback}
     return res, lt
+
+
+def experiment(preset="repo_128", target=1e-3, mac="aabb", states=None, n_groups=2048,
+               n_val_groups=1024, fit_steps=20000, seed=0, thetas=DEFAULT_THETAS,
+               target_theta=None, lambdas=None, log=print, overrides=None, val_states=None):
+    """Trains the network on the GPU from samples the method generates on the
+    training states, then evaluates it (see evaluate) on each held-out
+    state.  Returns [(result, LearnedTheta or None)] per held-out state and
+    the training info."""
+    from src.utils import config as cfg
+    c = cfg.preset(preset, MAC=mac, **(overrides or {}))
+    train_states, val = default_states(c) if states is None else states
+    val_states = [val] if val_states is None else list(val_states)
+    model, cost, info = train(c, train_states, n_groups, thetas, fit_steps, seed, log)
+    out = []
+    for v in val_states:
+        r = evaluate(c, model, cost, thetas, v, target, n_val_groups, seed, target_theta,
+                     lambdas, log)
+        res, lt = r if isinstance(r, tuple) else (r, None)
+        res.update(info)
+        out.append((res, lt))
+    return out
 
 
 def main():
@@ -446,9 +478,11 @@ def main():
     ap.add_argument("--mac", default="aabb")
     ap.add_argument("--target", type=float, default=1e-3)
     ap.add_argument("--fit-steps", type=int, default=20000)
-    ap.add_argument("--groups", type=int, default=2048)
+    ap.add_argument("--groups", type=int, default=10240)
     ap.add_argument("--target-theta", type=float, default=None)
     ap.add_argument("--val-groups", type=int, default=1024)
+    ap.add_argument("--val-seeds", default="",
+                    help="comma-separated seeds of extra held-out states (step 0)")
     ap.add_argument("--bucket", type=int, default=None)
     ap.add_argument("--compensated", type=int, default=None)
     ap.add_argument("--straddle", type=float, default=None)
@@ -461,17 +495,18 @@ def main():
         over["COMPENSATED"] = bool(args.compensated)
     if args.straddle is not None:
         over["STRADDLE_MASS"] = args.straddle
+    from src.utils import config as cfg
+    c = cfg.preset(args.preset, MAC=args.mac, **over)
+    train_states, val = default_states(c)
+    vals = [val] + [(int(x), 0) for x in args.val_seeds.split(",") if x]
     out = experiment(args.preset, args.target, args.mac, fit_steps=args.fit_steps,
                      n_groups=args.groups, n_val_groups=args.val_groups,
                      target_theta=args.target_theta, log=lambda m: print(m, flush=True),
-                     overrides=over)
-    if isinstance(out, tuple):
-        res, lt = out
-        if args.save:
-            lt.save(args.save)
-    else:
-        res = out
-    print(json.dumps(res, default=float), flush=True)
+                     overrides=over, val_states=vals)
+    for res, lt in out:
+        print(json.dumps(res, default=float), flush=True)
+    if args.save and out[0][1] is not None:
+        out[0][1].save(args.save)
 
 
 if __name__ == "__main__":

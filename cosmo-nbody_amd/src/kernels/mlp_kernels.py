@@ -18,10 +18,35 @@ GRAD_BLOCKS = 64
 _F32, _I16 = torch.float32, torch.int16
 
 
-def group_features(pos, tree, a, n_grid, out=None):
+PROFILE_BINS = 1024
+
+
+def straddle_profile(tree, straddle_mass, profile=None, scratch=None, nbins=PROFILE_BINS):
+    """Straddle profile (uint64 [2, 3, nbins]) of the tree for the AABB
+    test's straddle limit; see cosmo_straddle_profile."""
+    n = tree.n
+    M = 2 * n - 1
+    dev = tree.child.device
+    profile = (torch.empty(6 * nbins, dtype=torch.int64, device=dev) if profile is None
+               else profile)
+    scratch = (torch.empty(6 * nbins, dtype=torch.int64, device=dev) if scratch is None
+               else scratch)
+    check(load().cosmo_straddle_profile(
+        ptr(tree.parent, torch.int32, "parents", numel=M),
+        ptr(tree.mass, torch.float64, "n_mass", numel=M),
+        ptr(tree.aabb_min, _F32, "n_min", numel=3 * M),
+        ptr(tree.aabb_max, _F32, "n_max", numel=3 * M), n, float(straddle_mass),
+        ptr(profile, (torch.int64, torch.uint64), "profile", numel=6 * nbins),
+        ptr(scratch, (torch.int64, torch.uint64), "scratch", numel=6 * nbins), int(nbins),
+        stream_handle()), "straddle_profile")
+    return profile
+
+
+def group_features(pos, tree, a, n_grid, out=None, profile=None, nbins=PROFILE_BINS):
     """Nine features per group of 64 sorted particles (tree: OctreeManager).
     a: the scale factor, a float or a device float tensor (read at run time,
-    as a captured step needs)."""
+    as a captured step needs).  profile: the straddle profile (features 7
+    and 8), or None (both 0)."""
     n = pos[0].shape[0]
     G = (n + 63) // 64
     out = torch.empty((G, GROUP_FEATURES), dtype=_F32, device=pos[0].device) if out is None else out
@@ -29,6 +54,8 @@ def group_features(pos, tree, a, n_grid, out=None):
     on_dev = isinstance(a, torch.Tensor)
     fn = load().cosmo_group_features_dev if on_dev else load().cosmo_group_features
     a_arg = ptr(a, _F32, "a", numel=1) if on_dev else float(a)
+    prof = None if profile is None else ptr(profile, (torch.int64, torch.uint64), "profile",
+                                            numel=6 * nbins)
     check(fn(
         *[ptr(t, _F32, "pos", numel=n) for t in pos],
         ptr(tree.child, torch.int32, "children", numel=2 * M),
@@ -36,8 +63,9 @@ def group_features(pos, tree, a, n_grid, out=None):
         ptr(tree.mass, torch.float64, "n_mass", numel=M),
         ptr(tree.com, torch.float64, "n_com", numel=3 * M),
         ptr(tree.aabb_min, _F32, "n_min", numel=3 * M),
-        ptr(tree.aabb_max, _F32, "n_max", numel=3 * M), n, a_arg, float(n_grid),
-        ptr(out, _F32, "feat", numel=G * GROUP_FEATURES), stream_handle()), "group_features")
+        ptr(tree.aabb_max, _F32, "n_max", numel=3 * M), n, a_arg, float(n_grid), prof,
+        int(nbins), ptr(out, _F32, "feat", numel=G * GROUP_FEATURES), stream_handle()),
+        "group_features")
     return out
 
 

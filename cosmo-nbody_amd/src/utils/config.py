@@ -10,10 +10,12 @@ a dataclass with the same attribute names plus the knobs this build adds;
   small_32            N = 32^3 Zel'dovich ICs, theta 0.5, eps 0.02/32, 5 steps
   repo_128            N = 128^3, theta 0.5, eps 0.02/128, 20 steps, 1024^2 splat
   clustered_128       N = 128^3 evolved 300 steps, theta swept (precision study)
-  learned_128         N = 128^3 with the learned per-group opening angle
+  learned_128         N = 128^3 Zel'dovich ICs, AABB test with 32-leaf buckets and
+                      compensated sums, for the learned per-group opening angle
   large_256           N = 256^3, 50 steps, 2048^2 splat, learned theta
   halos_128           N = 128^3 heterogeneous test state (half in 64 Plummer
-                      spheres, half uniform), AABB acceptance test
+                      spheres, half uniform), AABB acceptance test with
+                      32-leaf buckets, compensated sums, straddle mass 1e-4
 
 Time stepping follows the reference schedule: da = (A_END - A_START) /
 N_STEPS with N_STEPS = 2500 (src/core/engine.py:46); RUN_STEPS is how many of
@@ -137,11 +139,12 @@ PRESETS = {
     "small_32": lambda: _plain(32, RUN_STEPS=5, NAME="small_32"),
     "repo_128": lambda: _plain(128, RUN_STEPS=20, RES=1024, NAME="repo_128"),
     "clustered_128": lambda: _plain(128, RUN_STEPS=300, NAME="clustered_128"),
-    "learned_128": lambda: _plain(128, RUN_STEPS=20, LEARNED_THETA=True, NAME="learned_128"),
+    "learned_128": lambda: _plain(128, RUN_STEPS=20, LEARNED_THETA=True, MAC="aabb", BUCKET=32,
+                                  COMPENSATED=True, NAME="learned_128"),
     "large_256": lambda: _plain(256, RUN_STEPS=50, RES=2048, LEARNED_THETA=True,
                                 NAME="large_256"),
-    "halos_128": lambda: _plain(128, RUN_STEPS=20, MAC="aabb", IC_KIND="halos",
-                                NAME="halos_128"),
+    "halos_128": lambda: _plain(128, RUN_STEPS=20, MAC="aabb", IC_KIND="halos", BUCKET=32,
+                                COMPENSATED=True, STRADDLE_MASS=1e-4, NAME="halos_128"),
 }
 
 

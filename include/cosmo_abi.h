@@ -94,7 +94,10 @@ typedef struct cosmo_inode_aabb {
   int32_t first, count;
 } cosmo_inode_aabb;
 
-/* Size in bytes of the packed records of an n-leaf tree for test `mac`. */
+/* Size in bytes of the packed records of an n-leaf tree for test `mac`:
+ * the n leaf and n-1 internal records plus one internal record of tail
+ * room (the traversal loads a whole internal-record size at every node,
+ * leaves included, and keeps only the leaf's 16 bytes). */
 int64_t cosmo_tree_bytes(int64_t n, int mac);
 
 typedef struct cosmo_ctx cosmo_ctx;

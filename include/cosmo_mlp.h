@@ -33,18 +33,35 @@ extern "C" {
 #define COSMO_MLP_PARAMS (16 * 64 + 64 + 64 * 64 + 64 + 64 + 1)
 #define COSMO_MLP_BF16_WEIGHTS (16 * 64 + 64 * 64 + 64)
 
+#define COSMO_MLP_PROFILE_MAX_BINS 65536
+
+/* Straddle profile for the AABB test's straddle limit: the straddle-level
+ * nodes are the internal nodes lighter than straddle_mass whose parent is
+ * not; along each axis, bin b (of nbins over [0, 1)) holds the sums of
+ * (M/straddle_mass)^2 and of M/straddle_mass over the straddle-level nodes
+ * whose box spans the bin, as integers in units of 2^-32.  profile and
+ * scratch: device uint64[2 * 3 * nbins] each, layout [sum][axis][bin]. */
+int cosmo_straddle_profile(const int32_t *parents, const double *n_mass,
+                           const float *n_min, const float *n_max, int64_t n,
+                           double straddle_mass, uint64_t *profile,
+                           uint64_t *scratch, int nbins, void *stream);
+
 /* Per-group features feat[g][9] for groups g = 0..ceil(n/64)-1 from the
  * sorted positions and the tree (reference layouts; FP64 normalised CoM):
  *  0 log2(group extent * n_grid)        1 log2(relative number density)
  *  2 scale factor a
- *  3,5,7 extent/distance of the sibling of the group's node and of its
- *        two next ancestors' siblings   4,6,8 log2 of their mass * n / cnt */
+ *  3,5 extent/distance of the sibling of the group's node and of the next
+ *      ancestor's sibling               4,6 log2 of their mass * n / cnt
+ *  7,8 log2(1 + S2), log2(1 + S1): the straddle profile (nullable: 0)
+ *      summed over the group's three minimum-image boundaries, the bins at
+ *      the group centre + 1/2 in each axis. */
 int cosmo_group_features(const float *pos_x, const float *pos_y,
                          const float *pos_z, const int32_t *children,
                          const int32_t *parents, const double *n_mass,
                          const double *n_com, const float *n_min,
                          const float *n_max, int64_t n, double a,
-                         double n_grid, float *feat, void *stream);
+                         double n_grid, const uint64_t *profile, int nbins,
+                         float *feat, void *stream);
 
 /* cosmo_group_features with the scale factor read from device memory
  * (a[0]), for callers that replay a captured step. */
@@ -53,7 +70,8 @@ int cosmo_group_features_dev(const float *pos_x, const float *pos_y,
                              const int32_t *parents, const double *n_mass,
                              const double *n_com, const float *n_min,
                              const float *n_max, int64_t n, const float *a,
-                             double n_grid, float *feat, void *stream);
+                             double n_grid, const uint64_t *profile, int nbins,
+                             float *feat, void *stream);
 
 /* X[g*K + k][0..15] = BF16 [thetas[k], feat[g][0..8], 0...]. */
 int cosmo_assemble_rows(const float *feat, int64_t groups,

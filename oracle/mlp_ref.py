@@ -102,8 +102,35 @@ def loss_and_grad(X, T, p, rounding=True):
     return float(np.mean(err ** 2)), g, float(np.sum(err ** 2))
 
 
-def group_features(pos, children, parents, n_mass, n_com, n_min, n_max, a, n_grid):
-    """Mirror of k_group_features (FP32 arithmetic where the kernel uses it)."""
+def straddle_profile(parents, n_mass, n_min, n_max, straddle_mass, nbins=1024):
+    """Mirror of cosmo_straddle_profile: uint64 [2, 3, nbins] sums of
+    (M/limit)^2 and M/limit (units of 2^-32) over the straddle-level nodes
+    (lighter than the limit, parent not) spanning each bin."""
+    n = (len(n_mass) + 1) // 2
+    prof = np.zeros((2, 3, nbins), np.uint64)
+    k = np.arange(n, 2 * n - 1)
+    m = np.asarray(n_mass, np.float64)[k]
+    par = np.asarray(parents)[k]
+    pm = np.where(par >= 0, np.asarray(n_mass, np.float64)[np.maximum(par, 0)], np.inf)
+    sel = (m < straddle_mass) & (pm >= straddle_mass)
+    k, m = k[sel], m[sel]
+    r = m / straddle_mass
+    w = [(r * r * 4294967296.0).astype(np.uint64), (r * 4294967296.0).astype(np.uint64)]
+    for ax in range(3):
+        b0 = (np.asarray(n_min, np.float32)[k, ax] * np.float32(nbins)).astype(np.int64)
+        b1 = (np.asarray(n_max, np.float32)[k, ax] * np.float32(nbins)).astype(np.int64)
+        b0 = np.clip(b0, 0, nbins - 1)
+        b1 = np.clip(b1, 0, nbins - 1)
+        for q in range(2):
+            for j in range(len(k)):
+                prof[q, ax, b0[j]:b1[j] + 1] += w[q][j]
+    return prof
+
+
+def group_features(pos, children, parents, n_mass, n_com, n_min, n_max, a, n_grid,
+                   profile=None):
+    """Mirror of k_group_features (FP32 arithmetic where the kernel uses it);
+    profile: straddle_profile's output, or None."""
     px, py, pz = (np.asarray(v, np.float32) for v in pos)
     n = len(px)
     G = (n + 63) // 64
@@ -131,7 +158,7 @@ def group_features(pos, children, parents, n_mass, n_com, n_min, n_max, a, n_gri
             if p < 0:
                 break
             node = p
-        for lvl in range(3):
+        for lvl in range(2):
             p = parents[node]
             r = mlog = 0.0
             if p >= 0:
@@ -146,4 +173,15 @@ def group_features(pos, children, parents, n_mass, n_com, n_min, n_max, a, n_gri
                 node = p
             out[g, 3 + 2 * lvl] = r
             out[g, 4 + 2 * lvl] = mlog
+        s2 = s1 = f32(0)
+        if profile is not None:
+            nb = profile.shape[2]
+            for ax in range(3):
+                x = f32(cen[ax] + f32(0.5))
+                x = f32(x - np.floor(x))
+                b = min(max(int(f32(x * f32(nb))), 0), nb - 1)
+                s2 = f32(s2 + f32(profile[0, ax, b]))
+                s1 = f32(s1 + f32(profile[1, ax, b]))
+        out[g, 7] = np.log2(f32(1) + f32(s2 * f32(2.3283064365386963e-10)))
+        out[g, 8] = np.log2(f32(1) + f32(s1 * f32(2.3283064365386963e-10)))
     return out

@@ -133,11 +133,19 @@ def test_group_features_match_mirror():
     st.sort_buffer_by_morton(cfg.preset("small_32"))
     tree = OctreeManager(n)
     tree.build(st.pos, st.mass, st.codes)
-    f = K.group_features(st.pos, tree, 0.02, 16).cpu().numpy()
     h = lambda t: t.cpu().numpy()
-    ref = M.group_features([h(t) for t in st.pos], h(tree.child), h(tree.parent), h(tree.mass),
-                           h(tree.com), h(tree.aabb_min), h(tree.aabb_max), 0.02, 16)
-    assert np.allclose(f, ref, rtol=1e-5, atol=1e-5)
+    for mu in (0.0, 3e-3, 2e-2):
+        prof = K.straddle_profile(tree, mu) if mu > 0 else None
+        f = K.group_features(st.pos, tree, 0.02, 16, profile=prof).cpu().numpy()
+        ref_prof = (M.straddle_profile(h(tree.parent), h(tree.mass), h(tree.aabb_min),
+                                       h(tree.aabb_max), mu) if mu > 0 else None)
+        if mu > 0:
+            assert np.array_equal(h(prof).view(np.uint64).reshape(2, 3, -1), ref_prof)
+        ref = M.group_features([h(t) for t in st.pos], h(tree.child), h(tree.parent),
+                               h(tree.mass), h(tree.com), h(tree.aabb_min), h(tree.aabb_max),
+                               0.02, 16, profile=ref_prof)
+        assert np.allclose(f, ref, rtol=1e-5, atol=1e-5)
+        assert (mu == 0) == (not np.any(f[:, 7:]))
 
 
 @pytest.mark.gpu
