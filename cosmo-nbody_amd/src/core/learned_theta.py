@@ -30,6 +30,8 @@ import torch
 from src.kernels import mlp_kernels as K
 from src.kernels.physics_kernels import direct_sum, order_groups, traverse
 
+WAVES_PER_SIMD = 8     # resident wavefronts per SIMD of the traversal kernel (its VGPR use)
+WAVE_BUDGET = 0.5      # largest single-wavefront time allowed, as a fraction of the fixed traversal
 DEFAULT_THETAS = (0.04, 0.05, 0.06, 0.07, 0.08, 0.1, 0.12, 0.15, 0.2, 0.25, 0.3, 0.4, 0.5, 0.6,
                   0.8, 1.0, 1.4, 2.0)
 LABEL_FLOOR = -40.0   # log2 of the smallest squared-error share a label keeps
@@ -187,12 +189,15 @@ def sample(engine, n_groups, thetas=DEFAULT_THETAS, seed=0, cost_kind="time", co
 
 
 class LearnedTheta:
-    """Selects engine.theta_group each step from the model's predictions."""
+    """Selects engine.theta_group each step from the model's predictions.
+    theta_min drops the candidates below it: one group at a very small theta
+    is a single wavefront whose duration bounds the whole traversal."""
 
-    def __init__(self, model, thetas, cost, lam, fallback):
+    def __init__(self, model, thetas, cost, lam, fallback, theta_min=0.0):
+        keep = [k for k, t in enumerate(thetas) if t >= theta_min]
         self.model = model
-        self.thetas_list = list(thetas)
-        self.cost_np = np.asarray(cost, np.float32)
+        self.thetas_list = [thetas[k] for k in keep]
+        self.cost_np = np.asarray(cost, np.float32)[keep]
         self.lam = float(lam)
         self.fallback = float(fallback)
         self._dev = None
@@ -387,13 +392,23 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
         mc = float(cost_arr[k].mean())
         if e <= target and (bound is None or mc < bound[1]):
             bound = (float(lam), mc)
-    lt = LearnedTheta(model, thetas, cost, 1.0, theta_star)
+    # Candidates whose average single-wavefront time would exceed half the
+    # fixed-theta traversal are dropped: the traversal cannot end before its
+    # slowest wavefront.  All groups at theta_k take cost_k * G ms with
+    # `concurrent` wavefronts resident at a time.
+    props = torch.cuda.get_device_properties(eng.state.pos[0].device)
+    concurrent = props.multi_processor_count * 4 * WAVES_PER_SIMD
+    wave_ms = cost_arr * concurrent
+    ok_k = [k for k in range(len(thetas)) if wave_ms[k] <= WAVE_BUDGET * t_fixed]
+    theta_min = float(thetas[ok_k[0]]) if ok_k else float(thetas[-1])
+    lt = LearnedTheta(model, thetas, cost, 1.0, theta_star, theta_min=theta_min)
     vg = torch.from_numpy(vgroups).to(eng.state.pos[0].device)
-    th_np = np.asarray(thetas, np.float32)
+    th_np = np.asarray(lt.thetas_list, np.float32)
+    cost_sel = np.asarray(lt.cost_np, np.float64)
 
     def model_cost(tg):
         k = np.clip(np.searchsorted(th_np, tg.cpu().numpy()), 0, len(th_np) - 1)
-        return float(cost_arr[k].mean())
+        return float(cost_sel[k].mean())
 
     lambdas = np.logspace(-2, 14, 129) if lambdas is None else lambdas
     sweep = []
@@ -442,8 +457,9 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
            "speedup_step": step_fixed / step_learned,
            "hindsight_bound_speedup": ((t_fixed / G) / bound[1] if bound else None),
            "interaction_ratio": i_fixed / i_learned,
-           "theta_group_mean": float(tg.mean().item()),
-           "theta_group_histogram": dict(zip([float(t) for t in thetas], hist_t.tolist())),
+           "theta_group_mean": float(tg.mean().item()), "theta_min": theta_min,
+           "theta_group_histogram": dict(zip([float(t) for t in lt.thetas_list],
+                                             hist_t.tolist())),
            "fallbacks": lt.n_fallback}
     return res, lt
 
