@@ -49,14 +49,10 @@ __device__ __forceinline__ float wrap_tie(float c, float p, float r) {
 }
 
 // Keeps the compiler from hoisting work on v out of a rarely taken branch
-// (an empty asm that claims to modify v; emits no instruction).  Only for
-// wave-uniform values (node records): the constraint keeps v in an SGPR, so
-// the record stays a scalar load (a VGPR constraint would turn the load of v
-// into a vector load with a wait on every pop).
+// (an empty asm that claims to modify v; emits no instruction).
 __device__ __forceinline__ float pin(float v) {
-  int u = __builtin_amdgcn_readfirstlane(__float_as_int(v));
-  asm volatile("" : "+s"(u));
-  return __int_as_float(u);
+  asm volatile("" : "+v"(v));
+  return v;
 }
 
 // Writes `value` into lane `slot` of v (v_writelane ignores EXEC).  The slot
@@ -79,6 +75,13 @@ __device__ __forceinline__ void neumaier_add(float &s, float &c, float x) {
   s = t;
 }
 
+// float in the constant address space: uniform loads through it are scalar
+// loads (the packed records are read-only for the whole traversal).
+typedef const __attribute__((address_space(4))) float ConstF;
+
+__device__ __forceinline__ float4 ld4(const ConstF *p) {
+  return make_float4(p[0], p[1], p[2], p[3]);
+}
 
 template <int MAC>
 struct InodeOf;
@@ -170,8 +173,6 @@ __global__ __launch_bounds__(256) void k_traverse(
     unsigned long long *__restrict__ stats) {
   typedef typename InodeOf<MAC>::type Inode;
   const float4 *__restrict__ leaves = reinterpret_cast<const float4 *>(tree);
-  const char *__restrict__ ibase =
-      reinterpret_cast<const char *>(tree) + int64_t(n_tree) * sizeof(cosmo_leaf);
   const int lane = threadIdx.x & (kWave - 1);
   // slot = launch position of this wavefront; wave = the group it takes
   const int slot = __builtin_amdgcn_readfirstlane(
@@ -232,31 +233,31 @@ __global__ __launch_bounds__(256) void k_traverse(
       in[h] = depth[h] > top;
       depth[h] = min(depth[h], top);
     }
-    // Leaf and internal records start with (x, y, z, m).  For a leaf the
-    // other fields copy A, and for the reference test B.x (sx^2) = -1, so
-    // that the test accepts it.  The loads are
-    // wave-uniform; the compiler issues the fields it keeps in SGPRs as
-    // scalar loads and the x coordinate, which both kinds share, as one
-    // vector load after the branch, so the two overlap.  (One scalar load
-    // of the whole record for both kinds, read through the constant address
-    // space, measured slower: 6.78 against 6.13 ms at 128^3.)
+    // Leaf and internal records start with (x, y, z, m).  The loads are
+    // wave-uniform (scalar cache).  For a leaf the other fields copy A, and
+    // for the reference test B.x (sx^2) = -1, so that the test accepts it.
+    // One wave-uniform byte offset for both kinds (a selected offset, not a
+    // pointer joined from two branches) read through the constant address
+    // space (the records are not written while a traversal runs): every
+    // field is then a scalar load, none a vector load with a wait.  Measured
+    // on one MI355X: 6.73 ms at 128^3 against 7.26 ms for the compiler's
+    // mix of scalar loads and a vector load of the shared x coordinate.
     const bool leaf = node < n_tree;
-    float4 A, B, C, D;
+    const int64_t ofs = leaf ? int64_t(node) * int64_t(sizeof(cosmo_leaf))
+                             : int64_t(n_tree) * int64_t(sizeof(cosmo_leaf)) +
+                                   int64_t(node - n_tree) * int64_t(sizeof(Inode));
+    // The whole internal-record size is loaded at once for either kind (one
+    // scalar round trip per pop; the record buffer has tail room for the
+    // last leaf), then a leaf keeps only its 16 bytes.
+    const ConstF *rec = (const ConstF *)(reinterpret_cast<const char *>(tree) + ofs);
+    float4 A = ld4(rec), B = ld4(rec + 4), C = A, D = A;
+    if constexpr (MAC == COSMO_MAC_AABB) {
+      C = ld4(rec + 8);
+      D = ld4(rec + 12);
+    }
     if (leaf) {
-      A = leaves[node];
       B = C = D = A;
       if constexpr (MAC == COSMO_MAC_REF) B.x = -1.0f;
-    } else {
-      const float4 *rec =
-          reinterpret_cast<const float4 *>(ibase + int64_t(node - n_tree) * sizeof(Inode));
-      A = rec[0];
-      B = rec[1];
-      if constexpr (MAC == COSMO_MAC_AABB) {
-        C = rec[2];
-        D = rec[3];
-      } else {
-        C = D = A;
-      }
     }
     // massless nodes are skipped (integer test on the uniform bits: m <= 0)
     if (__float_as_int(A.w) <= 0) continue;

@@ -499,6 +499,9 @@ def main():
     ap.add_argument("--val-groups", type=int, default=1024)
     ap.add_argument("--val-seeds", default="",
                     help="comma-separated seeds of extra held-out states (step 0)")
+    ap.add_argument("--train-seeds", default="",
+                    help="comma-separated seeds of the training states (step 0) in place of "
+                         "the preset's")
     ap.add_argument("--bucket", type=int, default=None)
     ap.add_argument("--compensated", type=int, default=None)
     ap.add_argument("--straddle", type=float, default=None)
@@ -514,11 +517,13 @@ def main():
     from src.utils import config as cfg
     c = cfg.preset(args.preset, MAC=args.mac, **over)
     train_states, val = default_states(c)
+    if args.train_seeds:
+        train_states = [(int(x), 0) for x in args.train_seeds.split(",") if x]
     vals = [val] + [(int(x), 0) for x in args.val_seeds.split(",") if x]
     out = experiment(args.preset, args.target, args.mac, fit_steps=args.fit_steps,
                      n_groups=args.groups, n_val_groups=args.val_groups,
                      target_theta=args.target_theta, log=lambda m: print(m, flush=True),
-                     overrides=over, val_states=vals)
+                     overrides=over, val_states=vals, states=(train_states, val))
     for res, lt in out:
         print(json.dumps(res, default=float), flush=True)
     if args.save and out[0][1] is not None:
