@@ -30,6 +30,7 @@ import torch
 from src.kernels import mlp_kernels as K
 from src.kernels.physics_kernels import direct_sum, order_groups, traverse
 
+N_MODELS = 4           # networks averaged by the selection
 WAVES_PER_SIMD = 8     # resident wavefronts per SIMD of the traversal kernel (its VGPR use)
 WAVE_BUDGET = 0.5      # largest single-wavefront time allowed, as a fraction of the fixed traversal
 DEFAULT_THETAS = (0.04, 0.05, 0.06, 0.07, 0.08, 0.1, 0.12, 0.15, 0.2, 0.25, 0.3, 0.4, 0.5, 0.6,
@@ -89,6 +90,34 @@ class ThetaModel:
             if log is not None and (s % log == 0 or s == steps - 1):
                 hist.append((s, float(g[-1].item()) / xb.shape[0]))
         return hist
+
+
+class Ensemble:
+    """Several networks trained on the same samples from different initial
+    weights and minibatch orders; predicts the mean of their log2 errors
+    (a non-finite prediction of any member makes the mean non-finite, so
+    the fixed-theta fallback still applies).  Allocates only on the first
+    call for a given number of rows."""
+
+    def __init__(self, models):
+        self.models = list(models)
+        self._buf = None
+
+    @property
+    def params(self):
+        return torch.stack([m.params for m in self.models])
+
+    def predict(self, X, out=None):
+        rows = X.shape[0]
+        out = self.models[0].predict(X, out)
+        if len(self.models) > 1:
+            if self._buf is None or self._buf.shape[0] != rows:
+                self._buf = torch.empty(rows, dtype=torch.float32, device=X.device)
+            for m in self.models[1:]:
+                m.predict(X, self._buf)
+                out.add_(self._buf)
+            out.mul_(1.0 / len(self.models))
+        return out
 
 
 @dataclasses.dataclass
@@ -250,7 +279,11 @@ class LearnedTheta:
     @classmethod
     def load(cls, path, device="cuda"):
         d = np.load(path, allow_pickle=False)
-        model = ThetaModel(torch.from_numpy(d["params"]), device=device)
+        p = torch.from_numpy(d["params"])
+        if p.dim() == 2:
+            model = Ensemble([ThetaModel(q, device=device) for q in p])
+        else:
+            model = ThetaModel(p, device=device)
         return cls(model, d["thetas"].tolist(), d["cost"], float(d["lam"]), float(d["fallback"]))
 
 
@@ -326,9 +359,10 @@ def default_states(c):
     return [(c.SEED, 0), (c.SEED, 10), (c.SEED, 20)], (c.SEED, 30)
 
 
-def train(c, states, n_groups, thetas, fit_steps, seed=0, log=print):
+def train(c, states, n_groups, thetas, fit_steps, seed=0, log=print, n_models=N_MODELS):
     """Samples from the training states (cost of each theta timed on the
-    first), trains the network; returns (model, cost, info)."""
+    first), trains n_models networks (an Ensemble when more than one);
+    returns (model, cost, info)."""
     Xs, Ts, costs = [], [], []
     for k, (sd, steps) in enumerate(states):
         eng = _engine_at(c, sd, steps)
@@ -341,14 +375,20 @@ def train(c, states, n_groups, thetas, fit_steps, seed=0, log=print):
         del eng
     X = torch.cat(Xs)
     T = torch.cat(Ts)
-    model = ThetaModel(seed=seed)
+    models, hists = [], []
     t0 = time.perf_counter()
-    hist = model.fit(X, T, steps=fit_steps, log=max(1, fit_steps // 10))
+    for i in range(n_models):
+        m = ThetaModel(seed=seed + i)
+        hists.append(m.fit(X, T, steps=fit_steps, seed=seed + i, log=max(1, fit_steps // 10)))
+        models.append(m)
     torch.cuda.synchronize()
     t_fit = time.perf_counter() - t0
-    log(f"fit {fit_steps} steps in {t_fit:.1f}s: loss {hist[0][1]:.3f} -> {hist[-1][1]:.3f}")
+    first = float(np.mean([h[0][1] for h in hists]))
+    last = float(np.mean([h[-1][1] for h in hists]))
+    log(f"fit {n_models} x {fit_steps} steps in {t_fit:.1f}s: loss {first:.3f} -> {last:.3f}")
     info = {"train_states": list(states), "train_rows": int(X.shape[0]), "fit_s": t_fit,
-            "loss_first": hist[0][1], "loss_last": hist[-1][1]}
+            "n_models": n_models, "loss_first": first, "loss_last": last}
+    model = models[0] if n_models == 1 else Ensemble(models)
     return model, np.mean(np.stack(costs), 0), info
 
 

@@ -81,3 +81,20 @@ def test_poisoned_network_falls_back_to_fixed_theta():
     e_fixed, i_fixed = probe.rel_rms(0.45)
     assert torch.equal(f_learned, probe._force) and i_learned == i_fixed
     assert math.isfinite(e_fixed)
+
+
+def test_ensemble_predicts_the_mean():
+    import torch
+    from src.core import learned_theta as L
+    from src.kernels import mlp_kernels as K
+    models = [L.ThetaModel(seed=s) for s in range(3)]
+    ens = L.Ensemble(models)
+    g = torch.Generator().manual_seed(5)
+    feat = torch.randn(100, K.GROUP_FEATURES, generator=g).cuda()
+    th = torch.tensor([0.2, 0.5, 0.9], dtype=torch.float32, device="cuda")
+    X = K.assemble_rows(feat, th)
+    y = ens.predict(X).clone()
+    ref = torch.stack([m.predict(X) for m in models]).mean(0)
+    assert torch.allclose(y, ref, rtol=1e-6, atol=1e-6)
+    models[1].params[-1] = float("nan")
+    assert torch.isnan(ens.predict(X)).all()
