@@ -87,6 +87,73 @@ __global__ __launch_bounds__(256) void k_render_density_dev(const float *__restr
   splat_one(px[i], py[i], pz[i], m, res, scale, grid);
 }
 
+// Colour mapping of a density frame (src/utils/visualizer.py:115-171), fused
+// after the splat: img = log1p(120 x) / (max(img) + 1e-9), val = img^2.2,
+// the four-band palette, x1.3, clipped to [0, 1]; x is the fixed-point bin
+// in FP32.  log1p is increasing, so max(img) = log1p(120 max x) and the
+// reduction is an exact integer maximum over the bins.  The FP32 constants
+// are those the reference's float32 arrays meet: T1 = 0.6, T2 = 0.72,
+// T3 = 0.82 and the band widths as rounded from their FP64 differences.
+__global__ void k_grid_max_clear(unsigned long long *mx) {
+  if (threadIdx.x == 0) mx[0] = 0ull;
+}
+
+__global__ __launch_bounds__(256) void k_grid_max(const unsigned long long *__restrict__ grid,
+                                                  int64_t len, unsigned long long *mx) {
+  __shared__ unsigned long long part[256];
+  unsigned long long m = 0ull;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < len;
+       i += int64_t(gridDim.x) * blockDim.x)
+    m = grid[i] > m ? grid[i] : m;
+  part[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      const unsigned long long o = part[threadIdx.x + w];
+      part[threadIdx.x] = o > part[threadIdx.x] ? o : part[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(mx, part[0]);
+}
+
+__global__ __launch_bounds__(256) void k_colorize(const unsigned long long *__restrict__ grid,
+                                                  int64_t len, float scale,
+                                                  const unsigned long long *__restrict__ mx,
+                                                  float *__restrict__ frame,
+                                                  uint8_t *__restrict__ rgb) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  const float top = log1pf((static_cast<float>(mx[0]) * scale) * 120.0f);
+  const float img = log1pf((static_cast<float>(grid[i]) * scale) * 120.0f) / (top + 1e-9f);
+  const float val = powf(img, 2.2f);
+  const float T1 = 0.6f, T2 = 0.72f, T3 = 0.82f;
+  float r = 0.0f, g = 0.0f, b = 0.0f;
+  if (val <= T1) {
+    b = (val / T1) * 0.8f;
+  } else if (val <= T2) {
+    const float t = (val - T1) / 0.12f;
+    b = (1.0f - t) * 0.8f;
+    g = t * 0.9f;
+  } else if (val <= T3) {
+    const float t = (val - T2) / 0.09999999999999998f;
+    g = 0.9f + t * 0.1f;
+    r = t;
+  } else {
+    const float t = (val - T3) / 0.18000000000000005f;
+    r = 1.0f;
+    g = 1.0f - t;
+  }
+  const float c[3] = {fminf(fmaxf(r * 1.3f, 0.0f), 1.0f), fminf(fmaxf(g * 1.3f, 0.0f), 1.0f),
+                      fminf(fmaxf(b * 1.3f, 0.0f), 1.0f)};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (frame) frame[3 * i + k] = c[k];
+    // engine.py:139: (frame * 255).astype(uint8), a truncation
+    if (rgb) rgb[3 * i + k] = static_cast<uint8_t>(c[k] * 255.0f);
+  }
+}
+
 }  // namespace cosmo
 
 using namespace cosmo;
@@ -154,6 +221,23 @@ int cosmo_render_density_dev(const float *px, const float *py, const float *pz, 
   k_render_density_dev<<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(
       px, py, pz, reinterpret_cast<unsigned long long *>(grid), mvp_dev, res,
       ldexpf(1.0f, frac_bits), n);
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+int cosmo_colorize_frame(const uint64_t *grid, int32_t res, int frac_bits, uint64_t *max_scratch,
+                         float *frame, uint8_t *rgb, void *stream) {
+  if (res < 1 || res > 65536 || frac_bits < 0 || frac_bits > 30) return COSMO_E_INVALID;
+  if (!grid || !max_scratch || (!frame && !rgb)) return COSMO_E_INVALID;
+  hipStream_t s = as_stream(stream);
+  const int64_t len = int64_t(res) * res;
+  unsigned long long *mx = reinterpret_cast<unsigned long long *>(max_scratch);
+  const unsigned long long *g = reinterpret_cast<const unsigned long long *>(grid);
+  k_grid_max_clear<<<1, 64, 0, s>>>(mx);
+  const unsigned blocks = static_cast<unsigned>(len / 256 < 1024 ? (len + 255) / 256 : 1024);
+  k_grid_max<<<blocks, 256, 0, s>>>(g, len, mx);
+  k_colorize<<<blocks_for(len, 256), 256, 0, s>>>(g, len, ldexpf(1.0f, -frac_bits), mx, frame,
+                                                   rgb);
   COSMO_CHECK_LAUNCH();
   return COSMO_OK;
 }

@@ -71,6 +71,7 @@ SIGNATURES = {
     "cosmo_render_density": ([_p, _p, _p, _p, _p, _i32, _i64, _int, _p], _int),
     "cosmo_kick_drift_dev": ([_p] * 8 + [_i64, _p], _int),
     "cosmo_render_density_dev": ([_p, _p, _p, _p, _p, _i32, _i64, _int, _p], _int),
+    "cosmo_colorize_frame": ([_p, _i32, _int, _p, _p, _p, _p], _int),
     "cosmo_direct_sum": ([_p, _p, _p, _p, _i64, _p, _i64, _f64, _f64, _p, _p], _int),
     "cosmo_validate_tree": ([_p, _p, _p, _i64, _p, _p], _int),
     # include/cosmo_mlp.h

@@ -14,7 +14,7 @@ import torch
 
 from ._hip import RENDER_FRAC_BITS, U64, check, kernel, load, ptr, stream_handle
 
-__all__ = ["render_density", "to_float", "RENDER_FRAC_BITS"]
+__all__ = ["render_density", "to_float", "colorize", "RENDER_FRAC_BITS"]
 
 
 def _mvp_host(mvp):
@@ -40,3 +40,23 @@ def render_density(pos_x, pos_y, pos_z, grid, mvp, res, frac_bits=RENDER_FRAC_BI
 def to_float(grid, frac_bits=RENDER_FRAC_BITS):
     """Fixed-point bins -> float64 density."""
     return grid.to(torch.float64) * (2.0 ** -frac_bits)
+
+
+def colorize(grid, frac_bits=RENDER_FRAC_BITS, frame=None, rgb=None, scratch=None):
+    """Colour mapping of the fixed-point density grid on the GPU
+    (src/utils/visualizer.py:115-171): fills frame (float32 [res, res, 3])
+    and/or rgb (uint8 [res, res, 3], the (frame * 255).astype(uint8) bytes
+    of engine.py:139).  With neither given, returns a new float frame."""
+    res = grid.shape[0]
+    if frame is None and rgb is None:
+        frame = torch.empty((res, res, 3), dtype=torch.float32, device=grid.device)
+    scratch = (torch.empty(1, dtype=torch.int64, device=grid.device) if scratch is None
+               else scratch)
+    check(load().cosmo_colorize_frame(
+        ptr(grid, U64, "grid", numel=res * res), res, int(frac_bits),
+        ptr(scratch, U64, "scratch", numel=1),
+        None if frame is None else ptr(frame, torch.float32, "frame", numel=3 * res * res),
+        None if rgb is None else ptr(rgb, torch.uint8, "rgb", numel=3 * res * res),
+        stream_handle()), "colorize_frame")
+    return frame if frame is not None else rgb
+

@@ -3,8 +3,8 @@
 The MVP matrix is built on the host in numpy float32 exactly as the
 reference builds it (look_at, perspective, proj @ view), and returned as a
 float32[4, 4] numpy array; render_density takes its 16 values as launch
-arguments.  colorize_frame (visualizer.py:115-171) is post-processing after
-the splat and is not part of this package.
+arguments.  colorize_frame (visualizer.py:115-171) runs as one fused gfx950
+pass over the fixed-point grid (render_kernels.colorize).
 """
 from __future__ import annotations
 
@@ -58,3 +58,11 @@ def get_mvp_matrix(step, box_center, box_size, config):
     view = look_at(eye, target, up)
     proj = perspective(config.FOV, 1.0, 0.1, radius * 3.0)
     return np.ascontiguousarray(proj @ view, dtype=np.float32)
+
+
+def colorize_frame(grid_d, frac_bits=None):
+    """src/utils/visualizer.py:115-171: the float32 frame [res, res, 3] in
+    [0, 1] of the density grid (the fixed-point grid of render_density)."""
+    from src.kernels.render_kernels import RENDER_FRAC_BITS, colorize
+    return colorize(grid_d, RENDER_FRAC_BITS if frac_bits is None else frac_bits)
+

@@ -303,6 +303,16 @@ int cosmo_render_density_dev(const float *pos_x, const float *pos_y,
                              const float *mvp_dev, int32_t res, int64_t n,
                              int frac_bits, void *stream);
 
+/* src/utils/visualizer.py:115-171 colorize_frame(grid_d), fused after the
+ * splat: img = log1p(120 x)/(max img + 1e-9), val = img^2.2, the four-band
+ * palette, x1.3, clip to [0, 1], with x = grid * 2^-frac_bits in FP32.
+ * frame (nullable): float[res][res][3]; rgb (nullable): the bytes
+ * (frame * 255).astype(uint8) that engine.py:139 stores.  max_scratch:
+ * device uint64[1] (the image maximum; overwritten). */
+int cosmo_colorize_frame(const uint64_t *grid, int32_t res, int frac_bits,
+                         uint64_t *max_scratch, float *frame, uint8_t *rgb,
+                         void *stream);
+
 /* FP64 direct sum (no reference counterpart; the error yardstick): for each
  * target t, acc[t][k] = sum_j G m_j (d^2+eps^2)^-3/2 d_k over all n sources
  * with the traversal's minimum image and d^2 > 1e-7 cut. */

@@ -239,3 +239,36 @@ def render_density(px, py, pz, mvp, res, frac_bits=RENDER_FRAC_BITS):
     lib().orc_render_density(_ptr(px), _ptr(py), _ptr(pz), _ptr(grid), _ptr(mvp), int(res),
                              px.shape[0], int(frac_bits))
     return grid
+
+
+def colorize_frame(grid):
+    """src/utils/visualizer.py:115-171 in numpy float32, as the reference's
+    CuPy float32 arrays compute it (Python-float operands take the array's
+    float32 type).  grid: float32 density [res, res].  Returns the float32
+    frame [res, res, 3] and the bytes (frame * 255).astype(uint8) that
+    engine.py:139 stores."""
+    g = np.asarray(grid, np.float32)
+    img = np.log1p(g * np.float32(120.0))
+    img = img / (img.max() + np.float32(1e-9))
+    val = np.power(img, np.float32(2.2))
+    T1, T2, T3 = np.float32(0.6), np.float32(0.72), np.float32(0.82)
+    r = np.zeros_like(val)
+    gg = np.zeros_like(val)
+    b = np.zeros_like(val)
+    m1 = val <= T1
+    b[m1] = (val[m1] / T1) * np.float32(0.8)
+    m2 = (val > T1) & (val <= T2)
+    t = (val[m2] - T1) / np.float32(0.72 - 0.6)
+    b[m2] = (np.float32(1.0) - t) * np.float32(0.8)
+    gg[m2] = t * np.float32(0.9)
+    m3 = (val > T2) & (val <= T3)
+    t = (val[m3] - T2) / np.float32(0.82 - 0.72)
+    gg[m3] = np.float32(0.9) + t * np.float32(0.1)
+    r[m3] = t
+    m4 = val > T3
+    t = (val[m4] - T3) / np.float32(1.0 - 0.82)
+    r[m4] = np.float32(1.0)
+    gg[m4] = np.float32(1.0) - t
+    frame = np.clip(np.stack([r, gg, b], axis=-1) * np.float32(1.3), 0, 1).astype(np.float32)
+    return frame, (frame * np.float32(255)).astype(np.uint8)
+

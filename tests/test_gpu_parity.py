@@ -504,6 +504,36 @@ def _run_engine(preset, n_grid, steps, seed=42, **kw):
     return eng, c, pos, vel
 
 
+@pytest.mark.parametrize("res", [64, 1024])
+def test_colorize_frame(orc, torch_mod, ics32, res):
+    """Fused colour mapping against the numpy float32 restatement of
+    visualizer.py:115-171 on the splat of a real frame: floats within a few
+    ulps (log1p and pow differ by about an ulp between libraries), bytes
+    equal except where a value sits at a truncation boundary."""
+    torch = torch_mod
+    from src.kernels.render_kernels import render_density, to_float
+    from src.utils import config as cfg
+    from src.utils.visualizer import colorize_frame, get_mvp_matrix
+    c = cfg.preset("small_32", RES=res)
+    grid = torch.zeros((res, res), dtype=torch.int64, device="cuda")
+    mvp = get_mvp_matrix(3, (0.5, 0.5, 0.5), 1.0, c)
+    render_density(*[dev(p, torch) for p in ics32], grid, mvp, res)
+    frame = colorize_frame(grid)
+    rgb = torch.zeros((res, res, 3), dtype=torch.uint8, device="cuda")
+    from src.kernels.render_kernels import colorize
+    colorize(grid, rgb=rgb)
+    ref_f, ref_b = orc.colorize_frame(to_float(grid).float().cpu().numpy())
+    f = host(frame)
+    assert np.abs(f - ref_f).max() <= 2e-6
+    b = host(rgb).astype(np.int32)
+    diff = np.abs(b - ref_b.astype(np.int32))
+    assert diff.max() <= 1 and np.mean(diff > 0) < 1e-3
+    assert np.array_equal(b, (f * np.float32(255)).astype(np.uint8))
+    # an empty grid is black (the reference's + 1e-9 guard)
+    grid.zero_()
+    assert host(colorize_frame(grid)).max() == 0.0
+
+
 def test_engine_steps_match_oracle(orc, torch_mod):
     """Five steps of small_32 on the GPU against the same five steps done
     stage by stage with the oracle (bit-exact sort/tree/moments/update for

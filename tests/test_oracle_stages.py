@@ -439,3 +439,21 @@ def test_render_centre_pixel_and_totals(orc):
     inside &= (sx >= 0) & (sx < res) & (sy >= 0) & (sy < res)
     expect = np.sum(np.rint(2.5 / clip[inside, 3] * 2 ** 20))
     assert abs(float(g.sum()) - expect) / expect < 1e-5
+
+
+def test_colorize_palette_known_answers(orc):
+    """visualizer.py:115-171 on chosen pixels: the brightest pixel maps to
+    val = 1 (red 1.3 -> 1, green 0 after the last band), zero stays black,
+    and a value at band edge T1 gives pure (clipped) blue."""
+    grid = np.zeros((4, 4), np.float32)
+    grid[0, 0] = 10.0
+    top = np.log1p(np.float32(1200.0))
+    # a pixel with val = 0.6 exactly at the first band edge: img = 0.6^(1/2.2)
+    img = np.float32(0.6) ** np.float32(1 / 2.2)
+    grid[1, 1] = np.expm1(img * top) / 120.0
+    frame, rgb = orc.colorize_frame(grid)
+    assert np.allclose(frame[0, 0], [1.0, 0.0, 0.0], atol=1e-6)
+    assert np.all(frame[3, 3] == 0)
+    assert abs(frame[1, 1, 2] - min(1.0, 0.8 * 1.3)) < 1e-3 and frame[1, 1, 0] == 0
+    assert rgb.dtype == np.uint8 and rgb[0, 0, 0] == 255
+
