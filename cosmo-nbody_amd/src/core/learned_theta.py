@@ -305,13 +305,23 @@ def _time_traversal(engine, theta, theta_group, reps=5, group_order=None):
     return e0.elapsed_time(e1) / reps
 
 
-def _time_steps(c, state, selector, steps=3):
-    """Average HIP-event time (ms) of full engine steps (sort, tree, per-group
-    theta selection when a selector is given, traversal, update) starting
-    from `state`; one untimed warm-up step."""
+def _time_steps(c, state, selector, steps=3, graph=False):
+    """Average time (ms) of full engine steps (sort, tree, per-group theta
+    selection when a selector is given, traversal, update) starting from
+    `state`; one untimed warm-up step.  graph: the steps (with the density
+    splat) are replayed from captured HIP graphs and timed by the host
+    clock between device synchronisations."""
     eng = _engine_at(c, *state)
     eng.theta_selector = selector
     eng.step()
+    if graph:
+        eng.capture_graphs(steps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            eng.step_graph()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(steps):
@@ -392,8 +402,19 @@ def train(c, states, n_groups, thetas, fit_steps, seed=0, log=print, n_models=N_
     return model, np.mean(np.stack(costs), 0), info
 
 
+def measure_costs(c, state, thetas, log=print):
+    """ms per group of a whole traversal of `state` at each fixed theta."""
+    eng = _engine_at(c, *state)
+    G = (eng.state.n + 63) // 64
+    out = []
+    for t in thetas:
+        out.append(_time_traversal(eng, t, None, reps=2) / G)
+        log(f"cost at theta {t}: {out[-1] * G:.1f} ms")
+    return np.asarray(out)
+
+
 def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0,
-             target_theta=None, lambdas=None, log=print):
+             target_theta=None, lambdas=None, log=print, graph=False):
     """Fixed theta against learned per-group theta on the held-out state
     `val` at equal relative RMS error (FP64 direct sum on sampled groups):
     the fixed theta meeting the target is found by bisection; lambda is swept
@@ -478,14 +499,15 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
     e1.record()
     torch.cuda.synchronize()
     t_select = e0.elapsed_time(e1) / 5
-    step_fixed = _time_steps(c.replace(THETA=theta_star), val, None)
-    step_learned = _time_steps(c, val, lt)
+    step_fixed = _time_steps(c.replace(THETA=theta_star), val, None, graph=graph)
+    step_learned = _time_steps(c, val, lt, graph=graph)
     hist_t = np.histogram(tg.cpu().numpy(), bins=np.concatenate([th_np - 1e-6, [np.inf]]))[0]
     res = {"reachable": True, "preset": c.NAME, "mac": c.MAC, "target": target,
            "bucket": int(getattr(c, "BUCKET", 0)),
            "compensated": bool(getattr(c, "COMPENSATED", False)),
            "straddle_mass": float(getattr(c, "STRADDLE_MASS", 0.0)),
            "n": eng.state.n, "val_state": val, "val_groups": int(len(vgroups)),
+           "hip_graph_steps": bool(graph),
            "theta_fixed": theta_star, "err_fixed": e_fixed,
            "inter_fixed": i_fixed, "lambda": lt.lam, "err_learned": e_learned,
            "inter_learned": i_learned, "traversal_ms_fixed": t_fixed,
@@ -545,6 +567,13 @@ def main():
     ap.add_argument("--bucket", type=int, default=None)
     ap.add_argument("--compensated", type=int, default=None)
     ap.add_argument("--straddle", type=float, default=None)
+    ap.add_argument("--eval-preset", default="",
+                    help="evaluate on this preset (its own theta costs, same network); "
+                         "e.g. a larger N than the training preset")
+    ap.add_argument("--eval-thetas", default="",
+                    help="comma-separated candidate thetas for --eval-preset")
+    ap.add_argument("--graph", action="store_true",
+                    help="time full steps replayed from HIP graphs (with the density splat)")
     ap.add_argument("--save", default="")
     args = ap.parse_args()
     over = {}
@@ -560,6 +589,22 @@ def main():
     if args.train_seeds:
         train_states = [(int(x), 0) for x in args.train_seeds.split(",") if x]
     vals = [val] + [(int(x), 0) for x in args.val_seeds.split(",") if x]
+    if args.eval_preset:
+        log = lambda m: print(m, flush=True)  # noqa: E731
+        model, _, info = train(c, train_states, args.groups, DEFAULT_THETAS, args.fit_steps,
+                               log=log)
+        ce = cfg.preset(args.eval_preset, MAC=args.mac, **over)
+        th = (tuple(float(x) for x in args.eval_thetas.split(",")) if args.eval_thetas
+              else DEFAULT_THETAS)
+        cost = measure_costs(ce, vals[0], th, log)
+        for v in vals:
+            r = evaluate(ce, model, cost, th, v, args.target, args.val_groups, log=log,
+                         graph=args.graph)
+            res = r[0] if isinstance(r, tuple) else r
+            res.update(info)
+            res["train_preset"] = args.preset
+            print(json.dumps(res, default=float), flush=True)
+        return
     out = experiment(args.preset, args.target, args.mac, fit_steps=args.fit_steps,
                      n_groups=args.groups, n_val_groups=args.val_groups,
                      target_theta=args.target_theta, log=lambda m: print(m, flush=True),

@@ -16,6 +16,8 @@ a dataclass with the same attribute names plus the knobs this build adds;
   halos_128           N = 128^3 heterogeneous test state (half in 64 Plummer
                       spheres, half uniform), AABB acceptance test with
                       32-leaf buckets, compensated sums, straddle mass 1e-4
+  halos_256           the same at N = 256^3 with the 2048^2 splat (the large
+                      clustered run with the learned theta)
 
 Time stepping follows the reference schedule: da = (A_END - A_START) /
 N_STEPS with N_STEPS = 2500 (src/core/engine.py:46); RUN_STEPS is how many of
@@ -145,6 +147,9 @@ PRESETS = {
                                 NAME="large_256"),
     "halos_128": lambda: _plain(128, RUN_STEPS=20, MAC="aabb", IC_KIND="halos", BUCKET=32,
                                 COMPENSATED=True, STRADDLE_MASS=1e-4, NAME="halos_128"),
+    "halos_256": lambda: _plain(256, RUN_STEPS=50, RES=2048, MAC="aabb", IC_KIND="halos",
+                                BUCKET=32, COMPENSATED=True, STRADDLE_MASS=1e-4,
+                                LEARNED_THETA=True, NAME="halos_256"),
 }
 
 
