@@ -233,31 +233,34 @@ __global__ __launch_bounds__(256) void k_traverse(
       in[h] = depth[h] > top;
       depth[h] = min(depth[h], top);
     }
-    // Leaf and internal records start with (x, y, z, m).  The loads are
-    // wave-uniform (scalar cache).  For a leaf the other fields copy A, and
-    // for the reference test B.x (sx^2) = -1, so that the test accepts it.
-    // One wave-uniform byte offset for both kinds (a selected offset, not a
-    // pointer joined from two branches) read through the constant address
-    // space (the records are not written while a traversal runs): every
-    // field is then a scalar load, none a vector load with a wait.  Measured
-    // on one MI355X: 6.73 ms at 128^3 against 7.26 ms for the compiler's
-    // mix of scalar loads and a vector load of the shared x coordinate.
-    const bool leaf = node < n_tree;
-    const int64_t ofs = leaf ? int64_t(node) * int64_t(sizeof(cosmo_leaf))
-                             : int64_t(n_tree) * int64_t(sizeof(cosmo_leaf)) +
-                                   int64_t(node - n_tree) * int64_t(sizeof(Inode));
-    // The whole internal-record size is loaded at once for either kind (one
-    // scalar round trip per pop; the record buffer has tail room for the
-    // last leaf), then a leaf keeps only its 16 bytes.
-    const ConstF *rec = (const ConstF *)(reinterpret_cast<const char *>(tree) + ofs);
-    float4 A = ld4(rec), B = ld4(rec + 4), C = A, D = A;
-    if constexpr (MAC == COSMO_MAC_AABB) {
+    // Records are wave-uniform loads read through the constant address space
+    // (the records are not written while a traversal runs), so every field
+    // is a scalar load.  REF: one 32-byte record per node id, a leaf's with
+    // sx^2 = -1 so that the test accepts it; no per-kind offset or field
+    // selection in the loop.  AABB: leaf and internal records start with
+    // (x, y, z, m); one wave-uniform byte offset is selected for either kind
+    // and the whole internal-record size is loaded at once (the buffer has
+    // tail room for the last leaf), then a leaf keeps only its 16 bytes.
+    bool leaf;
+    float4 A, B, C, D;
+    if constexpr (MAC == COSMO_MAC_REF) {
+      const ConstF *rec = (const ConstF *)(reinterpret_cast<const char *>(tree) +
+                                           int64_t(node) * int64_t(sizeof(Inode)));
+      A = ld4(rec);
+      B = ld4(rec + 4);
+      C = D = A;
+      leaf = __float_as_int(B.x) < 0;  // only read on the rare FP64 recheck
+    } else {
+      leaf = node < n_tree;
+      const int64_t ofs = leaf ? int64_t(node) * int64_t(sizeof(cosmo_leaf))
+                               : int64_t(n_tree) * int64_t(sizeof(cosmo_leaf)) +
+                                     int64_t(node - n_tree) * int64_t(sizeof(Inode));
+      const ConstF *rec = (const ConstF *)(reinterpret_cast<const char *>(tree) + ofs);
+      A = ld4(rec);
+      B = ld4(rec + 4);
       C = ld4(rec + 8);
       D = ld4(rec + 12);
-    }
-    if (leaf) {
-      B = C = D = A;
-      if constexpr (MAC == COSMO_MAC_REF) B.x = -1.0f;
+      if (leaf) B = C = D = A;
     }
     // massless nodes are skipped (integer test on the uniform bits: m <= 0)
     if (__float_as_int(A.w) <= 0) continue;

@@ -5,16 +5,18 @@
 // delta function including its equal-code tie-break (:55-56), so `children`
 // is bit-identical to the reference algorithm's for any input.  The upward
 // pass climbs from each leaf with an arrival counter per node (:164-191):
-// the second arrival forms the parent as left + right in FP64.  The eight
-// XCDs have separate L2 caches, so a child's moments must be visible to the
-// sibling's thread on another XCD.  Acquire/release at agent scope would do
-// it with a write-back and an invalidation of the whole L2 per climb step
-// (8 ms at 128^3); instead every moment store and load of the climb is a
-// relaxed agent-scope atomic (coherent at the device: written through and
-// read past the local L2), and each thread waits for its stores to complete
-// (s_waitcnt vmcnt(0)) before it bumps the arrival counter.  The second
-// arrival reads the sibling only after its counter update returned, i.e.
-// after the first arrival's stores completed.
+// the second arrival forms the parent as left + right in FP64.  Nodes whose
+// leaves all lie in one block's chunk are climbed in LDS; only nodes that
+// span chunks use device memory.  The eight XCDs have separate L2 caches, so
+// a child's moments must be visible to the sibling's thread on another XCD.
+// Acquire/release at agent scope would do it with a write-back and an
+// invalidation of the whole L2 per climb step (8 ms at 128^3); instead every
+// moment store and load of the device-wide climb is a relaxed agent-scope
+// atomic (coherent at the device: written through and read past the local
+// L2), and each thread waits for its stores to complete (s_waitcnt vmcnt(0))
+// before it bumps the arrival counter.  The second arrival reads the sibling
+// only after its counter update returned, i.e. after the first arrival's
+// stores completed.
 #include "common.h"
 
 namespace cosmo {
@@ -125,27 +127,67 @@ __device__ __forceinline__ void store_moments(double *n_mass, double *n_com, flo
 __device__ __forceinline__ float min_ref(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float max_ref(float a, float b) { return (b > a) ? b : a; }
 
-// tree_kernels.py:152-191 with FP64 mass and mass-weighted position sums.
-// The climbing thread keeps its own node's values in registers and loads
-// only the sibling's.  IEEE addition is commutative, so "mine + sibling"
-// equals the reference's "left + right" bit for bit; min/max keep the
-// left/right order explicitly.
-__global__ __launch_bounds__(256) void k_multipoles(
-    const float *__restrict__ px, const float *__restrict__ py,
-    const float *__restrict__ pz, const float *__restrict__ mass,
-    const int32_t *__restrict__ children, const int32_t *__restrict__ parents,
-    double *n_mass, double *n_com, float *n_min, float *n_max, int32_t *counts,
-    int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float x = px[i], y = py[i], z = pz[i];
-  double m = static_cast<double>(mass[i]);
-  double c0 = static_cast<double>(x) * m, c1 = static_cast<double>(y) * m,
-         c2 = static_cast<double>(z) * m;
-  float lo0 = x, lo1 = y, lo2 = z, hi0 = x, hi1 = y, hi2 = z;
-  store_moments(n_mass, n_com, n_min, n_max, i, m, c0, c1, c2, lo0, lo1, lo2, hi0, hi1, hi2);
-  int curr = i;
-  for (;;) {
+// Moments of one node as its climbing thread holds them.
+struct Moments {
+  double m, c0, c1, c2;
+  float lo0, lo1, lo2, hi0, hi1, hi2;
+};
+
+// tree_kernels.py:156-162: leaf mass, mass-weighted position (FP64) and AABB.
+__device__ __forceinline__ Moments leaf_moments(const float *__restrict__ px,
+                                                const float *__restrict__ py,
+                                                const float *__restrict__ pz,
+                                                const float *__restrict__ mass, int k) {
+  const float x = px[k], y = py[k], z = pz[k];
+  const double m = static_cast<double>(mass[k]);
+  return {m,  static_cast<double>(x) * m, static_cast<double>(y) * m, static_cast<double>(z) * m,
+          x,  y, z, x, y, z};
+}
+
+// tree_kernels.py:172-187: parent = left + right.  IEEE addition is
+// commutative, so "mine + sibling" equals "left + right" bit for bit; min/max
+// keep the left/right order explicitly.
+__device__ __forceinline__ void combine(Moments &v, const Moments &s, bool mine_left) {
+  v.m = v.m + s.m;
+  v.c0 = v.c0 + s.c0;
+  v.c1 = v.c1 + s.c1;
+  v.c2 = v.c2 + s.c2;
+  if (mine_left) {
+    v.lo0 = min_ref(v.lo0, s.lo0); v.lo1 = min_ref(v.lo1, s.lo1); v.lo2 = min_ref(v.lo2, s.lo2);
+    v.hi0 = max_ref(v.hi0, s.hi0); v.hi1 = max_ref(v.hi1, s.hi1); v.hi2 = max_ref(v.hi2, s.hi2);
+  } else {
+    v.lo0 = min_ref(s.lo0, v.lo0); v.lo1 = min_ref(s.lo1, v.lo1); v.lo2 = min_ref(s.lo2, v.lo2);
+    v.hi0 = max_ref(s.hi0, v.hi0); v.hi1 = max_ref(s.hi1, v.hi1); v.hi2 = max_ref(s.hi2, v.hi2);
+  }
+}
+
+__device__ __forceinline__ void store_plain(double *n_mass, double *n_com, float *n_min,
+                                            float *n_max, int64_t k, const Moments &v) {
+  n_mass[k] = v.m;
+  n_com[3 * k + 0] = v.c0;
+  n_com[3 * k + 1] = v.c1;
+  n_com[3 * k + 2] = v.c2;
+  n_min[3 * k + 0] = v.lo0;
+  n_min[3 * k + 1] = v.lo1;
+  n_min[3 * k + 2] = v.lo2;
+  n_max[3 * k + 0] = v.hi0;
+  n_max[3 * k + 1] = v.hi1;
+  n_max[3 * k + 2] = v.hi2;
+}
+
+// tree_kernels.py:164-191 across the device: the climb of a node whose
+// moments are complete in registers, through nodes that no single block
+// owns.  Every moment store and load here is device-coherent (see the file
+// header) and each thread's stores complete before it bumps the counter.
+__device__ void climb_device(const int32_t *__restrict__ children,
+                             const int32_t *__restrict__ parents, double *n_mass,
+                             double *n_com, float *n_min, float *n_max, int32_t *counts,
+                             Moments v, int curr, int n) {
+  store_moments(n_mass, n_com, n_min, n_max, curr, v.m, v.c0, v.c1, v.c2, v.lo0, v.lo1, v.lo2,
+                v.hi0, v.hi1, v.hi2);
+  // A climb is at most n - 1 levels; the bound only matters for a malformed
+  // tree (the validator reports those), where it keeps every thread finite.
+  for (int level = 0; level < n; ++level) {
     const int p = parents[curr];
     if (p == -1) break;
     wait_stores();
@@ -155,27 +197,99 @@ __global__ __launch_bounds__(256) void k_multipoles(
     const int2 ch = reinterpret_cast<const int2 *>(children)[p];
     const bool mine_left = (ch.x == curr);
     const int64_t sib = mine_left ? ch.y : ch.x;
-    const double sm = ld_dev(&n_mass[sib]);
-    const double s0 = ld_dev(&n_com[3 * sib + 0]), s1 = ld_dev(&n_com[3 * sib + 1]),
-                 s2 = ld_dev(&n_com[3 * sib + 2]);
-    const float sl0 = ld_dev(&n_min[3 * sib + 0]), sl1 = ld_dev(&n_min[3 * sib + 1]),
-                sl2 = ld_dev(&n_min[3 * sib + 2]);
-    const float sh0 = ld_dev(&n_max[3 * sib + 0]), sh1 = ld_dev(&n_max[3 * sib + 1]),
-                sh2 = ld_dev(&n_max[3 * sib + 2]);
-    m = m + sm;
-    c0 = c0 + s0;
-    c1 = c1 + s1;
-    c2 = c2 + s2;
-    if (mine_left) {
-      lo0 = min_ref(lo0, sl0); lo1 = min_ref(lo1, sl1); lo2 = min_ref(lo2, sl2);
-      hi0 = max_ref(hi0, sh0); hi1 = max_ref(hi1, sh1); hi2 = max_ref(hi2, sh2);
-    } else {
-      lo0 = min_ref(sl0, lo0); lo1 = min_ref(sl1, lo1); lo2 = min_ref(sl2, lo2);
-      hi0 = max_ref(sh0, hi0); hi1 = max_ref(sh1, hi1); hi2 = max_ref(sh2, hi2);
-    }
-    store_moments(n_mass, n_com, n_min, n_max, p, m, c0, c1, c2, lo0, lo1, lo2, hi0, hi1, hi2);
+    Moments s;
+    s.m = ld_dev(&n_mass[sib]);
+    s.c0 = ld_dev(&n_com[3 * sib + 0]);
+    s.c1 = ld_dev(&n_com[3 * sib + 1]);
+    s.c2 = ld_dev(&n_com[3 * sib + 2]);
+    s.lo0 = ld_dev(&n_min[3 * sib + 0]);
+    s.lo1 = ld_dev(&n_min[3 * sib + 1]);
+    s.lo2 = ld_dev(&n_min[3 * sib + 2]);
+    s.hi0 = ld_dev(&n_max[3 * sib + 0]);
+    s.hi1 = ld_dev(&n_max[3 * sib + 1]);
+    s.hi2 = ld_dev(&n_max[3 * sib + 2]);
+    combine(v, s, mine_left);
+    store_moments(n_mass, n_com, n_min, n_max, p, v.m, v.c0, v.c1, v.c2, v.lo0, v.lo1, v.lo2,
+                  v.hi0, v.hi1, v.hi2);
     curr = p;
   }
+}
+
+// tree_kernels.py:131-191, blocked.  Block b owns the leaf chunk
+// [b0, b0 + K) with b0 = b K.  An internal node lies inside the chunk
+// exactly when both of its children do, and its index (one end of its leaf
+// range) is then in [b0, b0 + K), so those nodes get an LDS arrival counter
+// and an LDS moment slot each.  Phase 1 climbs from every leaf through the
+// chunk's own nodes with LDS atomics.  A climb stops where the parent's
+// index or the sibling lies outside the chunk; a first arrival whose
+// sibling never came by the barrier (the sibling's range leaves the chunk)
+// stops there too.  Those threads hold a complete node whose parent spans
+// several chunks, and phase 2 continues them with the device-wide climb on
+// `counts`, which sees only such nodes.  The chunk's own internal nodes are
+// written out after the barrier with coalesced stores, and their counters
+// are set to 2, the value the per-node climb leaves on every internal node.
+// Sums are formed in the same order as the per-node climb (bit-identical).
+constexpr int kUpChunk = 1024;
+
+__global__ __launch_bounds__(kUpChunk) void k_multipoles(
+    const float *__restrict__ px, const float *__restrict__ py,
+    const float *__restrict__ pz, const float *__restrict__ mass,
+    const int32_t *__restrict__ children, const int32_t *__restrict__ parents,
+    double *n_mass, double *n_com, float *n_min, float *n_max, int32_t *counts,
+    int n) {
+  __shared__ double s_m[kUpChunk], s_c0[kUpChunk], s_c1[kUpChunk], s_c2[kUpChunk];
+  __shared__ float s_lo0[kUpChunk], s_lo1[kUpChunk], s_lo2[kUpChunk];
+  __shared__ float s_hi0[kUpChunk], s_hi1[kUpChunk], s_hi2[kUpChunk];
+  __shared__ int s_cnt[kUpChunk];
+  const int t = threadIdx.x;
+  const int b0 = blockIdx.x * kUpChunk;
+  const int leaf = b0 + t;
+  s_cnt[t] = 0;
+  __syncthreads();
+  // 0: climb ended (root, or completed by the sibling's thread);
+  // 1: first arrival at chunk node `wait`; 2: continue across chunks.
+  int state = 0, wait = 0, curr = leaf;
+  Moments v{};
+  if (leaf < n) {
+    v = leaf_moments(px, py, pz, mass, leaf);
+    store_plain(n_mass, n_com, n_min, n_max, leaf, v);
+    for (int level = 0; level < kUpChunk; ++level) {
+      const int p = parents[curr];
+      if (p == -1) break;
+      const int slot = p - n - b0;
+      if (slot < 0 || slot >= kUpChunk) { state = 2; break; }
+      const int2 ch = reinterpret_cast<const int2 *>(children)[p];
+      const bool mine_left = (ch.x == curr);
+      const int sib = mine_left ? ch.y : ch.x;
+      const int sslot = (sib >= n ? sib - n : sib) - b0;
+      if (sslot < 0 || sslot >= kUpChunk) { state = 2; break; }
+      const int old = __hip_atomic_fetch_add(&s_cnt[slot], 1, __ATOMIC_ACQ_REL,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0) { state = 1; wait = slot; break; }
+      Moments s;
+      if (sib >= n) {
+        s = {s_m[sslot],   s_c0[sslot],  s_c1[sslot],  s_c2[sslot],  s_lo0[sslot],
+             s_lo1[sslot], s_lo2[sslot], s_hi0[sslot], s_hi1[sslot], s_hi2[sslot]};
+      } else {
+        s = leaf_moments(px, py, pz, mass, sib);
+      }
+      combine(v, s, mine_left);
+      s_m[slot] = v.m; s_c0[slot] = v.c0; s_c1[slot] = v.c1; s_c2[slot] = v.c2;
+      s_lo0[slot] = v.lo0; s_lo1[slot] = v.lo1; s_lo2[slot] = v.lo2;
+      s_hi0[slot] = v.hi0; s_hi1[slot] = v.hi1; s_hi2[slot] = v.hi2;
+      curr = p;
+    }
+  }
+  __syncthreads();
+  if (state == 1 && s_cnt[wait] == 1) state = 2;
+  const int i = b0 + t;
+  if (i < n - 1 && s_cnt[t] == 2) {
+    const Moments w{s_m[t],   s_c0[t],  s_c1[t],  s_c2[t],  s_lo0[t],
+                    s_lo1[t], s_lo2[t], s_hi0[t], s_hi1[t], s_hi2[t]};
+    store_plain(n_mass, n_com, n_min, n_max, int64_t(n) + i, w);
+    counts[n + i] = 2;
+  }
+  if (state == 2) climb_device(children, parents, n_mass, n_com, n_min, n_max, counts, v, curr, n);
 }
 
 // tree_kernels.py:193-214.
@@ -239,19 +353,26 @@ __device__ __forceinline__ void write_record(void *__restrict__ out, int64_t n, 
                                              int first, int count) {
   const float fx = static_cast<float>(c0), fy = static_cast<float>(c1), fz = static_cast<float>(c2);
   const float fm = static_cast<float>(nm);
+  if (MAC == COSMO_MAC_REF) {
+    // One 32-byte record per node id.  physics_kernels.py:129-130 for an
+    // internal node; a leaf has sx2 = -1 (always accepted) and no children.
+    float4 *dst = reinterpret_cast<float4 *>(out) + 2 * k;
+    dst[0] = make_float4(fx, fy, fz, fm);
+    if (k < n) {
+      dst[1] = make_float4(-1.0f, 0.0f, __int_as_float(-1), __int_as_float(-1));
+    } else {
+      float sx = hi0 - lo0;
+      sx = (sx > boxf * 0.5f) ? boxf : sx;
+      dst[1] = make_float4(sx * sx, sx, __int_as_float(ch.x), __int_as_float(ch.y));
+    }
+    return;
+  }
   if (k < n) {
     reinterpret_cast<float4 *>(out)[k] = make_float4(fx, fy, fz, fm);
     return;
   }
   char *ibase = reinterpret_cast<char *>(out) + n * sizeof(cosmo_leaf);
-  if (MAC == COSMO_MAC_REF) {
-    // physics_kernels.py:129-130
-    float sx = hi0 - lo0;
-    sx = (sx > boxf * 0.5f) ? boxf : sx;
-    float4 *dst = reinterpret_cast<float4 *>(ibase + (k - n) * sizeof(cosmo_inode_ref));
-    dst[0] = make_float4(fx, fy, fz, fm);
-    dst[1] = make_float4(sx * sx, sx, __int_as_float(ch.x), __int_as_float(ch.y));
-  } else {
+  {
     const float e0 = hi0 - lo0, e1 = hi1 - lo1, e2 = hi2 - lo2;
     float sz = 0.0f;
     sz = e0 > sz ? e0 : sz;
@@ -434,7 +555,7 @@ int cosmo_compute_multipoles(const float *px, const float *py, const float *pz,
   if (!px || !py || !pz || !mass || !children || !parents || !n_mass || !n_com || !n_min ||
       !n_max || !counts)
     return COSMO_E_INVALID;
-  k_multipoles<<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(
+  k_multipoles<<<blocks_for(n, kUpChunk), kUpChunk, 0, as_stream(stream)>>>(
       px, py, pz, mass, children, parents, n_mass, n_com, n_min, n_max, counts,
       static_cast<int>(n));
   COSMO_CHECK_LAUNCH();

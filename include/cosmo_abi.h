@@ -61,13 +61,15 @@ extern "C" {
 #define COSMO_RENDER_FRAC_BITS 20
 
 /* Packed FP32 tree records read by the traversal.  One buffer of
- * cosmo_tree_bytes(n, mac) bytes holds
- *   n leaf records (cosmo_leaf, 16 B; node ids 0..n-1): CoM and mass;
- *   then n-1 internal records for node ids n..2n-2, of the test's type:
- *     REF  cosmo_inode_ref  (32 B): sx = x-extent, replaced by box when
- *          > box/2 (physics_kernels.py:129-130), and sx2 = sx*sx rounded
- *          to FP32;
- *     AABB cosmo_inode_aabb (64 B): s = largest extent of the node's box,
+ * cosmo_tree_bytes(n, mac) bytes holds, for the test `mac`:
+ *   REF  2n-1 records cosmo_inode_ref (32 B), record k for node id k:
+ *          internal nodes (ids n..2n-2): sx = x-extent, replaced by box
+ *          when > box/2 (physics_kernels.py:129-130), and sx2 = sx*sx
+ *          rounded to FP32; leaves (ids 0..n-1): CoM and mass with
+ *          sx2 = -1, sx = 0 and c1 = c2 = -1;
+ *   AABB n leaf records (cosmo_leaf, 16 B; node ids 0..n-1): CoM and mass,
+ *        then n-1 internal records for node ids n..2n-2:
+ *        cosmo_inode_aabb (64 B): s = largest extent of the node's box,
  *          dl = |CoM - box centre|, lo[k] = CoM_k - min_k, hi[k] = max_k - CoM_k,
  *          and, for a node of at most `bucket` leaves packed by
  *          cosmo_normalize_pack_buckets, its leaf range [first,
@@ -95,9 +97,9 @@ typedef struct cosmo_inode_aabb {
 } cosmo_inode_aabb;
 
 /* Size in bytes of the packed records of an n-leaf tree for test `mac`:
- * the n leaf and n-1 internal records plus one internal record of tail
- * room (the traversal loads a whole internal-record size at every node,
- * leaves included, and keeps only the leaf's 16 bytes). */
+ * REF: the 2n-1 records; AABB: the n leaf and n-1 internal records plus one
+ * internal record of tail room (the traversal loads a whole internal-record
+ * size at every node, leaves included, and keeps only the leaf's 16 bytes). */
 int64_t cosmo_tree_bytes(int64_t n, int mac);
 
 typedef struct cosmo_ctx cosmo_ctx;

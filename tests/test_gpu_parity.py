@@ -189,6 +189,39 @@ def test_multipoles_bit_exact(orc, torch_mod, ics32):
     assert np.array_equal(host(nc), orc.normalize_com(r_nm, r_nc))
 
 
+@pytest.mark.parametrize("n,power", [(1, 1), (2, 1), (1023, 1), (1025, 3), (20001, 3),
+                                     (70000, 2)])
+def test_multipoles_bit_exact_clustered(orc, torch_mod, n, power):
+    """Upward pass on clustered points (u^power per axis) with sizes that do
+    not fill the 1024-leaf chunks of the kernel: deep subtrees cross chunk
+    boundaries, so both the in-chunk climb and the device-wide climb run.
+    Bit-exact moments and counters equal to 2 on every internal node."""
+    torch = torch_mod
+    rng = np.random.default_rng(n)
+    pos, codes = sorted_case(orc, list((rng.random((3, n)) ** power).astype(np.float32)))
+    ch, par = orc.build_tree(codes)
+    if orc.validate_tree(ch, par, n) != 0:
+        pytest.skip("equal Morton codes give a malformed tree for this draw")
+    mass = rng.uniform(0.5, 1.5, n).astype(np.float32) / n
+    from src.kernels.tree_kernels import compute_multipoles
+    M = 2 * n - 1
+    nm = torch.zeros(M, dtype=torch.float64, device="cuda")
+    nc = torch.zeros((M, 3), dtype=torch.float64, device="cuda")
+    lo = torch.zeros((M, 3), dtype=torch.float32, device="cuda")
+    hi = torch.zeros((M, 3), dtype=torch.float32, device="cuda")
+    cnt = torch.zeros(M, dtype=torch.int32, device="cuda")
+    compute_multipoles[(n + 255) // 256, 256](*[dev(p, torch) for p in pos], dev(mass, torch),
+                                              dev(ch, torch), dev(par, torch), nm, nc, lo, hi,
+                                              cnt)
+    r_nm, r_nc, r_lo, r_hi = orc.compute_multipoles(*pos, mass, ch, par)
+    assert np.array_equal(host(nm), r_nm)
+    assert np.array_equal(host(nc), r_nc)
+    assert np.array_equal(host(lo), r_lo)
+    assert np.array_equal(host(hi), r_hi)
+    c = host(cnt)
+    assert (c[:n] == 0).all() and (c[n:] == 2).all()
+
+
 def test_top_tree_cache(orc, torch_mod, ics16):
     torch = torch_mod
     from src.kernels.tree_kernels import build_top_tree_cache
@@ -507,9 +540,12 @@ def _run_engine(preset, n_grid, steps, seed=42, **kw):
 @pytest.mark.parametrize("res", [64, 1024])
 def test_colorize_frame(orc, torch_mod, ics32, res):
     """Fused colour mapping against the numpy float32 restatement of
-    visualizer.py:115-171 on the splat of a real frame: floats within a few
-    ulps (log1p and pow differ by about an ulp between libraries), bytes
-    equal except where a value sits at a truncation boundary."""
+    visualizer.py:115-171 on the splat of a real frame.  log1p, the division
+    by the maximum and pow each differ by an ulp or two between libraries, so
+    val = img^2.2 carries up to about 8 ulps (<= 1e-6 absolute, val <= 1);
+    the steepest palette segment multiplies that by 1.3 / 0.1 = 13, hence
+    1.5e-5 on the floats.  Bytes are equal except where a value sits at a
+    truncation boundary."""
     torch = torch_mod
     from src.kernels.render_kernels import render_density, to_float
     from src.utils import config as cfg
@@ -524,7 +560,7 @@ def test_colorize_frame(orc, torch_mod, ics32, res):
     colorize(grid, rgb=rgb)
     ref_f, ref_b = orc.colorize_frame(to_float(grid).float().cpu().numpy())
     f = host(frame)
-    assert np.abs(f - ref_f).max() <= 2e-6
+    assert np.abs(f - ref_f).max() <= 1.5e-5
     b = host(rgb).astype(np.int32)
     diff = np.abs(b - ref_b.astype(np.int32))
     assert diff.max() <= 1 and np.mean(diff > 0) < 1e-3
