@@ -6,7 +6,8 @@
 // 4096 keys per workgroup: a digit count, a per-digit scan over tiles, and a
 // scatter whose within-tile ranks come from wave64 match masks (8 ballots),
 // so equal digits keep their input order and the result is the permutation
-// of a sort on (code, index).
+// of a sort on (code, index).  The scatter reorders each tile by digit in
+// LDS before it writes, so its global writes are contiguous runs.
 #include <stdio.h>
 #include <string.h>
 
@@ -104,25 +105,34 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v,
   return before + x - v;
 }
 
+// Digit counts of one tile: plain LDS atomics (the order of equal digits
+// does not matter for a count), one histogram per wave to spread conflicts.
 __global__ __launch_bounds__(kSortThreads) void k_radix_count(
     const uint64_t *__restrict__ keys, int64_t n, int shift,
     uint32_t *__restrict__ hist, int ntiles) {
-  __shared__ uint32_t cnt[kRadix];
+  constexpr int kWaves = kSortThreads / kWave;
+  __shared__ uint32_t cnt[kWaves][kRadix];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  cnt[tid] = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) cnt[w][tid] = 0;
   __syncthreads();
   const int64_t base = int64_t(blockIdx.x) * kSortTile + wave * (kSortItems * kWave);
-#pragma unroll 4
+  uint64_t k[kSortItems];
+#pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     const int64_t idx = base + r * kWave + lane;
-    const bool valid = idx < n;
-    const uint32_t d = valid ? uint32_t(keys[idx] >> shift) & (kRadix - 1) : 0u;
-    const uint64_t m = match_digit(d, valid);
-    if (valid && lane == __ffsll((unsigned long long)m) - 1)
-      atomicAdd(&cnt[d], (uint32_t)__popcll(m));
+    k[r] = idx < n ? keys[idx] : 0ull;
+  }
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t idx = base + r * kWave + lane;
+    if (idx < n) atomicAdd(&cnt[wave][uint32_t(k[r] >> shift) & (kRadix - 1)], 1u);
   }
   __syncthreads();
-  hist[int64_t(tid) * ntiles + blockIdx.x] = cnt[tid];
+  uint32_t total = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) total += cnt[w][tid];
+  hist[int64_t(tid) * ntiles + blockIdx.x] = total;
 }
 
 // One block per digit: exclusive scan of that digit's per-tile counts.
@@ -141,6 +151,11 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_rowscan(
   if (threadIdx.x == 0) digit_tot[blockIdx.x] = running;
 }
 
+// Stable scatter of one tile, staged through LDS: every key gets its rank
+// among the tile's keys of the same digit (wave match masks, then waves in
+// order), the tile is reordered by digit in LDS, and the write-out walks
+// the reordered tile so that consecutive threads write consecutive
+// addresses of one digit's output run.
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint64_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
     uint64_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out, int64_t n,
@@ -148,15 +163,19 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint32_t *__restrict__ digit_tot, int ntiles) {
   constexpr int kWaves = kSortThreads / kWave;
   __shared__ uint32_t wcnt[kWaves][kRadix];
-  __shared__ uint32_t tile_base[kRadix];
+  __shared__ uint32_t lstart[kRadix];  // first tile-local slot of each digit
+  __shared__ uint32_t gofs[kRadix];    // output position - tile-local slot
+  __shared__ uint64_t sk[kSortTile];
+  __shared__ uint32_t sv[kSortTile];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
 #pragma unroll
   for (int w = 0; w < kWaves; ++w) wcnt[w][tid] = 0;
   const uint32_t dstart = block_exclusive_scan_256(digit_tot[tid], nullptr);
-  tile_base[tid] = dstart + hist[int64_t(tid) * ntiles + blockIdx.x];
+  const uint32_t gbase = dstart + hist[int64_t(tid) * ntiles + blockIdx.x];
   __syncthreads();
 
-  const int64_t base = int64_t(blockIdx.x) * kSortTile + wave * (kSortItems * kWave);
+  const int64_t tile0 = int64_t(blockIdx.x) * kSortTile;
+  const int64_t base = tile0 + wave * (kSortItems * kWave);
   const uint64_t lt = lanemask_lt();
   uint64_t k[kSortItems];
   uint32_t v[kSortItems], rank[kSortItems];
@@ -174,24 +193,37 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
       wcnt[wave][d] = prior + (uint32_t)__popcll(m);
   }
   __syncthreads();
-  {
-    uint32_t run = 0;
+  uint32_t run = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
-      const uint32_t t = wcnt[w][tid];
-      wcnt[w][tid] = run;
-      run += t;
-    }
+  for (int w = 0; w < kWaves; ++w) {
+    const uint32_t t = wcnt[w][tid];
+    wcnt[w][tid] = run;
+    run += t;
   }
+  const uint32_t ls = block_exclusive_scan_256(run, nullptr);
+  lstart[tid] = ls;
+  gofs[tid] = gbase - ls;  // modular: gofs + slot is the output position
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     const int64_t idx = base + r * kWave + lane;
     if (idx < n) {
       const uint32_t d = uint32_t(k[r] >> shift) & (kRadix - 1);
-      const uint32_t pos = tile_base[d] + wcnt[wave][d] + rank[r];
-      keys_out[pos] = k[r];
-      vals_out[pos] = v[r];
+      const uint32_t slot = lstart[d] + wcnt[wave][d] + rank[r];
+      sk[slot] = k[r];
+      sv[slot] = v[r];
+    }
+  }
+  __syncthreads();
+  const int64_t valid_in_tile = n - tile0 < kSortTile ? n - tile0 : kSortTile;
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int j = r * kSortThreads + tid;
+    if (j < valid_in_tile) {
+      const uint64_t key = sk[j];
+      const uint32_t pos = gofs[uint32_t(key >> shift) & (kRadix - 1)] + uint32_t(j);
+      keys_out[pos] = key;
+      vals_out[pos] = sv[j];
     }
   }
 }
