@@ -264,6 +264,8 @@ const char *cosmo_status_string(int status) {
       return "size exceeds the context capacity";
     case COSMO_E_MALFORMED_TREE:
       return "malformed tree";
+    case COSMO_E_FFT:
+      return "hipFFT error";
     default:
       return "unknown status";
   }

@@ -16,6 +16,8 @@
 // holds more than 64 entries and lives in three VGPRs indexed by lane
 // (v_readlane with a wave-uniform slot; a push selects one lane).  Node records are
 // wave-uniform loads (scalar cache).
+#include <cstring>
+
 #include "common.h"
 
 namespace cosmo {
@@ -30,6 +32,7 @@ struct TravParams {
   float half_box;
   float straddle_mu;  // AABB: nodes lighter than this pass the minimum-image guard
   int wave_stats;   // STATS: also store each wavefront's counts
+  int big_bits;     // REF: bits of 2e-7 theta^2 (fixed theta; see k_traverse)
   const int32_t *order;  // launch order of the groups (nullable)
   int64_t n_waves;       // number of groups when order is given
 };
@@ -220,6 +223,19 @@ __global__ __launch_bounds__(256) void k_traverse(
     ax[h] = ay[h] = az[h] = cx[h] = cy[h] = cz[h] = 0.0f;
     depth[h] = active[h] ? 1 : 0;
   }
+  // A node with sx^2 >= 2e-7 theta^2 (largest theta of the wavefront) is
+  // "big": a lane within 1e-7 + 2e-10 of its CoM has theta^2 d^2 below half
+  // of sx^2, far outside the rounding band, so it rejects the node and the
+  // d^2 > 1e-7 cut cannot change a decision.  For big nodes the cut and its
+  // band are skipped.  Compared as integer bits (sx^2 >= 0; a leaf's -1 is
+  // negative), so the test is a scalar compare.
+  int big_bits = p.big_bits;
+  if (GROUP_THETA) {
+    float th2max = th2[0];
+#pragma unroll
+    for (int h = 1; h < PPL; ++h) th2max = fmaxf(th2max, th2[h]);
+    big_bits = __builtin_amdgcn_readfirstlane(__float_as_int(th2max * 2e-7f));
+  }
   int stk = (lane == 0) ? root : 0;  // lane k holds stack slot k
   int top = 1;
 
@@ -290,8 +306,17 @@ __global__ __launch_bounds__(256) void k_traverse(
         const float t = th2[h] * d.d2;
         accept = sx2 < t;
         const float bnd = __builtin_fmaf(k1[h], B.y, k2[h]);
-        near_ok = d.d2 > 1e-7f;
-        const bool unsure = in[h] & ((fabsf(d.d2 - 1e-7f) <= 2e-10f) | (fabsf(sx2 - t) <= bnd));
+        bool unsure;
+        if (__float_as_int(B.x) >= big_bits) {
+          near_ok = true;
+          unsure = in[h] & (fabsf(sx2 - t) <= bnd);
+        } else {
+          // (the empty asm keeps this path a branch: the compiler would
+          // otherwise evaluate it for every node and select)
+          asm volatile("");
+          near_ok = d.d2 > 1e-7f;
+          unsure = in[h] & ((fabsf(d.d2 - 1e-7f) <= 2e-10f) | (fabsf(sx2 - t) <= bnd));
+        }
         if (__builtin_expect(unsure, 0)) {
           const double e2 = exact_d2(A.x, A.y, A.z, x[h], y[h], z[h]);
           if (!leaf) {
@@ -576,6 +601,10 @@ static int traverse_impl(const float *px, const float *py, const float *pz, cons
   const float thf = static_cast<float>(theta);
   p.th2d = theta * theta;
   p.th2 = thf * thf;
+  {
+    const float big = p.th2 * 2e-7f;
+    memcpy(&p.big_bits, &big, sizeof big);
+  }
   p.inv_theta = 1.0f / thf;
   p.theta = thf;
   p.G = static_cast<float>(G);

@@ -53,6 +53,8 @@ SIGNATURES = {
     "cosmo_build_tree_spans": ([_p, _p, _p, _p, _i64, _p], _int),
     "cosmo_compute_multipoles": ([_p] * 11 + [_i64, _p], _int),
     "cosmo_normalize_com": ([_p, _p, _i64, _p], _int),
+    "cosmo_zeldovich_work_bytes": ([_i64], _i64),
+    "cosmo_zeldovich_ics": ([_i64, _p, _f64, _f64, _p] + [_p] * 6 + [_p], _int),
     "cosmo_find_root": ([_p, _p, _i64, _p], _int),
     "cosmo_build_top_tree_cache": ([_p, _p, _p, _p, _i64, _p], _int),
     "cosmo_pack_nodes": ([_p] * 5 + [_i64, _f64, _int, _p, _p], _int),

@@ -27,9 +27,14 @@ def velocity_coupling(config) -> float:
     return float(h_start * om_a ** 0.55 * a0 / 20)
 
 
+def draw_phases(n_grid: int, seed: int):
+    """Phases U[0, 2 pi) of the n^3 modes (cosmology.py:35), PCG64(seed)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return rng.uniform(0, 2 * np.pi, (n_grid, n_grid, n_grid))
+
+
 def displacement_field(n_grid: int, seed: int):
     """Unscaled Zel'dovich displacement (cosmology.py:27-46), FP64 [3, n^3]."""
-    rng = np.random.Generator(np.random.PCG64(seed))
     k = np.fft.fftfreq(n_grid) * n_grid
     kx, ky, kz = np.meshgrid(k, k, k, indexing="ij")
     k2 = kx ** 2 + ky ** 2 + kz ** 2
@@ -37,7 +42,7 @@ def displacement_field(n_grid: int, seed: int):
     k_mag[0, 0, 0] = 1e-10
     P = k_mag ** -2.5
     P[0, 0, 0] = 0
-    phase = rng.uniform(0, 2 * np.pi, (n_grid, n_grid, n_grid))
+    phase = draw_phases(n_grid, seed)
     delta_k = np.sqrt(P) * (np.cos(phase) + 1j * np.sin(phase))
     del phase, P
     k_mag2 = k_mag ** 2 + 1e-10
@@ -62,6 +67,34 @@ def generate_zeldovich_ics(config, seed=None):
     coupling = velocity_coupling(config)
     pos = [((g + d[c]) % 1.0).astype(np.float32) for c, g in enumerate(grid)]
     vel = [(d[c] * coupling).astype(np.float32) for c in range(3)]
+    return pos, vel
+
+
+def generate_zeldovich_ics_device(config, seed=None, device="cuda"):
+    """generate_zeldovich_ics with everything after the phase draw on the GPU
+    (cosmo_zeldovich_ics: spectrum, three hipFFT inverse transforms, std,
+    lattice + displacement): the same phases, and FP32 outputs equal to the
+    host version's except where the FFT libraries' different rounding
+    crosses an FP32 rounding boundary.  Returns ([pos_x, pos_y, pos_z],
+    [vel_x, vel_y, vel_z]) as float32 tensors on `device`."""
+    import torch
+    from src.kernels import _hip
+    n_grid = int(config.N_GRID)
+    seed = config.SEED if seed is None else seed
+    dev = torch.device(device)
+    phase = torch.from_numpy(draw_phases(n_grid, seed).reshape(-1)).to(dev)
+    L = _hip.load()
+    work = torch.empty(int(L.cosmo_zeldovich_work_bytes(n_grid)), dtype=torch.uint8, device=dev)
+    n3 = n_grid ** 3
+    pos = [torch.empty(n3, dtype=torch.float32, device=dev) for _ in range(3)]
+    vel = [torch.empty(n3, dtype=torch.float32, device=dev) for _ in range(3)]
+    _hip.check(L.cosmo_zeldovich_ics(
+        n_grid, _hip.ptr(phase, torch.float64, "phase"),
+        float(getattr(config, "ZA_AMPLITUDE", 0.3)), velocity_coupling(config),
+        _hip.ptr(work, torch.uint8, "work"),
+        *[_hip.ptr(t, torch.float32, "pos") for t in pos],
+        *[_hip.ptr(t, torch.float32, "vel") for t in vel], _hip.stream_handle()),
+        "zeldovich_ics")
     return pos, vel
 
 

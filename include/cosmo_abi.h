@@ -35,6 +35,7 @@ extern "C" {
 #define COSMO_E_HIP 2            /* HIP runtime error                      */
 #define COSMO_E_TOO_LARGE 3      /* n exceeds the context's capacity       */
 #define COSMO_E_MALFORMED_TREE 4 /* tree validator found a defect          */
+#define COSMO_E_FFT 5            /* hipFFT plan or execution failed        */
 
 /* Morton mask sets: FULL21 = 21 bits per axis (63-bit code);
  * REF10 = the masks of the reference's spread_bits as written, which keep
@@ -322,6 +323,22 @@ int cosmo_direct_sum(const float *pos_x, const float *pos_y,
                      const float *pos_z, const float *mass, int64_t n,
                      const int64_t *targets, int64_t n_targets, double G,
                      double soft, double *acc, void *stream);
+
+/* src/utils/cosmology.py:12-73 generate_zeldovich_ics after the phase draw
+ * (:35), on the device: phase is double[n^3] in [0, 2 pi), flat index
+ * (i n + j) n + l with i along x (numpy's ij meshgrid order).  P(k) = k^-2.5
+ * with k = fftfreq(n) n, displacement d_c = Re ifftn(i k_c / (k^2 + 1e-10)
+ * sqrt(P) e^{i phase}) (three in-place double-complex inverse FFTs through
+ * hipFFT), scaled by (1/n) amplitude / std(d_x); positions = (lattice + d)
+ * mod 1 and velocities = d * coupling, rounded to float.  work: device
+ * memory of cosmo_zeldovich_work_bytes(n_grid) bytes (the FFT buffer, the
+ * three components and the reduction scratch), overwritten.  n_grid in
+ * [1, 1024].  Returns COSMO_E_FFT if hipFFT fails. */
+int64_t cosmo_zeldovich_work_bytes(int64_t n_grid);
+int cosmo_zeldovich_ics(int64_t n_grid, const double *phase, double amplitude,
+                        double coupling, void *work, float *pos_x, float *pos_y,
+                        float *pos_z, float *vel_x, float *vel_y, float *vel_z,
+                        void *stream);
 
 /* Tree validator (no reference counterpart): writes a bit set to
  * flags_out (device int32[1]): 1 = parentless-node count != 1,

@@ -66,3 +66,20 @@ def test_invalid_arguments_rejected_without_gpu():
     assert L.cosmo_tree_bytes(10, 0) == 19 * 32  # one record per node id
     assert L.cosmo_tree_bytes(10, 1) == 10 * 16 + 10 * 64
     assert L.cosmo_render_density(None, None, None, None, None, 0, 10, 20, None) == 1
+
+
+def test_zeldovich_ics_arguments_checked_on_host():
+    """Sizes outside [1, 1024] and null pointers are refused before any HIP
+    call (so this runs without a GPU)."""
+    from src.kernels import _hip
+    if not os.path.exists(_hip.LIB_PATH):
+        pytest.skip("libcosmo_hip.so not built")
+    L = _hip.load()
+    assert L.cosmo_zeldovich_work_bytes(0) == 0
+    assert L.cosmo_zeldovich_work_bytes(2048) == 0
+    assert L.cosmo_zeldovich_work_bytes(4) == 64 * 16 + 3 * 64 * 8 + 1026 * 8
+    assert L.cosmo_zeldovich_ics(0, None, 0.3, 1.0, None, None, None, None, None, None, None,
+                                 None) == 1
+    assert L.cosmo_zeldovich_ics(4, None, 0.3, 1.0, None, None, None, None, None, None, None,
+                                 None) == 1
+    assert b"hipFFT" in L.cosmo_status_string(5)
