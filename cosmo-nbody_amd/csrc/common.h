@@ -53,6 +53,24 @@ constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 
+// Split traversal (small launches): at most this many top-tree subtrees per
+// wavefront group, top-tree paths of at most kSplitMaxPath nodes, and at
+// most kSplitMaxCount particles per launch.
+constexpr int kSplitMaxFrontier = 64;
+constexpr int kSplitMaxPath = 16;
+constexpr int64_t kSplitMaxCount = 65536;
+
+struct SplitFrontier {
+  int count;
+  int pad[3];
+  int node[kSplitMaxFrontier];
+  int slot[kSplitMaxFrontier];   // stack slot of the subtree root
+  int plen[kSplitMaxFrontier];   // ancestors on the path from the root
+  uint32_t lmask[kSplitMaxFrontier];  // bit k: leftmost subtree below path[k]
+  int path[kSplitMaxFrontier][kSplitMaxPath];
+  int pslot[kSplitMaxFrontier][kSplitMaxPath];
+};
+
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 // Mask of lanes below this one.
@@ -74,4 +92,6 @@ struct cosmo_ctx {
   uint32_t *digit_tot;  // [kRadix]
   void *nodes;          // packed records, cosmo_tree_bytes(n_max, AABB)
   int32_t *scratch_i32; // [4 n_max + 4] validator counters
+  void *split_frontier; // cosmo::SplitFrontier of the split traversal
+  float *split_partial; // [kSplitMaxFrontier][min(n_max, kSplitMaxCount)][3]
 };

@@ -11,6 +11,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
+
 #include "common.h"
 
 namespace cosmo {
@@ -287,6 +289,10 @@ int cosmo_ctx_create(int64_t n_max, cosmo_ctx **out) {
   if (e == hipSuccess) e = hipMalloc(&c->digit_tot, sizeof(uint32_t) * kRadix);
   if (e == hipSuccess) e = hipMalloc(&c->nodes, cosmo_tree_bytes(n_max, COSMO_MAC_AABB));
   if (e == hipSuccess) e = hipMalloc(&c->scratch_i32, sizeof(int32_t) * (4 * n_max + 4));
+  if (e == hipSuccess) e = hipMalloc(&c->split_frontier, sizeof(SplitFrontier));
+  if (e == hipSuccess)
+    e = hipMalloc(&c->split_partial, sizeof(float) * 3 * kSplitMaxFrontier *
+                                         std::min<int64_t>(n_max, kSplitMaxCount));
   if (e != hipSuccess) {
     set_hip_error(e);
     cosmo_ctx_destroy(c);
@@ -298,6 +304,8 @@ int cosmo_ctx_create(int64_t n_max, cosmo_ctx **out) {
 
 int cosmo_ctx_destroy(cosmo_ctx *c) {
   if (!c) return COSMO_OK;
+  (void)hipFree(c->split_frontier);
+  (void)hipFree(c->split_partial);
   (void)hipFree(c->keys_alt);
   (void)hipFree(c->vals_a);
   (void)hipFree(c->vals_b);

@@ -16,6 +16,7 @@
 // holds more than 64 entries and lives in three VGPRs indexed by lane
 // (v_readlane with a wave-uniform slot; a push selects one lane).  Node records are
 // wave-uniform loads (scalar cache).
+#include <algorithm>
 #include <cstring>
 
 #include "common.h"
@@ -157,6 +158,52 @@ __device__ __forceinline__ void accumulate(const Disp &d, float gm, float eps2, 
   }
 }
 
+// The reference test on one node for one lane (physics_kernels.py:122-136):
+// accept = sx^2 < theta^2 d^2 and near_ok = d^2 > 1e-7, both as the
+// reference decides them in FP64.  The FP32 decision is final unless
+// rounding could flip it.  With |d_k - d_exact,k| <= u |c_k - p_k| <= u and a
+// few u on the products, E = |error of sx^2 - theta^2 d^2| <=
+// theta^2 u (10.1 d^2 + 3 |d|_1) <= 11.8 u theta^2 |d|_1 (|d| <= sqrt(3)/2,
+// |d|_1 <= 3/2).  A flip needs |sx^2 - theta^2 d^2| <= 2E, so
+// theta^2 d^2 <= sx^2 + 35.4 u, |d|_1 <= sqrt(3) d <= 1.7321 sx / theta +
+// 2.52e-3 and 2E <= u (40.9 theta sx + 0.0595 theta^2), which the band
+// k1 sx + k2 covers.  Near the 1e-7 cut |d^2 - d_exact^2| <= 2e-10.  Lanes
+// of the entry (in) inside either band redo both tests in FP64 exactly as
+// the reference does.  A leaf has sx^2 = -1: always accepted, never inside
+// the band.  A node with sx^2 bits >= big_bits (sx^2 >= 2e-7 theta^2) is
+// "big": a lane within 1e-7 + 2e-10 of its CoM has theta^2 d^2 below half of
+// sx^2, far outside the band, so it rejects the node and the cut cannot
+// change a decision; for big nodes the cut and its band are skipped.
+__device__ __forceinline__ void ref_decide(const float4 &A, const float4 &B, bool leaf,
+                                           float x, float y, float z, const Disp &d,
+                                           float th2, double th2d, float k1, float k2,
+                                           int big_bits, bool in, bool &accept,
+                                           bool &near_ok) {
+  const float sx2 = B.x;
+  const float t = th2 * d.d2;
+  accept = sx2 < t;
+  const float bnd = __builtin_fmaf(k1, B.y, k2);
+  bool unsure;
+  if (__float_as_int(B.x) >= big_bits) {
+    near_ok = true;
+    unsure = in & (fabsf(sx2 - t) <= bnd);
+  } else {
+    // (the empty asm keeps this path a branch: the compiler would
+    // otherwise evaluate it for every node and select)
+    asm volatile("");
+    near_ok = d.d2 > 1e-7f;
+    unsure = in & ((fabsf(d.d2 - 1e-7f) <= 2e-10f) | (fabsf(sx2 - t) <= bnd));
+  }
+  if (__builtin_expect(unsure, 0)) {
+    const double e2 = exact_d2(A.x, A.y, A.z, x, y, z);
+    if (!leaf) {
+      const double sxd = static_cast<double>(pin(B.y));
+      accept = (sxd * sxd) < (th2d * e2);
+    }
+    near_ok = e2 > 1e-7;
+  }
+}
+
 // PPL particles per lane: a wavefront walks for 64*PPL Morton-consecutive
 // particles (lane l holds particles base + 64h + l, h < PPL).  The stack holds
 // node ids only.  The lanes of an entry are those that opened its parent, so
@@ -223,12 +270,9 @@ __global__ __launch_bounds__(256) void k_traverse(
     ax[h] = ay[h] = az[h] = cx[h] = cy[h] = cz[h] = 0.0f;
     depth[h] = active[h] ? 1 : 0;
   }
-  // A node with sx^2 >= 2e-7 theta^2 (largest theta of the wavefront) is
-  // "big": a lane within 1e-7 + 2e-10 of its CoM has theta^2 d^2 below half
-  // of sx^2, far outside the rounding band, so it rejects the node and the
-  // d^2 > 1e-7 cut cannot change a decision.  For big nodes the cut and its
-  // band are skipped.  Compared as integer bits (sx^2 >= 0; a leaf's -1 is
-  // negative), so the test is a scalar compare.
+  // threshold of the "big" nodes of ref_decide (largest theta of the
+  // wavefront), compared as integer bits (sx^2 >= 0; a leaf's -1 is
+  // negative), so the test is a scalar compare
   int big_bits = p.big_bits;
   if (GROUP_THETA) {
     float th2max = th2[0];
@@ -290,18 +334,9 @@ __global__ __launch_bounds__(256) void k_traverse(
       Disp d = displace(A.x, A.y, A.z, x[h], y[h], z[h]);
       bool accept, near_ok;
       if constexpr (MAC == COSMO_MAC_REF) {
-        // The reference decides in FP64 (physics_kernels.py:122-136).  The
-        // FP32 decision is final unless rounding could flip it.  With
-        // |d_k - d_exact,k| <= u |c_k - p_k| <= u and a few u on the products,
-        // E = |error of sx^2 - theta^2 d^2| <= theta^2 u (10.1 d^2 + 3 |d|_1)
-        // <= 11.8 u theta^2 |d|_1 (|d| <= sqrt(3)/2, |d|_1 <= 3/2).  A flip
-        // needs |sx^2 - theta^2 d^2| <= 2E, so theta^2 d^2 <= sx^2 + 35.4 u,
-        // |d|_1 <= sqrt(3) d <= 1.7321 sx / theta + 2.52e-3 and
-        // 2E <= u (40.9 theta sx + 0.0595 theta^2), which the band below
-        // covers.  Near the 1e-7 cut (physics_kernels.py:136)
-        // |d^2 - d_exact^2| <= 2e-10.  Lanes inside either band redo both
-        // tests in FP64 exactly as the reference does.  A leaf has
-        // sx^2 = -1: always accepted, never inside the band.
+        // ref_decide() written out: the same decision, kept inline here because
+        // passing the lane masks through a function costs this loop extra
+        // VALU instructions (measured in the generated code)
         const float sx2 = B.x;
         const float t = th2[h] * d.d2;
         accept = sx2 < t;
@@ -311,8 +346,6 @@ __global__ __launch_bounds__(256) void k_traverse(
           near_ok = true;
           unsure = in[h] & (fabsf(sx2 - t) <= bnd);
         } else {
-          // (the empty asm keeps this path a branch: the compiler would
-          // otherwise evaluate it for every node and select)
           asm volatile("");
           near_ok = d.d2 > 1e-7f;
           unsure = in[h] & ((fabsf(d.d2 - 1e-7f) <= 2e-10f) | (fabsf(sx2 - t) <= bnd));
@@ -441,6 +474,253 @@ __global__ __launch_bounds__(256) void k_traverse(
       ws[2] = w_pop;
     }
   }
+}
+
+// ------------------------------------------------------------ split walk
+//
+// A launch over few particles (the reference's 5000-particle batches,
+// engine.py:91-111) has few wavefronts, and each walks the whole tree alone
+// on its CU: the walk is bound by the latency of its chain of dependent
+// record loads.  The split walk gives each group of 64 particles F
+// wavefronts, one per subtree below a small top tree (the frontier, built
+// by k_split_frontier).  Wavefront (g, q) first decides the top-tree nodes
+// on the path to subtree q for its lanes exactly as the walk would (a lane
+// that accepts an ancestor interacts with it in the wavefront of the
+// leftmost subtree below that ancestor and goes no further; a massless
+// ancestor is skipped with its subtree), then walks subtree q for the lanes
+// that opened every ancestor, starting at the stack slot the subtree root
+// has in the one-wavefront walk, so the 64-entry overflow rule applies at
+// the same depths.  Every lane visits and accepts exactly the nodes it does
+// in k_traverse; its terms are summed per subtree, and k_split_reduce adds
+// the subtree sums in subtree order: the same terms, associated
+// differently.  Reference test only (packed 32-byte records per node id).
+
+// The frontier: breadth-first expansion of the top tree (one wavefront, one
+// entry per lane) until `target` subtrees, kept in depth-first order (c1
+// before c2).  Children come from the records (c1, c2 at words 6 and 7).
+__global__ __launch_bounds__(64) void k_split_frontier(const void *__restrict__ tree, int n_tree,
+                                                       const int32_t *__restrict__ root_idx,
+                                                       int target,
+                                                       SplitFrontier *__restrict__ fr) {
+  __shared__ int s_node[2][kSplitMaxFrontier], s_slot[2][kSplitMaxFrontier];
+  __shared__ int s_plen[2][kSplitMaxFrontier];
+  __shared__ uint32_t s_lmask[2][kSplitMaxFrontier];
+  __shared__ int s_path[2][kSplitMaxFrontier][kSplitMaxPath];
+  __shared__ int s_pslot[2][kSplitMaxFrontier][kSplitMaxPath];
+  const int t = threadIdx.x;
+  const int32_t *words = reinterpret_cast<const int32_t *>(tree);
+  const int root = root_idx[0];
+  int cnt = root < 0 ? 0 : 1;
+  int cur = 0;
+  if (t == 0) {
+    s_node[0][0] = root;
+    s_slot[0][0] = 0;
+    s_plen[0][0] = 0;
+    s_lmask[0][0] = 0;
+  }
+  __syncthreads();
+  for (int level = 0; level < kSplitMaxPath - 1 && cnt < target; ++level) {
+    const bool have = t < cnt;
+    const int node = have ? s_node[cur][t] : 0;
+    const int slot = have ? s_slot[cur][t] : 0;
+    const int plen = have ? s_plen[cur][t] : 0;
+    const bool expandable = have && plen == level && node >= n_tree && slot + 1 < 64;
+    const int rank = __popcll(__ballot(expandable) & lanemask_lt());
+    const bool expand = expandable && rank < target - cnt;
+    const uint64_t xm = __ballot(expand);
+    if (xm == 0) break;
+    const int pos = t + __popcll(xm & lanemask_lt());
+    const int nxt = cur ^ 1;
+    if (have) {
+      const int outs = expand ? 2 : 1;
+      for (int o = 0; o < outs; ++o) {
+        for (int k = 0; k < plen; ++k) {
+          s_path[nxt][pos + o][k] = s_path[cur][t][k];
+          s_pslot[nxt][pos + o][k] = s_pslot[cur][t][k];
+        }
+      }
+      if (expand) {
+        const int c1 = words[8 * int64_t(node) + 6], c2 = words[8 * int64_t(node) + 7];
+        s_path[nxt][pos][plen] = s_path[nxt][pos + 1][plen] = node;
+        s_pslot[nxt][pos][plen] = s_pslot[nxt][pos + 1][plen] = slot;
+        s_node[nxt][pos] = c1;
+        s_slot[nxt][pos] = slot + 1;
+        s_plen[nxt][pos] = plen + 1;
+        s_lmask[nxt][pos] = s_lmask[cur][t] | (1u << plen);
+        s_node[nxt][pos + 1] = c2;
+        s_slot[nxt][pos + 1] = slot;
+        s_plen[nxt][pos + 1] = plen + 1;
+        s_lmask[nxt][pos + 1] = 0u;
+      } else {
+        s_node[nxt][pos] = node;
+        s_slot[nxt][pos] = slot;
+        s_plen[nxt][pos] = plen;
+        s_lmask[nxt][pos] = s_lmask[cur][t];
+      }
+    }
+    cnt += __popcll(xm);
+    cur = nxt;
+    __syncthreads();
+  }
+  if (t < cnt) {
+    fr->node[t] = s_node[cur][t];
+    fr->slot[t] = s_slot[cur][t];
+    fr->plen[t] = s_plen[cur][t];
+    fr->lmask[t] = s_lmask[cur][t];
+    for (int k = 0; k < s_plen[cur][t]; ++k) {
+      fr->path[t][k] = s_path[cur][t][k];
+      fr->pslot[t][k] = s_pslot[cur][t][k];
+    }
+  }
+  if (t == 0) fr->count = cnt;
+}
+
+template <bool STATS>
+__global__ __launch_bounds__(256) void k_traverse_split(
+    const float *__restrict__ px, const float *__restrict__ py, const float *__restrict__ pz,
+    const void *__restrict__ tree, const SplitFrontier *__restrict__ fr, int F,
+    float *__restrict__ partial, TravParams p, int64_t lo, int64_t hi,
+    unsigned long long *__restrict__ stats) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = __builtin_amdgcn_readfirstlane(
+      static_cast<int>((int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave));
+  const int g = w / F, q = w % F;
+  if (q >= fr->count) return;
+  const int64_t base = lo + int64_t(g) * kWave;
+  if (base >= hi) return;
+  const int64_t i = base + lane;
+  const bool active = i < hi;
+  float x = 0.0f, y = 0.0f, z = 0.0f;
+  if (active) {
+    x = px[i];
+    y = py[i];
+    z = pz[i];
+  }
+  const float th2 = p.th2;
+  const double th2d = p.th2d;
+  const float k1 = (48.0f * kU) * p.theta, k2 = (0.08f * kU) * th2;
+  const int big_bits = p.big_bits;
+  float ax = 0.0f, ay = 0.0f, az = 0.0f, unused0 = 0.0f, unused1 = 0.0f, unused2 = 0.0f;
+  uint32_t w_int = 0, w_vis = 0;
+  const char *tb = reinterpret_cast<const char *>(tree);
+  const int plen = fr->plen[q];
+  const uint32_t lmask = fr->lmask[q];
+  bool reach = active;
+  for (int k = 0; k < plen; ++k) {
+    const int anc = fr->path[q][k];
+    const ConstF *rec = (const ConstF *)(tb + int64_t(anc) * int64_t(sizeof(cosmo_inode_ref)));
+    const float4 A = ld4(rec), B = ld4(rec + 4);
+    if (__float_as_int(A.w) <= 0) {  // the walk skips a massless node and its subtree
+      reach = false;
+      break;
+    }
+    Disp d = displace(A.x, A.y, A.z, x, y, z);
+    bool accept, near_ok;
+    ref_decide(A, B, __float_as_int(B.x) < 0, x, y, z, d, th2, th2d, k1, k2, big_bits, reach,
+               accept, near_ok);
+    const bool hit = reach & accept & near_ok;
+    if ((lmask >> k) & 1u) {
+      if (hit) {
+        fix_ties(d, A.x, A.y, A.z, x, y, z);
+        accumulate<false>(d, A.w, p.eps2, ax, ay, az, unused0, unused1, unused2);
+      }
+      if (STATS) {
+        w_vis += __builtin_popcountll(__ballot(reach));
+        w_int += __builtin_popcountll(__ballot(hit));
+      }
+    }
+    reach = reach & !accept & (fr->pslot[q][k] + 1 < 64);
+  }
+  const int floor_slot = fr->slot[q];
+  if (__ballot(reach) != 0) {
+    int depth = reach ? floor_slot + 1 : 0;
+    int stk = (lane == floor_slot) ? fr->node[q] : 0;
+    // The walk of k_traverse with the next record loaded ahead: while a node
+    // is decided, the records of both possible next pops (its c1 if it is
+    // opened, else the entry below it) are already on their way, so the
+    // load latency overlaps the decision instead of following it.
+    int top = floor_slot;  // slot of the current node
+    int node = fr->node[q];
+    float4 A = ld4((const ConstF *)(tb + int64_t(node) * int64_t(sizeof(cosmo_inode_ref))));
+    float4 B = ld4((const ConstF *)(tb + int64_t(node) * int64_t(sizeof(cosmo_inode_ref))) + 4);
+    for (;;) {
+      const bool in = depth > top;
+      depth = min(depth, top);
+      const int below = top > floor_slot ? __builtin_amdgcn_readlane(stk, top - 1) : node;
+      const int c1 = __float_as_int(B.x) >= 0 ? __float_as_int(B.z) : node;
+      const ConstF *rb = (const ConstF *)(tb + int64_t(below) * int64_t(sizeof(cosmo_inode_ref)));
+      const ConstF *rc = (const ConstF *)(tb + int64_t(c1) * int64_t(sizeof(cosmo_inode_ref)));
+      const float4 Ab = ld4(rb), Bb = ld4(rb + 4), Ac = ld4(rc), Bc = ld4(rc + 4);
+      bool pushed = false;
+      if (__float_as_int(A.w) > 0) {  // massless nodes are skipped
+        const bool room = top + 1 < 64;
+        Disp d = displace(A.x, A.y, A.z, x, y, z);
+        bool accept, near_ok;
+        ref_decide(A, B, __float_as_int(B.x) < 0, x, y, z, d, th2, th2d, k1, k2, big_bits, in,
+                   accept, near_ok);
+        const bool hit = in & accept & near_ok;
+        const bool open = in & !accept & room;
+        if (hit) {
+          fix_ties(d, A.x, A.y, A.z, x, y, z);
+          accumulate<false>(d, A.w, p.eps2, ax, ay, az, unused0, unused1, unused2);
+        }
+        if (STATS) {
+          w_vis += __builtin_popcountll(__ballot(in));
+          w_int += __builtin_popcountll(__ballot(hit));
+        }
+        if (__ballot(open) != 0) {
+          stk = write_lane(stk, __float_as_int(B.w), top);
+          stk = write_lane(stk, c1, top + 1);
+          depth = open ? top + 2 : depth;
+          pushed = true;
+        }
+      }
+      if (pushed) {  // next pop: c1, at slot top + 1
+        top += 1;
+        node = c1;
+        A = Ac;
+        B = Bc;
+      } else if (top > floor_slot) {  // next pop: the entry below
+        top -= 1;
+        node = below;
+        A = Ab;
+        B = Bb;
+      } else {
+        break;
+      }
+    }
+  }
+  if (active) {
+    float *o = partial + (int64_t(q) * (hi - lo) + (i - lo)) * 3;
+    o[0] = ax;
+    o[1] = ay;
+    o[2] = az;
+  }
+  if (STATS && lane == 0) {
+    atomicAdd(&stats[0], static_cast<unsigned long long>(w_int));
+    atomicAdd(&stats[1], static_cast<unsigned long long>(w_vis));
+  }
+}
+
+// force = G * (sum of the subtree partial sums in subtree order).
+__global__ __launch_bounds__(256) void k_split_reduce(const float *__restrict__ partial,
+                                                      const SplitFrontier *__restrict__ fr,
+                                                      int64_t lo, int64_t hi, float G,
+                                                      float *__restrict__ force) {
+  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t m = hi - lo;
+  if (j >= m) return;
+  const int nf = fr->count;
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  for (int q = 0; q < nf; ++q) {
+    const float *o = partial + (int64_t(q) * m + j) * 3;
+    sx += o[0];
+    sy += o[1];
+    sz += o[2];
+  }
+  force[3 * (lo + j) + 0] = G * sx;
+  force[3 * (lo + j) + 1] = G * sy;
+  force[3 * (lo + j) + 2] = G * sz;
 }
 
 // FP64 direct sum: a block of 1024 threads takes 64 targets (one per lane);
@@ -687,8 +967,62 @@ int cosmo_compute_forces(cosmo_ctx *ctx, const float *px, const float *py, const
   const int rc = cosmo_pack_nodes(children, n_mass, n_com, n_min, n_max, 2 * n_bodies - 1, box,
                                   COSMO_MAC_REF, ctx->nodes, stream);
   if (rc != COSMO_OK) return rc;
+  // a launch of at most 1024 groups cannot fill the device with one
+  // wavefront per group: split each group's walk over the top tree
+  const int64_t todo = std::min(off + count, n_bodies) - off;
+  if (todo > 0 && todo <= std::min<int64_t>(ctx->n_max, 1024 * kWave))
+    return cosmo_traverse_split(ctx, px, py, pz, ctx->nodes, n_bodies, root_idx, force, theta,
+                                G, soft, box, off, count, n_bodies, nullptr, stream);
   return cosmo_traverse(px, py, pz, ctx->nodes, n_bodies, root_idx, force, theta, nullptr, G,
                         soft, box, off, count, n_bodies, COSMO_MAC_REF, 0, nullptr, stream);
+}
+
+int cosmo_traverse_split(cosmo_ctx *ctx, const float *px, const float *py, const float *pz,
+                         const void *nodes, int64_t n_tree, const int32_t *root_idx,
+                         float *force, double theta, double G, double soft, double box,
+                         int64_t off, int64_t count, int64_t n_bodies, uint64_t *stats,
+                         void *stream) {
+  (void)box;
+  if (!ctx) return COSMO_E_INVALID;
+  if (n_bodies < 0 || n_bodies > kMaxBodies || off < 0 || count < 0 || n_tree < 1 ||
+      n_tree > kMaxBodies)
+    return COSMO_E_INVALID;
+  const int64_t lo = off;
+  const int64_t hi = (off + count < n_bodies) ? off + count : n_bodies;
+  if (hi <= lo) return COSMO_OK;
+  if (!px || !py || !pz || !nodes || !root_idx || !force) return COSMO_E_INVALID;
+  if (hi - lo > std::min<int64_t>(ctx->n_max, kSplitMaxCount)) return COSMO_E_TOO_LARGE;
+  TravParams p;
+  memset(&p, 0, sizeof p);
+  const float thf = static_cast<float>(theta);
+  p.th2d = theta * theta;
+  p.th2 = thf * thf;
+  {
+    const float big = p.th2 * 2e-7f;
+    memcpy(&p.big_bits, &big, sizeof big);
+  }
+  p.theta = thf;
+  p.G = static_cast<float>(G);
+  const float sf = static_cast<float>(soft);
+  p.eps2 = sf * sf;
+  const int64_t groups = (hi - lo + kWave - 1) / kWave;
+  // enough wavefronts to occupy the device: about 4096 in all
+  const int F = static_cast<int>(std::max<int64_t>(
+      1, std::min<int64_t>(kSplitMaxFrontier, 4096 / groups)));
+  hipStream_t s = as_stream(stream);
+  SplitFrontier *fr = reinterpret_cast<SplitFrontier *>(ctx->split_frontier);
+  k_split_frontier<<<1, kWave, 0, s>>>(nodes, static_cast<int>(n_tree), root_idx, F, fr);
+  const unsigned grid = blocks_for(groups * F * kWave, 256);
+  if (stats)
+    k_traverse_split<true><<<grid, 256, 0, s>>>(px, py, pz, nodes, fr, F, ctx->split_partial, p,
+                                                lo, hi, reinterpret_cast<unsigned long long *>(stats));
+  else
+    k_traverse_split<false><<<grid, 256, 0, s>>>(px, py, pz, nodes, fr, F, ctx->split_partial, p,
+                                                 lo, hi, nullptr);
+  k_split_reduce<<<blocks_for(hi - lo, 256), 256, 0, s>>>(ctx->split_partial, fr, lo, hi, p.G,
+                                                          force);
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
 }
 
 int cosmo_direct_sum(const float *px, const float *py, const float *pz, const float *mass,

@@ -443,7 +443,9 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
         return {"reachable": False, "val_state": val, "target": target,
                 "fixed_curve": fixed_curve}
     e_fixed, i_fixed = probe.rel_rms(theta_star)
+    log(f"fixed theta meeting {target:.1e}: {theta_star:.4f} (error {e_fixed:.4e})")
     t_fixed = _time_traversal(eng, theta_star, None)
+    log(f"fixed-theta traversal: {t_fixed:.1f} ms")
     E = np.stack(err2_k, 1)
     cost_arr = np.asarray(cost, np.float64)
     bound = None
@@ -473,10 +475,12 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
 
     lambdas = np.logspace(-2, 14, 129) if lambdas is None else lambdas
     sweep = []
-    for lam in lambdas:
+    for j, lam in enumerate(lambdas):
         tg = lt.theta_groups(eng, float(lam))
         e, inter = probe.rel_rms(0.5, tg.index_select(0, vg).contiguous())
         sweep.append((float(lam), e, inter, model_cost(tg)))
+        if j % 32 == 31:
+            log(f"lambda sweep {j + 1}/{len(lambdas)}")
     ok = sorted((sw for sw in sweep if sw[1] <= target), key=lambda sw: sw[3])
     if not ok:
         return {"reachable": False, "val_state": val, "target": target,
@@ -485,6 +489,7 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
     for lam, _, _, _ in ok[:5]:  # the cheapest settings by the cost model, timed
         tg = lt.theta_groups(eng, lam).clone()
         t = _time_traversal(eng, 0.5, tg, group_order=lt.launch_order(tg).clone())
+        log(f"lambda {lam:.3g}: learned traversal {t:.1f} ms")
         if best is None or t < best[1]:
             best = (lam, t)
     lt.lam = best[0]
@@ -500,7 +505,9 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
     torch.cuda.synchronize()
     t_select = e0.elapsed_time(e1) / 5
     step_fixed = _time_steps(c.replace(THETA=theta_star), val, None, graph=graph)
+    log(f"fixed-theta step: {step_fixed:.1f} ms")
     step_learned = _time_steps(c, val, lt, graph=graph)
+    log(f"learned-theta step: {step_learned:.1f} ms")
     hist_t = np.histogram(tg.cpu().numpy(), bins=np.concatenate([th_np - 1e-6, [np.inf]]))[0]
     res = {"reachable": True, "preset": c.NAME, "mac": c.MAC, "target": target,
            "bucket": int(getattr(c, "BUCKET", 0)),
@@ -570,6 +577,8 @@ def main():
     ap.add_argument("--eval-preset", default="",
                     help="evaluate on this preset (its own theta costs, same network); "
                          "e.g. a larger N than the training preset")
+    ap.add_argument("--eval-straddle", type=float, default=None,
+                    help="straddle mass of --eval-preset (default: that of the training run)")
     ap.add_argument("--eval-thetas", default="",
                     help="comma-separated candidate thetas for --eval-preset")
     ap.add_argument("--graph", action="store_true",
@@ -593,7 +602,10 @@ def main():
         log = lambda m: print(m, flush=True)  # noqa: E731
         model, _, info = train(c, train_states, args.groups, DEFAULT_THETAS, args.fit_steps,
                                log=log)
-        ce = cfg.preset(args.eval_preset, MAC=args.mac, **over)
+        over_e = dict(over)
+        if args.eval_straddle is not None:
+            over_e["STRADDLE_MASS"] = args.eval_straddle
+        ce = cfg.preset(args.eval_preset, MAC=args.mac, **over_e)
         th = (tuple(float(x) for x in args.eval_thetas.split(",")) if args.eval_thetas
               else DEFAULT_THETAS)
         cost = measure_costs(ce, vals[0], th, log)

@@ -62,6 +62,8 @@ SIGNATURES = {
     "cosmo_normalize_pack_buckets": ([_p] * 6 + [_i64, _f64, _int, _int, _p, _p], _int),
     "cosmo_traverse": ([_p] * 4 + [_i64, _p, _p, _f64, _p, _f64, _f64, _f64, _i64, _i64, _i64,
                                     _int, _int, _p, _p], _int),
+    "cosmo_traverse_split": ([_p] * 5 + [_i64, _p, _p, _f64, _f64, _f64, _f64, _i64, _i64, _i64,
+                              _p, _p], _int),
     "cosmo_traverse_ordered": ([_p] * 4 + [_i64, _p, _p, _f64, _p, _f64, _f64, _f64, _i64, _i64,
                                             _i64, _int, _int, _p, _p, _p], _int),
     "cosmo_traverse_ex": ([_p] * 4 + [_i64, _p, _p, _f64, _p, _f64, _f64, _f64, _i64, _i64,

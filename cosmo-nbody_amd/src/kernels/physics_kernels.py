@@ -88,6 +88,24 @@ def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0
           "traverse")
 
 
+def traverse_split(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0,
+                   count=None, stats=None):
+    """The reference-test walk for a small launch (at most 65536 particles):
+    every 64-particle group gets one wavefront per subtree below a small top
+    tree (cosmo_traverse_split).  Same visited and accepted nodes as
+    traverse(); the per-subtree sums are added in subtree order.  stats:
+    int64[2] accumulating (interactions, node visits)."""
+    n = pos_x.shape[0]
+    count = n - off if count is None else count
+    ctx = shared_context(n)
+    st = None if stats is None else ptr(stats, (torch.int64, torch.uint64), "stats", numel=2)
+    check(load().cosmo_traverse_split(
+        ctx.handle, *_xyz(pos_x, pos_y, pos_z, n), ptr(nodes, torch.uint8, "nodes"), n,
+        ptr(root, torch.int32, "root"), ptr(force, _F32, "force", numel=3 * n), float(theta),
+        float(G), float(soft), float(box), int(off), int(count), n, st, stream_handle()),
+        "traverse_split")
+
+
 def order_groups(theta_group, out=None):
     """Launch order of the groups by increasing theta (longest first)."""
     g = theta_group.shape[0]

@@ -219,6 +219,23 @@ int cosmo_traverse(const float *pos_x, const float *pos_y, const float *pos_z,
                    int64_t count, int64_t n_bodies, int mac, int flags,
                    uint64_t *stats, void *stream);
 
+/* The same walk with the reference test for a launch of at most
+ * min(ctx capacity, 65536) particles (no reference counterpart; what
+ * compute_forces uses for the 5000-particle batches of engine.py:91-111):
+ * each group of 64 particles gets one wavefront per subtree below a small
+ * top tree, which decides the top-tree nodes on its path exactly as the
+ * walk does and walks its subtree from the stack depth the walk reaches it
+ * at.  Each particle visits and accepts the same nodes as with
+ * cosmo_traverse; its subtree sums are added in subtree order.  nodes: the
+ * reference test's packed records.  stats (nullable, device uint64[2]):
+ * interactions and node visits are added.  Returns COSMO_E_TOO_LARGE for
+ * larger launches. */
+int cosmo_traverse_split(cosmo_ctx *ctx, const float *pos_x, const float *pos_y,
+                         const float *pos_z, const void *nodes, int64_t n_tree,
+                         const int32_t *root_idx, float *force, double theta, double G,
+                         double soft, double box, int64_t off, int64_t count,
+                         int64_t n_bodies, uint64_t *stats, void *stream);
+
 /* cosmo_traverse with the wavefront groups launched in the order
  * group_order[0..W) (a permutation of the W = ceil(count/64) groups; no
  * reference counterpart).  Results do not depend on the order; a good

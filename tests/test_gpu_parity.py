@@ -251,9 +251,14 @@ def _case(orc, pos, mass=None):
 
 @pytest.mark.parametrize("theta", [0.3, 0.5, 0.7])
 def test_compute_forces_reference_mac(orc, torch_mod, ics32, theta):
-    """compute_forces drop-in (batched launches like engine.py:91-111):
-    identical interaction sets to the FP32 oracle and <= 1e-5 relative RMS
-    against the reference method's FP64 arithmetic."""
+    """compute_forces drop-in (batched launches like engine.py:91-111, which
+    take the split walk): <= 1e-5 relative RMS against the reference method's
+    FP64 arithmetic.  The split walk sums each particle's terms per top-tree
+    subtree, so against the FP32 oracle (one running sum in walk order) the
+    two FP32 roundings of the same terms differ by up to about twice the
+    oracle's own FP32 error (2e-6 at theta = 0.3): 5e-6.  The visited and
+    accepted nodes themselves are compared exactly in
+    test_traverse_split_matches_walk."""
     torch = torch_mod
     from src.kernels.physics_kernels import compute_forces
     pos, mass, ch, nm, com, lo, hi = _case(orc, ics32)
@@ -272,7 +277,59 @@ def test_compute_forces_reference_mac(orc, torch_mod, ics32, theta):
     e64, e32 = rel_rms(f, f64), rel_rms(f, f32)
     print(f"theta={theta} relRMS vs FP64 {e64:.3e} vs FP32 oracle {e32:.3e}")
     assert e64 <= 1e-5
-    assert e32 <= 2e-6
+    assert e32 <= 5e-6
+
+
+def _split_case(orc, torch, pos, theta, soft, slices, mass=None):
+    from src.kernels.physics_kernels import traverse, traverse_split
+    from src.kernels.tree_kernels import pack_nodes
+    pos, mass, ch, nm, com, lo, hi = _case(orc, pos, mass)
+    n = len(mass)
+    nodes = pack_nodes(dev(ch, torch), dev(nm, torch), dev(com, torch), dev(lo, torch),
+                       dev(hi, torch), 1.0, 0)
+    root = torch.tensor([n], dtype=torch.int32, device="cuda")
+    P = [dev(p, torch) for p in pos]
+    f32, ni, nv = orc.compute_forces(*pos, ch, nm, com, lo, hi, theta, 1.0, soft, 1.0, n,
+                                     precision=32, mac=0, counts=True)
+    for off, count in slices:
+        end = min(n, off + count)
+        f_one = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+        f_split = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+        st = torch.zeros(2, dtype=torch.int64, device="cuda")
+        traverse(*P, nodes, root, f_one, theta, 1.0, soft, 1.0, off=off, count=count)
+        traverse_split(*P, nodes, root, f_split, theta, 1.0, soft, 1.0, off=off, count=count,
+                       stats=st)
+        s = host(st)
+        # the same visits and interactions, particle by particle summed
+        assert int(s[0]) == int(ni[off:end].sum())
+        assert int(s[1]) == int(nv[off:end].sum())
+        a, b = host(f_split)[off:end], host(f_one)[off:end]
+        # the same terms summed per subtree: FP32 rounding differs from the
+        # one running sum by up to about twice that sum's own error
+        assert rel_rms(a, f32[off:end]) <= 5e-6
+        assert rel_rms(a, b) <= 5e-6
+        # nothing outside the slice is written
+        assert not host(f_split)[:off].any() and not host(f_split)[end:].any()
+
+
+@pytest.mark.parametrize("theta", [0.3, 0.7])
+def test_traverse_split_matches_walk(orc, torch_mod, ics32, theta):
+    """Split walk (one wavefront per top-tree subtree and group): exactly the
+    oracle's visits and interactions on slices of every size class, forces
+    within 5e-6 of the FP32 oracle and of the one-wavefront walk (the same
+    terms, summed per subtree)."""
+    _split_case(orc, torch_mod, ics32, theta, 0.02 / 32,
+                [(0, 5000), (12345, 777), (0, 32768), (32000, 5000), (100, 1)])
+
+
+def test_traverse_split_stack_overflow(orc, torch_mod):
+    """Codes 2^k (k < 63) and a pair of zeros give a valid tree that is one
+    chain of left (c1) edges ending in the pair, whose parent sits at stack
+    slot 63: at theta = 0 the walk drops the pair there
+    (physics_kernels.py:143).  The split walk, whose top tree covers only
+    the first 16 levels, drops the same interactions (the oracle's counts)."""
+    from test_oracle_stages import _code_chain
+    _split_case(orc, torch_mod, _code_chain(63), 0.0, 1e-3, [(0, 65), (5, 40)])
 
 
 @pytest.mark.parametrize("mac", [0, 1])

@@ -348,26 +348,42 @@ def test_fp32_mode_tracks_fp64(orc, ics32, mac):
     assert np.mean(n32 == n64) > (0.999 if mac == 0 else 0.995)
 
 
+def _code_chain(m):
+    """Positions whose FULL21 Morton codes are 2^k for k < m, plus two with
+    code 0 (at the origin and 1e-9 from it)."""
+    pts = np.zeros((3, m + 2), np.float32)
+    for k in range(m):
+        pts[k % 3, k + 1] = np.float32(2.0 ** (k // 3 - 21))
+    pts[0, m + 1] = np.float32(1e-9)
+    return list(pts)
+
+
 def test_stack_overflow_drops_children(orc):
-    """A geometric chain gives a tree deeper than 64; particles deep in it
-    drop children once their stack would exceed 64 entries
-    (physics_kernels.py:143), so theta=0 no longer equals the direct sum."""
-    n = 100
-    x = (0.5 + 0.25 * (0.5 ** (np.arange(n) * 0.2))).astype(np.float32)
-    x = np.unique(x)
-    n = len(x)
-    pos = [x, np.full(n, 0.5, np.float32), np.full(n, 0.5, np.float32)]
-    p, m, ch, nm, com, nmin, nmax = _tree(orc, pos)
-    f, ni, nv = orc.compute_forces(*p, ch, nm, com, nmin, nmax, 0.0, 1.0, 1e-3, 1.0, n,
-                                   precision=64, counts=True)
-    assert ni.min() < n - 1 - 10  # some particles lost interactions
-    # shallow chain: no drops
-    pos_s = [a[:40] for a in pos]
-    p, m, ch, nm, com, nmin, nmax = _tree(orc, pos_s)
-    f, ni, nv = orc.compute_forces(*p, ch, nm, com, nmin, nmax, 0.0, 1.0, 1e-3, 1.0, len(m),
-                                   precision=64, counts=True)
-    k = len(m)
-    assert np.all(ni == k - (np.diff(np.sort(p[0])).min() ** 2 <= 1e-7) - 1) or np.all(ni <= k - 1)
+    """Codes 2^k (k < m) and a pair of zeros give a valid tree (a pair of
+    equal codes is split by the tie-break) that is one chain of left (c1)
+    edges ending in the pair.  At theta = 0 every internal node is opened.
+    With m = 63 the pair's parent sits at stack slot 63, where the reference
+    drops children (physics_kernels.py:143): the particles that reach it
+    lose exactly the two interactions with the pair.  With m = 40 the stack
+    stays below 64 and every particle interacts with every other one beyond
+    the 1e-7 cut, as in the direct sum."""
+    for m, drops in ((63, True), (40, False)):
+        pos = _code_chain(m)
+        codes, _ = orc.compute_codes(*pos)
+        perm = orc.argsort_stable(codes)
+        assert orc.validate_tree(*orc.build_tree(codes[perm]), m + 2) == 0
+        p, ms, ch, nm, com, nmin, nmax = _tree(orc, pos)
+        f, ni, nv = orc.compute_forces(*p, ch, nm, com, nmin, nmax, 0.0, 1.0, 1e-3, 1.0, m + 2,
+                                       precision=64, counts=True)
+        P = np.stack(p, 1).astype(np.float64)
+        d = P[None, :, :] - P[:, None, :]
+        d -= np.rint(d)
+        beyond = ((d ** 2).sum(2) > 1e-7).sum(1)
+        if drops:
+            lost = beyond - ni
+            assert set(np.unique(lost)) == {0, 2} and np.count_nonzero(lost) > 10
+        else:
+            assert np.array_equal(ni, beyond)
 
 
 def test_direct_sum_three_body(orc):
