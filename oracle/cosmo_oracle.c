@@ -20,7 +20,8 @@
  *   compute_forces    theta=0 -> direct sum (both acceptance tests),
  *                     two-body closed form, FP64 vs FP32 agreement,
  *                     convergence to the direct sum (AABB test), stack-
- *                     overflow drop on a deep chain                 pinned
+ *                     overflow drop on a valid 64-deep chain (exactly the
+ *                     interactions with the dropped pair)           pinned
  *   direct_sum        two-body / three-body closed form             pinned
  *   kick_drift        closed form per element                       pinned
  *   render_density    projection closed form, centre pixel, totals  pinned
