@@ -241,10 +241,18 @@ __global__ __launch_bounds__(kUpChunk) void k_multipoles(
   __shared__ float s_lo0[kUpChunk], s_lo1[kUpChunk], s_lo2[kUpChunk];
   __shared__ float s_hi0[kUpChunk], s_hi1[kUpChunk], s_hi2[kUpChunk];
   __shared__ int s_cnt[kUpChunk];
+  // the chunk's topology, loaded once with coalesced reads: parents of its
+  // leaves and of its internal nodes, children of its internal nodes
+  __shared__ int s_par_leaf[kUpChunk], s_par_int[kUpChunk];
+  __shared__ int2 s_ch[kUpChunk];
   const int t = threadIdx.x;
   const int b0 = blockIdx.x * kUpChunk;
   const int leaf = b0 + t;
   s_cnt[t] = 0;
+  s_par_leaf[t] = leaf < n ? parents[leaf] : -1;
+  s_par_int[t] = leaf < n - 1 ? parents[n + leaf] : -1;
+  s_ch[t] = leaf < n - 1 ? reinterpret_cast<const int2 *>(children)[n + leaf]
+                         : make_int2(-1, -1);
   __syncthreads();
   // 0: climb ended (root, or completed by the sibling's thread);
   // 1: first arrival at chunk node `wait`; 2: continue across chunks.
@@ -254,11 +262,12 @@ __global__ __launch_bounds__(kUpChunk) void k_multipoles(
     v = leaf_moments(px, py, pz, mass, leaf);
     store_plain(n_mass, n_com, n_min, n_max, leaf, v);
     for (int level = 0; level < kUpChunk; ++level) {
-      const int p = parents[curr];
+      // curr is a leaf or an internal node of the chunk
+      const int p = curr < n ? s_par_leaf[curr - b0] : s_par_int[curr - n - b0];
       if (p == -1) break;
       const int slot = p - n - b0;
       if (slot < 0 || slot >= kUpChunk) { state = 2; break; }
-      const int2 ch = reinterpret_cast<const int2 *>(children)[p];
+      const int2 ch = s_ch[slot];
       const bool mine_left = (ch.x == curr);
       const int sib = mine_left ? ch.y : ch.x;
       const int sslot = (sib >= n ? sib - n : sib) - b0;
