@@ -33,6 +33,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--preset", default="repo_128")
+    ap.add_argument("--theta", type=float, default=None,
+                    help="opening angle in place of the preset's")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--pairs", action="store_true",
@@ -117,13 +119,17 @@ def main():
     from src.core.state import SimulationState
     from src.kernels.physics_kernels import integrate
     from src.utils import config as cfg
-    from src.utils.cosmology import generate_zeldovich_ics
+    from src.utils.cosmology import generate_ics
 
     if args.learned:
         learned_line(args)
         return
     c = cfg.preset(args.preset, TRAVERSAL_PAIRS=args.pairs)
-    pos, vel = generate_zeldovich_ics(c)
+    if args.theta is not None:
+        c = c.replace(THETA=args.theta)
+    pos, vel = generate_ics(c)
+    ics_name = ("halo ICs" if getattr(c, "IC_KIND", "zeldovich") == "halos"
+                else "Zel'dovich ICs")
     n = len(pos[0])
     torch.cuda.set_device(0)
     st = SimulationState(n, pos, vel)
@@ -212,7 +218,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": f"{c.NAME}: N={n} Zel'dovich ICs seed {c.SEED}, theta={c.THETA}, "
+        "config": {"workload": f"{c.NAME}: N={n} {ics_name} seed {c.SEED}, theta={c.THETA}, "
                                f"eps={c.SOFTENING:.4g}, MAC={c.MAC}, {c.RES}^2 splat, "
                                f"reference schedule da={c.DA:.3g}",
                    "n_bodies": n, "theta": c.THETA, "res": c.RES, "hip_graph": args.graph},
