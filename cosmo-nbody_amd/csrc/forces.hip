@@ -400,18 +400,18 @@ __global__ __launch_bounds__(256) void k_traverse(
       for (int h = 0; h < PPL; ++h) bm |= __ballot(rej[h]);
       if (bm != 0) {
         const int first = __float_as_int(D.z);
-        float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (lane < bcount) q = leaves[first + lane];
+        // the bucket's leaf records, read as wave-uniform scalar loads
+        const ConstF *lrec = (const ConstF *)(leaves + first);
         // compensated mode: the bucket's terms form a short FP32 partial sum
         // that enters the running sum as one compensated term
         float bx[PPL], by[PPL], bz[PPL];
 #pragma unroll
         for (int h = 0; h < PPL; ++h) bx[h] = by[h] = bz[h] = 0.0f;
+#pragma unroll 4
         for (int k = 0; k < bcount; ++k) {
-          const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.x), k));
-          const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.y), k));
-          const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q.z), k));
-          const int qm = __builtin_amdgcn_readlane(__float_as_int(q.w), k);
+          const float4 L = ld4(lrec + 4 * k);
+          const float qx = L.x, qy = L.y, qz = L.z;
+          const int qm = __float_as_int(L.w);
           if (qm <= 0) continue;  // massless leaves are skipped, as in the walk
 #pragma unroll
           for (int h = 0; h < PPL; ++h) {
