@@ -42,8 +42,8 @@ inline unsigned blocks_for(int64_t n, int threads) {
 // Largest particle count any entry point accepts: node ids (2n-1) and the
 // tree-build search range (up to 4n) stay inside int32.
 constexpr int64_t kMaxBodies = int64_t(1) << 29;
-// Largest leaf bucket of the AABB records: one wavefront loads a bucket's
-// leaf records in one vector load, one record per lane.
+// Largest leaf bucket of the AABB records (a bucket's leaves are summed in
+// one pass of the traversal's leaf loop).
 constexpr int kMaxBucket = 64;
 
 // Radix sort geometry: 256 threads, 16 keys each, 8-bit digits.

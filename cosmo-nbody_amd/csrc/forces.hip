@@ -390,8 +390,8 @@ __global__ __launch_bounds__(256) void k_traverse(
     }
     // AABB records of small nodes carry their leaf range (count > 0): the
     // lanes that reject such a node sum its leaves directly, in leaf order,
-    // which is the order in which the walk would reach them.  The wavefront
-    // loads the leaf records with one vector load and broadcasts them.
+    // which is the order in which the walk would reach them.  The leaf
+    // records are wave-uniform scalar loads.
     int bcount = 0;
     if constexpr (MAC == COSMO_MAC_AABB) bcount = leaf ? 0 : __float_as_int(D.w);
     if (bcount > 0) {
