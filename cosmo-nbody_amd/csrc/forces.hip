@@ -2,20 +2,19 @@
 //
 // Reference: src/kernels/physics_kernels.py:14-153 (compute_forces), one
 // thread per particle with a private 64-entry stack.  Here one wavefront
-// walks the tree for 64 Morton-consecutive particles with ONE shared stack:
-// each entry carries the 64-bit mask of the lanes that must visit it.  On a
-// pop every masked lane runs its own acceptance test, accepted lanes
-// interact, and the lanes that open the node become the mask of the pushed
-// children (c2 then c1, so c1 is visited first, physics_kernels.py:144-149).
-// Each lane therefore visits exactly its own nodes in its own depth-first
-// order, so its interaction set and FP32 summation order are those of the
-// per-particle walk.  A per-lane depth counter equals the depth of the
-// reference's private stack and reproduces its overflow drop
-// (`stack_top + 1 < 64`, :143).  Masks of stacked entries are nested (an
-// entry's lanes opened every node below it), so the shared stack never
-// holds more than 64 entries and lives in three VGPRs indexed by lane
-// (v_readlane with a wave-uniform slot; a push selects one lane).  Node records are
-// wave-uniform loads (scalar cache).
+// walks the tree for 64 Morton-consecutive particles with ONE shared stack
+// of node ids, held in one VGPR indexed by lane (v_readlane pops,
+// v_writelane pushes).  On a pop every lane of the entry runs its own
+// acceptance test, accepted lanes interact, and the lanes that open the
+// node push c2 then c1 (so c1 is visited first, physics_kernels.py:144-149).
+// Which lanes belong to an entry is not stored: the entries a lane belongs
+// to are a prefix of the stack whose length is the lane's own depth (the
+// depth of the reference's private stack), so a lane is in the popped entry
+// exactly when its depth exceeds the new top.  Each lane therefore visits
+// exactly its own nodes in its own depth-first order, so its interaction set
+// and FP32 summation order are those of the per-particle walk, and the
+// depth counter reproduces the reference's overflow drop
+// (`stack_top + 1 < 64`, :143).  Node records are wave-uniform scalar loads.
 #include <algorithm>
 #include <cstring>
 
