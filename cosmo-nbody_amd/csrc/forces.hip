@@ -373,10 +373,15 @@ __global__ __launch_bounds__(256) void k_traverse(
         near_ok = d.d2 > 1e-7f;
         const float r = __builtin_fmaf(B.x, inv_theta[h], B.y);
         const float hb = p.half_box;
-        const bool inside = (d.dx + C.w < hb) && (d.dx - C.x > -hb) &&
-                            (d.dy + D.x < hb) && (d.dy - C.y > -hb) &&
-                            (d.dz + D.y < hb) && (d.dz - C.z > -hb);
-        accept = leaf || ((r * r < d.d2) && (inside || A.w < p.straddle_mu));
+        // only internal nodes at or above the straddle mass must pass the
+        // guard (a wave-uniform condition); the others skip it
+        bool inside = true;
+        if (!leaf && !(A.w < p.straddle_mu)) {
+          asm volatile("");
+          inside = (d.dx + C.w < hb) && (d.dx - C.x > -hb) && (d.dy + D.x < hb) &&
+                   (d.dy - C.y > -hb) && (d.dz + D.y < hb) && (d.dz - C.z > -hb);
+        }
+        accept = leaf || ((r * r < d.d2) && inside);
       }
       hit[h] = in[h] & accept & near_ok;
       rej[h] = in[h] & !accept;
