@@ -227,7 +227,9 @@ int cosmo_traverse(const float *pos_x, const float *pos_y, const float *pos_z,
  * walk does and walks its subtree from the stack depth the walk reaches it
  * at.  Each particle visits and accepts the same nodes as with
  * cosmo_traverse; its subtree sums are added in subtree order.  nodes: the
- * reference test's packed records.  stats (nullable, device uint64[2]):
+ * reference test's packed records (they already hold the extents clamped to
+ * the box, so box is not read; it is kept for the argument order of
+ * cosmo_traverse).  stats (nullable, device uint64[2]):
  * interactions and node visits are added.  Returns COSMO_E_TOO_LARGE for
  * larger launches. */
 int cosmo_traverse_split(cosmo_ctx *ctx, const float *pos_x, const float *pos_y,
