@@ -48,7 +48,39 @@ def parse():
                     help="learned-theta vs fixed-theta step rate at equal 1e-3 relative RMS "
                          "force error (FP64 direct sum) on the 128^3 halo state: trains the "
                          "network on three training states, evaluates on a held-out one")
+    ap.add_argument("--shards", type=int, default=8,
+                    help="also time the traversal of each of this many contiguous Morton "
+                         "slices of the targets against the whole tree (0: skip)")
     return ap.parse_args()
+
+
+def shard_slices(eng, c, parts, repeats=3):
+    """Per-slice traversal time of a step sharded over `parts` GPUs with a
+    replicated tree: the targets are cut into `parts` contiguous Morton ranges
+    (multiples of 64 particles) and each range is traversed against the whole
+    tree, as one GPU of such a step would.  HIP events on the stream the
+    traversal is launched on; the tree is the one the timed steps ended on."""
+    import torch
+
+    from src.kernels.physics_kernels import traverse
+    s = eng.state
+    n = s.n
+    cuts = [min(n, ((k * n) // parts + 63) // 64 * 64) for k in range(parts)] + [n]
+    ms = []
+    for k in range(parts):
+        off, cnt = cuts[k], cuts[k + 1] - cuts[k]
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(repeats):
+            traverse(*s.pos, eng.tree_manager.nodes, eng.tree_manager.root_idx, s.force,
+                     c.THETA, c.G, c.SOFTENING, c.BOX_SIZE, off=off, count=cnt, mac=eng.mac,
+                     pairs=eng.pairs, straddle_mass=eng.straddle_mass)
+        e1.record()
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1) / repeats)
+    mean = sum(ms) / parts
+    return {"parts": parts, "slice_ms": ms, "max_ms": max(ms), "mean_ms": mean,
+            "imbalance": max(ms) / mean}
 
 
 def learned_line(args):
@@ -200,6 +232,9 @@ def main():
     eng.compute_forces()
     inter, visits, pops = (int(x) for x in eng.stats.cpu().tolist())
     eng.stats = None
+    shards = None
+    if args.shards > 1 and eng.theta_group is None:
+        shards = shard_slices(eng, c, args.shards)
     ms = elapsed * 1e3 / args.steps
     trav_s = stage["forces"] * 1e-3
     flops = FLOP_PER_INTERACTION * inter
@@ -233,6 +268,8 @@ def main():
                      "hbm_frac": (traffic / trav_s / 1e9 / PEAK_HBM_GBS) if traffic else None,
                      "kernel": "k_traverse (FP32 VALU; 20 flop per accepted interaction)"},
     }
+    if shards is not None:
+        out["shard_slices"] = shards
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pos, vel, c, args.cpu_threads)
     print(json.dumps(out))

@@ -507,21 +507,37 @@ __global__ __launch_bounds__(256) void k_traverse(
 // the subtree sums in subtree order: the same terms, associated
 // differently.  Reference test only (packed 32-byte records per node id).
 
-// The frontier: breadth-first expansion of the top tree (one wavefront, one
-// entry per lane) until `target` subtrees, kept in depth-first order (c1
-// before c2).  Children come from the records (c1, c2 at words 6 and 7).
+// The frontier: a cut of the top tree into at most `target` subtrees (one
+// wavefront, one entry per lane), kept in depth-first order (c1 before c2).
+// Children come from the records (c1, c2 at words 6 and 7).  Each entry
+// carries its leaf range: the root of a tree built by k_build_tree is
+// internal node 0 (id n_tree) over [0, n_tree - 1], and an internal node's
+// c1 ends its range at the split index (c1's leaf id, or its internal index
+// = id - n_tree).  Subtrees are split largest-first by weight = leaves,
+// times `focus` for a subtree that holds some of the launch's own
+// particles [lo, hi): the near field of every group lies there, and its
+// walks are the longest of the launch.  Each round expands the entries
+// within a factor 2 of the heaviest one, heaviest first, so the frontier
+// takes a few dependent record loads, not one per expansion.  Any cut gives
+// the same visits and interactions per particle; the weight only shapes the
+// work per wavefront.  With another root the ranges are unknown and the
+// weight falls back to 2^-depth (breadth-first).
 __global__ __launch_bounds__(64) void k_split_frontier(const void *__restrict__ tree, int n_tree,
                                                        const int32_t *__restrict__ root_idx,
-                                                       int target,
+                                                       int target, int64_t lo, int64_t hi,
+                                                       uint64_t focus,
                                                        SplitFrontier *__restrict__ fr) {
   __shared__ int s_node[2][kSplitMaxFrontier], s_slot[2][kSplitMaxFrontier];
   __shared__ int s_plen[2][kSplitMaxFrontier];
+  __shared__ int s_rlo[2][kSplitMaxFrontier], s_rhi[2][kSplitMaxFrontier];
   __shared__ uint32_t s_lmask[2][kSplitMaxFrontier];
   __shared__ int s_path[2][kSplitMaxFrontier][kSplitMaxPath];
   __shared__ int s_pslot[2][kSplitMaxFrontier][kSplitMaxPath];
+  __shared__ uint64_t s_w[kSplitMaxFrontier];
   const int t = threadIdx.x;
   const int32_t *words = reinterpret_cast<const int32_t *>(tree);
   const int root = root_idx[0];
+  const bool ranged = root == n_tree;
   int cnt = root < 0 ? 0 : 1;
   int cur = 0;
   if (t == 0) {
@@ -529,16 +545,39 @@ __global__ __launch_bounds__(64) void k_split_frontier(const void *__restrict__ 
     s_slot[0][0] = 0;
     s_plen[0][0] = 0;
     s_lmask[0][0] = 0;
+    s_rlo[0][0] = 0;
+    s_rhi[0][0] = n_tree - 1;
   }
   __syncthreads();
-  for (int level = 0; level < kSplitMaxPath - 1 && cnt < target; ++level) {
+  for (int round = 0; round < 2 * kSplitMaxPath && cnt < target; ++round) {
     const bool have = t < cnt;
     const int node = have ? s_node[cur][t] : 0;
     const int slot = have ? s_slot[cur][t] : 0;
     const int plen = have ? s_plen[cur][t] : 0;
-    const bool expandable = have && plen == level && node >= n_tree && slot + 1 < 64;
-    const int rank = __popcll(__ballot(expandable) & lanemask_lt());
-    const bool expand = expandable && rank < target - cnt;
+    const int rlo = have ? s_rlo[cur][t] : 0, rhi = have ? s_rhi[cur][t] : 0;
+    const bool expandable =
+        have && node >= n_tree && slot + 1 < 64 && plen + 1 < kSplitMaxPath;
+    uint64_t w = 0;
+    if (expandable) {
+      if (ranged) {
+        w = static_cast<uint64_t>(rhi >= rlo ? rhi - rlo + 1 : 1);
+        if (rlo < hi && rhi >= lo) w *= focus;
+      } else {
+        w = uint64_t(1) << (kSplitMaxPath - plen);
+      }
+    }
+    s_w[t] = w;
+    __syncthreads();
+    uint64_t wmax = 0;
+    int rank = 0;  // heavier entries, ties broken by position
+    for (int u = 0; u < kSplitMaxFrontier; ++u) {
+      const uint64_t wu = s_w[u];
+      wmax = wu > wmax ? wu : wmax;
+      rank += (wu > w || (wu == w && u < t)) ? 1 : 0;
+    }
+    __syncthreads();
+    if (wmax == 0) break;
+    const bool expand = w != 0 && 2 * w > wmax && rank < target - cnt;
     const uint64_t xm = __ballot(expand);
     if (xm == 0) break;
     const int pos = t + __popcll(xm & lanemask_lt());
@@ -553,21 +592,29 @@ __global__ __launch_bounds__(64) void k_split_frontier(const void *__restrict__ 
       }
       if (expand) {
         const int c1 = words[8 * int64_t(node) + 6], c2 = words[8 * int64_t(node) + 7];
+        const int split = c1 < n_tree ? c1 : c1 - n_tree;
+        const int mid = min(max(split, rlo), rhi - 1);
         s_path[nxt][pos][plen] = s_path[nxt][pos + 1][plen] = node;
         s_pslot[nxt][pos][plen] = s_pslot[nxt][pos + 1][plen] = slot;
         s_node[nxt][pos] = c1;
         s_slot[nxt][pos] = slot + 1;
         s_plen[nxt][pos] = plen + 1;
         s_lmask[nxt][pos] = s_lmask[cur][t] | (1u << plen);
+        s_rlo[nxt][pos] = rlo;
+        s_rhi[nxt][pos] = mid;
         s_node[nxt][pos + 1] = c2;
         s_slot[nxt][pos + 1] = slot;
         s_plen[nxt][pos + 1] = plen + 1;
         s_lmask[nxt][pos + 1] = 0u;
+        s_rlo[nxt][pos + 1] = mid + 1;
+        s_rhi[nxt][pos + 1] = rhi;
       } else {
         s_node[nxt][pos] = node;
         s_slot[nxt][pos] = slot;
         s_plen[nxt][pos] = plen;
         s_lmask[nxt][pos] = s_lmask[cur][t];
+        s_rlo[nxt][pos] = rlo;
+        s_rhi[nxt][pos] = rhi;
       }
     }
     cnt += __popcll(xm);
@@ -1018,12 +1065,18 @@ int cosmo_traverse_split(cosmo_ctx *ctx, const float *px, const float *py, const
   const float sf = static_cast<float>(soft);
   p.eps2 = sf * sf;
   const int64_t groups = (hi - lo + kWave - 1) / kWave;
-  // enough wavefronts to occupy the device: about 4096 in all
+  // about 8192 wavefronts in all (at most kSplitMaxFrontier per group)
   const int F = static_cast<int>(std::max<int64_t>(
-      1, std::min<int64_t>(kSplitMaxFrontier, 4096 / groups)));
+      1, std::min<int64_t>(kSplitMaxFrontier, 8192 / groups)));
   hipStream_t s = as_stream(stream);
   SplitFrontier *fr = reinterpret_cast<SplitFrontier *>(ctx->split_frontier);
-  k_split_frontier<<<1, kWave, 0, s>>>(nodes, static_cast<int>(n_tree), root_idx, F, fr);
+  // weight of the subtrees that hold the launch's particles: half the
+  // number of launches of this size that would cover the tree, at least 8
+  // (the reference harness's 5000-particle batches at 250k-2M particles
+  // and theta 0.5-0.7 run fastest near it)
+  const uint64_t focus = static_cast<uint64_t>(std::max<int64_t>(8, n_tree / (hi - lo) / 2));
+  k_split_frontier<<<1, kWave, 0, s>>>(nodes, static_cast<int>(n_tree), root_idx, F, lo, hi,
+                                        focus, fr);
   const unsigned grid = blocks_for(groups * F * kWave, 256);
   if (stats)
     k_traverse_split<true><<<grid, 256, 0, s>>>(px, py, pz, nodes, fr, F, ctx->split_partial, p,
