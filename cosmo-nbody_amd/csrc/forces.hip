@@ -517,44 +517,40 @@ __global__ __launch_bounds__(256) void k_traverse(
 // times `focus` for a subtree that holds some of the launch's own
 // particles [lo, hi): the near field of every group lies there, and its
 // walks are the longest of the launch.  Each round expands the entries
-// within a factor 2 of the heaviest one, heaviest first, so the frontier
-// takes a few dependent record loads, not one per expansion.  Any cut gives
-// the same visits and interactions per particle; the weight only shapes the
-// work per wavefront.  With another root the ranges are unknown and the
-// weight falls back to 2^-depth (breadth-first).
+// within a factor 2 of the heaviest one (heaviest first when they exceed
+// the budget), so the frontier takes a few dependent record loads, not one
+// per expansion.  Any cut gives the same visits and interactions per
+// particle; the weight only shapes the work per wavefront.  With another
+// root the ranges are unknown and the weight falls back to 2^-depth
+// (breadth-first).
+//
+// Expanded nodes go to an append-only history (node, stack slot, history
+// index of its parent, whether it is its parent's c1); an entry keeps the
+// history index of its parent and whether it is a c1.  Paths and leftmost
+// masks are read back from the history once, at the end.
 __global__ __launch_bounds__(64) void k_split_frontier(const void *__restrict__ tree, int n_tree,
                                                        const int32_t *__restrict__ root_idx,
                                                        int target, int64_t lo, int64_t hi,
                                                        uint64_t focus,
                                                        SplitFrontier *__restrict__ fr) {
+  constexpr int kHist = kSplitMaxFrontier;  // at most target - 1 expansions
   __shared__ int s_node[2][kSplitMaxFrontier], s_slot[2][kSplitMaxFrontier];
-  __shared__ int s_plen[2][kSplitMaxFrontier];
+  __shared__ int s_plen[2][kSplitMaxFrontier], s_hpar[2][kSplitMaxFrontier];
+  __shared__ int s_left[2][kSplitMaxFrontier];
   __shared__ int s_rlo[2][kSplitMaxFrontier], s_rhi[2][kSplitMaxFrontier];
-  __shared__ uint32_t s_lmask[2][kSplitMaxFrontier];
-  __shared__ int s_path[2][kSplitMaxFrontier][kSplitMaxPath];
-  __shared__ int s_pslot[2][kSplitMaxFrontier][kSplitMaxPath];
+  __shared__ int h_node[kHist], h_slot[kHist], h_par[kHist], h_left[kHist];
   __shared__ uint64_t s_w[kSplitMaxFrontier];
   const int t = threadIdx.x;
   const int32_t *words = reinterpret_cast<const int32_t *>(tree);
   const int root = root_idx[0];
   const bool ranged = root == n_tree;
   int cnt = root < 0 ? 0 : 1;
+  int hcnt = 0;
   int cur = 0;
-  if (t == 0) {
-    s_node[0][0] = root;
-    s_slot[0][0] = 0;
-    s_plen[0][0] = 0;
-    s_lmask[0][0] = 0;
-    s_rlo[0][0] = 0;
-    s_rhi[0][0] = n_tree - 1;
-  }
-  __syncthreads();
+  // this lane's entry
+  int node = root, slot = 0, plen = 0, hpar = -1, left = 0, rlo = 0, rhi = n_tree - 1;
   for (int round = 0; round < 2 * kSplitMaxPath && cnt < target; ++round) {
     const bool have = t < cnt;
-    const int node = have ? s_node[cur][t] : 0;
-    const int slot = have ? s_slot[cur][t] : 0;
-    const int plen = have ? s_plen[cur][t] : 0;
-    const int rlo = have ? s_rlo[cur][t] : 0, rhi = have ? s_rhi[cur][t] : 0;
     const bool expandable =
         have && node >= n_tree && slot + 1 < 64 && plen + 1 < kSplitMaxPath;
     uint64_t w = 0;
@@ -566,70 +562,95 @@ __global__ __launch_bounds__(64) void k_split_frontier(const void *__restrict__ 
         w = uint64_t(1) << (kSplitMaxPath - plen);
       }
     }
-    s_w[t] = w;
-    __syncthreads();
-    uint64_t wmax = 0;
-    int rank = 0;  // heavier entries, ties broken by position
-    for (int u = 0; u < kSplitMaxFrontier; ++u) {
-      const uint64_t wu = s_w[u];
-      wmax = wu > wmax ? wu : wmax;
-      rank += (wu > w || (wu == w && u < t)) ? 1 : 0;
+    uint64_t wmax = w;
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1) {
+      const uint64_t o = __shfl_xor(wmax, m);
+      wmax = o > wmax ? o : wmax;
     }
-    __syncthreads();
     if (wmax == 0) break;
-    const bool expand = w != 0 && 2 * w > wmax && rank < target - cnt;
+    bool expand = w != 0 && 2 * w > wmax;
+    const int budget = target - cnt;
+    if (__popcll(__ballot(expand)) > budget) {  // heaviest first, ties by position
+      s_w[t] = expand ? w : 0;
+      __syncthreads();
+      int rank = 0;
+      for (int u = 0; u < kWave; ++u) {
+        const uint64_t wu = s_w[u];
+        rank += (wu > w || (wu == w && u < t)) ? 1 : 0;
+      }
+      __syncthreads();
+      expand = expand && rank < budget;
+    }
     const uint64_t xm = __ballot(expand);
-    if (xm == 0) break;
-    const int pos = t + __popcll(xm & lanemask_lt());
+    const int below = __popcll(xm & lanemask_lt());
+    const int pos = t + below;
     const int nxt = cur ^ 1;
     if (have) {
-      const int outs = expand ? 2 : 1;
-      for (int o = 0; o < outs; ++o) {
-        for (int k = 0; k < plen; ++k) {
-          s_path[nxt][pos + o][k] = s_path[cur][t][k];
-          s_pslot[nxt][pos + o][k] = s_pslot[cur][t][k];
-        }
-      }
       if (expand) {
+        const int h = hcnt + below;
+        h_node[h] = node;
+        h_slot[h] = slot;
+        h_par[h] = hpar;
+        h_left[h] = left;
         const int c1 = words[8 * int64_t(node) + 6], c2 = words[8 * int64_t(node) + 7];
         const int split = c1 < n_tree ? c1 : c1 - n_tree;
         const int mid = min(max(split, rlo), rhi - 1);
-        s_path[nxt][pos][plen] = s_path[nxt][pos + 1][plen] = node;
-        s_pslot[nxt][pos][plen] = s_pslot[nxt][pos + 1][plen] = slot;
         s_node[nxt][pos] = c1;
         s_slot[nxt][pos] = slot + 1;
         s_plen[nxt][pos] = plen + 1;
-        s_lmask[nxt][pos] = s_lmask[cur][t] | (1u << plen);
+        s_hpar[nxt][pos] = h;
+        s_left[nxt][pos] = 1;
         s_rlo[nxt][pos] = rlo;
         s_rhi[nxt][pos] = mid;
         s_node[nxt][pos + 1] = c2;
         s_slot[nxt][pos + 1] = slot;
         s_plen[nxt][pos + 1] = plen + 1;
-        s_lmask[nxt][pos + 1] = 0u;
+        s_hpar[nxt][pos + 1] = h;
+        s_left[nxt][pos + 1] = 0;
         s_rlo[nxt][pos + 1] = mid + 1;
         s_rhi[nxt][pos + 1] = rhi;
       } else {
         s_node[nxt][pos] = node;
         s_slot[nxt][pos] = slot;
         s_plen[nxt][pos] = plen;
-        s_lmask[nxt][pos] = s_lmask[cur][t];
+        s_hpar[nxt][pos] = hpar;
+        s_left[nxt][pos] = left;
         s_rlo[nxt][pos] = rlo;
         s_rhi[nxt][pos] = rhi;
       }
     }
+    hcnt += __popcll(xm);
     cnt += __popcll(xm);
     cur = nxt;
     __syncthreads();
+    if (t < cnt) {
+      node = s_node[cur][t];
+      slot = s_slot[cur][t];
+      plen = s_plen[cur][t];
+      hpar = s_hpar[cur][t];
+      left = s_left[cur][t];
+      rlo = s_rlo[cur][t];
+      rhi = s_rhi[cur][t];
+    }
   }
   if (t < cnt) {
-    fr->node[t] = s_node[cur][t];
-    fr->slot[t] = s_slot[cur][t];
-    fr->plen[t] = s_plen[cur][t];
-    fr->lmask[t] = s_lmask[cur][t];
-    for (int k = 0; k < s_plen[cur][t]; ++k) {
-      fr->path[t][k] = s_path[cur][t][k];
-      fr->pslot[t][k] = s_pslot[cur][t][k];
+    fr->node[t] = node;
+    fr->slot[t] = slot;
+    fr->plen[t] = plen;
+    // walk up the history: path[k] is the ancestor at depth k; the entry
+    // is leftmost under path[k] while every step up from it is a c1
+    uint32_t lmask = 0;
+    bool chain = left != 0;
+    int h = hpar;
+    for (int k = plen - 1; k >= 0; --k) {
+      fr->path[t][k] = h_node[h];
+      fr->pslot[t][k] = h_slot[h];
+      if (chain) lmask |= 1u << k;
+      chain = chain && h_left[h] != 0;
+      h = h_par[h];
     }
+    fr->lmask[t] = lmask;
   }
   if (t == 0) fr->count = cnt;
 }
