@@ -787,20 +787,24 @@ __global__ __launch_bounds__(256) void k_split_reduce(const float *__restrict__ 
                                                       const SplitFrontier *__restrict__ fr,
                                                       int64_t lo, int64_t hi, float G,
                                                       float *__restrict__ force) {
-  const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t m = hi - lo;
-  if (j >= m) return;
+  // one thread per force component (particle j, axis c at e = 3 j + c):
+  // consecutive threads read consecutive words of every subtree's block
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t m3 = (hi - lo) * 3;
+  if (e >= m3) return;
   const int nf = fr->count;
-  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-  for (int q = 0; q < nf; ++q) {
-    const float *o = partial + (int64_t(q) * m + j) * 3;
-    sx += o[0];
-    sy += o[1];
-    sz += o[2];
+  const float *o = partial + e;
+  float s = 0.0f;
+  int q = 0;
+  for (; q + 8 <= nf; q += 8) {  // eight loads in flight, added in subtree order
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = o[int64_t(q + u) * m3];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
   }
-  force[3 * (lo + j) + 0] = G * sx;
-  force[3 * (lo + j) + 1] = G * sy;
-  force[3 * (lo + j) + 2] = G * sz;
+  for (; q < nf; ++q) s += o[int64_t(q) * m3];
+  force[3 * lo + e] = G * s;
 }
 
 // FP64 direct sum: a block of 1024 threads takes 64 targets (one per lane);
@@ -1105,7 +1109,7 @@ int cosmo_traverse_split(cosmo_ctx *ctx, const float *px, const float *py, const
   else
     k_traverse_split<false><<<grid, 256, 0, s>>>(px, py, pz, nodes, fr, F, ctx->split_partial, p,
                                                  lo, hi, nullptr);
-  k_split_reduce<<<blocks_for(hi - lo, 256), 256, 0, s>>>(ctx->split_partial, fr, lo, hi, p.G,
+  k_split_reduce<<<blocks_for(3 * (hi - lo), 256), 256, 0, s>>>(ctx->split_partial, fr, lo, hi, p.G,
                                                           force);
   COSMO_CHECK_LAUNCH();
   return COSMO_OK;
