@@ -57,7 +57,7 @@ constexpr int kRadix = 1 << kRadixBits;
 // wavefront group, top-tree paths of at most kSplitMaxPath nodes, and at
 // most kSplitMaxCount particles per launch.
 constexpr int kSplitMaxFrontier = 64;
-constexpr int kSplitMaxPath = 16;
+constexpr int kSplitMaxPath = 24;
 constexpr int64_t kSplitMaxCount = 65536;
 
 struct SplitFrontier {

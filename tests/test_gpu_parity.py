@@ -327,7 +327,7 @@ def test_traverse_split_stack_overflow(orc, torch_mod):
     chain of left (c1) edges ending in the pair, whose parent sits at stack
     slot 63: at theta = 0 the walk drops the pair there
     (physics_kernels.py:143).  The split walk, whose top tree covers only
-    the first 16 levels, drops the same interactions (the oracle's counts)."""
+    the first 24 levels, drops the same interactions (the oracle's counts)."""
     from test_oracle_stages import _code_chain
     _split_case(orc, torch_mod, _code_chain(63), 0.0, 1e-3, [(0, 65), (5, 40)])
 
