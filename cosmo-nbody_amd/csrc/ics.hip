@@ -14,6 +14,9 @@
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
 
 #include "common.h"
 
@@ -140,6 +143,28 @@ __global__ __launch_bounds__(256) void k_za_finish(const double *__restrict__ di
 
 constexpr int kSumBlocks = 1024;
 
+// One double-complex 3-D plan per (device, grid size), created on first use
+// and kept until the process ends: later calls allocate nothing.  The plan
+// is bound to the caller's stream on every call.
+static bool cached_plan(int64_t n, hipfftHandle *out) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int64_t>, hipfftHandle> plans;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return false;
+  std::lock_guard<std::mutex> lock(mu);
+  const auto key = std::make_pair(dev, n);
+  auto it = plans.find(key);
+  if (it == plans.end()) {
+    hipfftHandle plan;
+    if (hipfftPlan3d(&plan, static_cast<int>(n), static_cast<int>(n), static_cast<int>(n),
+                     HIPFFT_Z2Z) != HIPFFT_SUCCESS)
+      return false;
+    it = plans.emplace(key, plan).first;
+  }
+  *out = it->second;
+  return true;
+}
+
 }  // namespace cosmo
 
 using namespace cosmo;
@@ -168,9 +193,7 @@ int cosmo_zeldovich_ics(int64_t n_grid, const double *phase, double amplitude,
   double *mean = partial + kSumBlocks;
   double *var = mean + 1;
   hipfftHandle plan;
-  if (hipfftPlan3d(&plan, static_cast<int>(n), static_cast<int>(n), static_cast<int>(n),
-                   HIPFFT_Z2Z) != HIPFFT_SUCCESS)
-    return COSMO_E_FFT;
+  if (!cached_plan(n, &plan)) return COSMO_E_FFT;
   int rc = COSMO_OK;
   if (hipfftSetStream(plan, s) != HIPFFT_SUCCESS) rc = COSMO_E_FFT;
   const unsigned g = blocks_for(n3, 256);
@@ -185,7 +208,6 @@ int cosmo_zeldovich_ics(int64_t n_grid, const double *phase, double amplitude,
     k_real_part<<<g, 256, 0, s>>>(buf, disp + c * n3, n3, inv);
     if (hipGetLastError() != hipSuccess) rc = COSMO_E_HIP;
   }
-  hipfftDestroy(plan);
   if (rc != COSMO_OK) return rc;
   // numpy.std(d_x) (ddof 0): mean, then the mean square of the deviations
   const int parts = static_cast<int>(std::min<int64_t>(kSumBlocks, blocks_for(n3, 256)));

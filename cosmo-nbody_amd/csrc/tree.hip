@@ -193,6 +193,11 @@ __device__ void climb_device(const int32_t *__restrict__ children,
     wait_stores();
     const int old = __hip_atomic_fetch_add(&counts[p], 1, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
+    // the sibling's moments may only be read after the counter update has
+    // returned: no load below may be moved above it by the compiler (the
+    // hardware issues them after the branch on `old`, which waits for the
+    // returned value)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
     if (old == 0) break;
     const int2 ch = reinterpret_cast<const int2 *>(children)[p];
     const bool mine_left = (ch.x == curr);

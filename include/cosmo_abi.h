@@ -6,8 +6,13 @@
  * allocator (torch tensors' data_ptr(), hipMalloc), sizes are int64_t and
  * `stream` is a hipStream_t passed as void* (NULL = the default stream).
  * Every call only enqueues work on `stream` and returns a status:
- * COSMO_OK, or an error whose text cosmo_status_string() gives.  Nothing is
- * allocated per call except by cosmo_ctx_create().
+ * COSMO_OK, or an error whose text cosmo_status_string() gives.  Device
+ * memory is allocated only by cosmo_ctx_create() and by the first
+ * cosmo_zeldovich_ics() call per device and grid size (its hipFFT plan,
+ * kept until the process ends); every other call allocates nothing.
+ * Calls that share a cosmo_ctx must be issued on one stream (the context's
+ * sort scratch, packed records and split-walk buffers are reused by every
+ * call); use one context per stream for concurrent work.
  *
  * Each entry point names the reference (Cosmo-NBody) function it replaces.
  * The reference launches Numba kernels as kern[blocks, threads](args) on

@@ -628,10 +628,14 @@ void orc_compute_forces(const float *px, const float *py, const float *pz,
                         double G, double soft, double box, int32_t root,
                         int64_t off, int64_t count, int64_t n, int precision,
                         int mac, const float *theta_g, int bucket,
-                        double straddle_mass, int64_t *n_inter, int64_t *n_visit,
-                        double *force64) {
+                        double straddle_mass, const int64_t *targets,
+                        int64_t *n_inter, int64_t *n_visit, double *force64) {
   int64_t hi = off + count;
   if (hi > n) hi = n;
+  if (targets) { /* the listed particles; output row t is targets[t] */
+    off = 0;
+    hi = count;
+  }
   int32_t *bfirst = NULL, *bcount = NULL;
   if (mac != ORC_MAC_AABB) bucket = 0;
   if (bucket > 0) {
@@ -640,7 +644,8 @@ void orc_compute_forces(const float *px, const float *py, const float *pz,
     if (leaf_spans(children, n, root, bfirst, bcount) != 0) bucket = 0;
   }
 #pragma omp parallel for schedule(dynamic, 64)
-  for (int64_t i = off; i < hi; ++i) {
+  for (int64_t t = off; t < hi; ++t) {
+    const int64_t i = targets ? targets[t] : t;
     double a[3];
     int64_t ni = 0, nv = 0;
     if (precision == 64) {
@@ -656,16 +661,16 @@ void orc_compute_forces(const float *px, const float *py, const float *pz,
                  precision == 31, bfirst, bcount, bucket, (float)straddle_mass,
                  a, &ni, &nv);
     }
-    force[3 * i + 0] = (float)a[0];
-    force[3 * i + 1] = (float)a[1];
-    force[3 * i + 2] = (float)a[2];
+    force[3 * t + 0] = (float)a[0];
+    force[3 * t + 1] = (float)a[1];
+    force[3 * t + 2] = (float)a[2];
     if (force64) {
-      force64[3 * i + 0] = a[0];
-      force64[3 * i + 1] = a[1];
-      force64[3 * i + 2] = a[2];
+      force64[3 * t + 0] = a[0];
+      force64[3 * t + 1] = a[1];
+      force64[3 * t + 2] = a[2];
     }
-    if (n_inter) n_inter[i] = ni;
-    if (n_visit) n_visit[i] = nv;
+    if (n_inter) n_inter[t] = ni;
+    if (n_visit) n_visit[t] = nv;
   }
   free(bfirst);
   free(bcount);

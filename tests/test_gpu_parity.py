@@ -628,32 +628,20 @@ def test_colorize_frame(orc, torch_mod, ics32, res):
 
 
 def test_engine_steps_match_oracle(orc, torch_mod):
-    """Five steps of small_32 on the GPU against the same five steps done
-    stage by stage with the oracle (bit-exact sort/tree/moments/update for
-    identical forces; forces within 1e-5, so positions are compared with a
-    tolerance after the first step)."""
-    from src.utils.cosmology import get_hubble_param
-    eng, c, pos, vel = _run_engine("small_32", 32, 1)
-    st = eng.state
-    n = len(pos[0])
-    codes, _ = orc.compute_codes(*pos)
-    perm = orc.argsort_stable(codes)
-    p = [x[perm] for x in pos]
-    v = [x[perm] for x in vel]
-    mass = np.full(n, 1.0 / n, np.float32)
-    ch, par = orc.build_tree(codes[perm])
-    nm, nc, lo, hi = orc.compute_multipoles(*p, mass, ch, par)
-    com = orc.normalize_com(nm, nc)
-    assert np.array_equal(host(eng.tree_manager.child), ch)
-    assert np.array_equal(host(eng.tree_manager.com), com)
-    f64 = orc.compute_forces(*p, ch, nm, com, lo, hi, c.THETA, c.G, c.SOFTENING, 1.0, n)
-    assert rel_rms(host(st.force), f64) <= 1e-5
-    a0 = c.A_START
-    H = get_hubble_param(a0, c)
-    dt = c.DA / (a0 * H) * 0.8
-    ref = orc.integrate(*p, *v, host(st.force), a0, H, dt)
-    for t, r in zip(st.pos + st.vel, ref):
-        assert np.array_equal(host(t), r)
+    """C0: the five steps of small_32 (N = 32^3) on the GPU, each stage
+    compared with the oracle on the same input (tests/step_check.py):
+    codes, permutation, tree, FP64 moments, kick-drift and density grid bit
+    for bit, accelerations of every particle within 1e-5 relative RMS of the
+    reference method's FP64 arithmetic."""
+    from src.core.engine import SimulationEngine
+    from src.core.state import SimulationState
+    from src.utils import config as cfg
+    from step_check import check_steps
+    c = cfg.preset("small_32")
+    pos, vel = zeldovich(32, c.SEED)
+    eng = SimulationEngine(c, SimulationState(len(pos[0]), pos, vel))
+    errs = check_steps(orc, eng, c.RUN_STEPS)
+    assert len(errs) == 5
 
 
 def test_engine_deterministic(torch_mod):

@@ -473,3 +473,65 @@ def test_colorize_palette_known_answers(orc):
     assert abs(frame[1, 1, 2] - min(1.0, 0.8 * 1.3)) < 1e-3 and frame[1, 1, 0] == 0
     assert rgb.dtype == np.uint8 and rgb[0, 0, 0] == 255
 
+
+
+# ------------------------------------------- hand-derived known answers
+
+@pytest.mark.parametrize("name", ["x_extent", "clamp", "no_clamp"])
+def test_reference_mac_known_answers(orc, name):
+    """The reference test on hand-built three-particle trees
+    (physics_kernels.py:128-136, derivations in kat_cases.py): the x-extent
+    alone decides (a node thin in x but tall in y is accepted where a
+    largest-extent test would open it), and an x-extent above box/2 is
+    replaced by the box, so such a node is always opened for theta <= 1,
+    while an x-extent just below box/2 is kept.  Visits, interactions and
+    T's acceleration are compared with the hand-derived values."""
+    import kat_cases as K
+    case = K.FORCE_CASES[name]
+    pos, mass, ch, par = K.tree_arrays(case["points"], case["masses"])
+    nm, nc, lo, hi = orc.compute_multipoles(*pos, mass, ch, par)
+    com = orc.normalize_com(nm, nc)
+    expect = K.expected_force(case)
+    other = K.opened_force(case) if case["interactions"] == 1 else None
+    for prec, tol in ((64, 1e-12), (32, 2e-6)):
+        f, ni, nv, f64 = orc.compute_forces(*pos, ch, nm, com, lo, hi, case["theta"], K.G,
+                                            K.SOFT, 1.0, 3, off=0, count=1, precision=prec,
+                                            counts=True, want_f64=True)
+        assert int(ni[0]) == case["interactions"]
+        assert int(nv[0]) == case["visits"]
+        got = f64[0] if prec == 64 else f[0].astype(np.float64)
+        assert np.linalg.norm(got - expect) <= tol * np.linalg.norm(expect)
+        if other is not None:  # an opened node would be far from the answer
+            assert np.linalg.norm(other - expect) > 1e-3 * np.linalg.norm(expect)
+
+
+def test_render_orientation_known_answers(orc):
+    """Grid rows and columns against the pinhole projection of the
+    reference's camera derived in FP64 (kat_cases.pinhole_pixel): a point
+    left of the centre on screen lands at a column < res/2 in the centre's
+    row, a point above it at a row < res/2 in the centre's column; a
+    transposed or flipped grid fails."""
+    import kat_cases as K
+    from src.utils import config as cfg
+    from src.utils.visualizer import get_mvp_matrix
+    c = cfg.preset("repo_128")
+    for res in (256, 1024):
+        mvp = get_mvp_matrix(0, (0.5, 0.5, 0.5), 1.0, c)
+        centre = orc.render_density(*[np.array([0.5], np.float32)] * 3, mvp, res)
+        (r0,), (c0,) = np.nonzero(centre)
+        for name, p in K.SPLAT_POINTS.items():
+            row, col, fr, fc = K.pinhole_pixel(p, res)
+            g = orc.render_density(*[np.array([v], np.float32) for v in p], mvp, res)
+            ys, xs = np.nonzero(g)
+            assert len(ys) == 1
+            # away from pixel borders the FP32 pipeline must give the FP64 pixel
+            if 1e-3 < fr < 1 - 1e-3:
+                assert int(ys[0]) == row, (name, res)
+            if 1e-3 < fc < 1 - 1e-3:
+                assert int(xs[0]) == col, (name, res)
+            if name == "left":
+                assert xs[0] < res // 2 - 10 and ys[0] == r0
+            if name == "up":
+                assert ys[0] < res // 2 - 10 and xs[0] == c0
+            if name == "right_down":
+                assert xs[0] > res // 2 + 5 and ys[0] > res // 2 + 5
