@@ -5,7 +5,11 @@ eps = 0.02/128, reference time schedule; one step = Morton encode + radix
 sort + permute, tree build, FP64 upward pass + node packing, Barnes-Hut
 traversal (reference acceptance test, FP32), kick-drift update, and the
 1024^2 fixed-point density splat.  W untimed steps, then K timed steps
-between device synchronisations.  Prints one JSON line.
+between device synchronisations.  Prints one JSON line.  It also carries
+the learned-theta half of the metric ("learned_vs_fixed"): the full-step
+rate of the learned per-group opening angle against the fixed opening
+angle at equal 1e-3 relative RMS force error (FP64 direct sum), with the
+network trained on the GPU in the same run.
 """
 import argparse
 import json
@@ -18,7 +22,10 @@ for p in (ROOT, os.path.join(ROOT, "cosmo-nbody_amd")):
     if p not in sys.path:
         sys.path.insert(0, p)
 
-PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 (vector, packed) / dense MFMA FP32 figure
+PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 published peak (packed FP32 / dense MFMA FP32)
+# FP32 VALU without packed instructions (256 CUs x 64 lanes x 2 flop per FMA x
+# 2.4 GHz): the ceiling of the traversal, which is built without packed FP32
+PEAK_VALU_FP32_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0         # MI355X HBM3E
 FLOP_PER_INTERACTION = 20     # standard particle-node interaction count
 # HBM bytes per traversal launch measured with rocprofv3 --pmc FETCH_SIZE and
@@ -48,6 +55,8 @@ def parse():
                     help="learned-theta vs fixed-theta step rate at equal 1e-3 relative RMS "
                          "force error (FP64 direct sum) on the 128^3 halo state: trains the "
                          "network on three training states, evaluates on a held-out one")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-theta vs fixed-theta comparison of the default line")
     ap.add_argument("--shards", type=int, default=8,
                     help="also time the traversal of each of this many contiguous Morton "
                          "slices of the targets against the whole tree (0: skip)")
@@ -81,6 +90,42 @@ def shard_slices(eng, c, parts, repeats=3):
     mean = sum(ms) / parts
     return {"parts": parts, "slice_ms": ms, "max_ms": max(ms), "mean_ms": mean,
             "imbalance": max(ms) / mean}
+
+
+def learned_summary(log):
+    """The learned-theta half of the metric: network trained on the GPU from
+    three training halo states (128^3), judged on a held-out one: the fixed
+    theta meeting 1e-3 relative RMS error (FP64 direct sum) against the
+    learned per-group theta at the same error, both errors confirmed on a
+    disjoint sample of groups, both full-step times."""
+    from src.core import learned_theta as L
+    from src.utils import config as cfg
+    t0 = time.perf_counter()
+    c = cfg.preset("halos_128")
+    out = L.experiment("halos_128", 1e-3, "aabb", n_groups=10240, log=log)
+    res = out[0][0]
+    wall = time.perf_counter() - t0
+    if not res.get("reachable"):
+        return {"reachable": False, "wall_s": wall, "detail": res}
+    return {
+        "ratio": res["speedup_step"], "target_rel_rms": res["target"],
+        "held_out_state": f"halos_128 seed {res['val_state'][0]} (half of N={res['n']} in 64 "
+                          f"Plummer spheres, half uniform; training seeds "
+                          f"{[s for s, _ in res['train_states']]})",
+        "test": f"AABB, bucket {c.BUCKET}, compensated sums, straddle mass {c.STRADDLE_MASS}",
+        "theta_fixed": res["theta_fixed"], "lambda": res["lambda"],
+        "err_fixed": res["err_fixed"], "err_learned": res["err_learned"],
+        "tuning_groups": res["val_groups"],
+        "err_fixed_confirm": res["err_fixed_confirm"],
+        "err_learned_confirm": res["err_learned_confirm"],
+        "confirm_groups": res["confirm_groups"],
+        "step_ms_fixed": res["step_ms_fixed"], "step_ms_learned": res["step_ms_learned"],
+        "traversal_ms_fixed": res["traversal_ms_fixed"],
+        "traversal_ms_learned": res["traversal_ms_learned"], "select_ms": res["select_ms"],
+        "hindsight_bound": res["hindsight_bound_speedup"],
+        "interactions_fixed": res["inter_fixed"], "interactions_learned": res["inter_learned"],
+        "train_rows": res["train_rows"], "fit_s": res["fit_s"], "wall_s": wall,
+    }
 
 
 def learned_line(args):
@@ -262,17 +307,25 @@ def main():
         "visits_per_particle": visits / n,
         "wave_pops_per_particle": pops / n,
         "traversal": "pairs (128 particles per wavefront)" if eng.pairs else "64 per wavefront",
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_VALU_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_VALU_FP32_TFLOPS,
+                     "peak_fp32_published": PEAK_FP32_TFLOPS,
+                     "frac_of_published": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic,
                      "hbm_frac": (traffic / trav_s / 1e9 / PEAK_HBM_GBS) if traffic else None,
-                     "kernel": "k_traverse (FP32 VALU; 20 flop per accepted interaction)"},
+                     "kernel": "k_traverse (FP32 VALU, no packed FP32; 20 flop per accepted "
+                               "interaction)"},
     }
     if shards is not None:
         out["shard_slices"] = shards
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pos, vel, c, args.cpu_threads)
-    print(json.dumps(out))
+    if not args.no_learned and c.NAME == "repo_128":
+        del eng, st
+        torch.cuda.empty_cache()
+        out["learned_vs_fixed"] = learned_summary(
+            lambda m: print(f"[learned] {m}", file=sys.stderr, flush=True))
+    print(json.dumps(out, default=float))
 
 
 if __name__ == "__main__":

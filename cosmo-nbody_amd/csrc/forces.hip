@@ -1037,6 +1037,21 @@ int cosmo_order_groups(const float *theta_group, int64_t groups, int32_t *order,
   return COSMO_OK;
 }
 
+// The walk of compute_forces over the reference-test records in ctx->nodes:
+// a launch of at most 1024 groups cannot fill the device with one wavefront
+// per group, so it splits each group's walk over the top tree.
+static int forces_from_ctx_records(cosmo_ctx *ctx, const float *px, const float *py,
+                                   const float *pz, float *force, double theta, double G,
+                                   double soft, double box, const int32_t *root_idx,
+                                   int64_t off, int64_t count, int64_t n_bodies, void *stream) {
+  const int64_t todo = std::min(off + count, n_bodies) - off;
+  if (todo > 0 && todo <= std::min<int64_t>(ctx->n_max, 1024 * kWave))
+    return cosmo_traverse_split(ctx, px, py, pz, ctx->nodes, n_bodies, root_idx, force, theta,
+                                G, soft, box, off, count, n_bodies, nullptr, stream);
+  return cosmo_traverse(px, py, pz, ctx->nodes, n_bodies, root_idx, force, theta, nullptr, G,
+                        soft, box, off, count, n_bodies, COSMO_MAC_REF, 0, nullptr, stream);
+}
+
 int cosmo_compute_forces(cosmo_ctx *ctx, const float *px, const float *py, const float *pz,
                          const int32_t *children, const double *n_mass, const double *n_com,
                          const float *n_min, const float *n_max, float *force, double theta,
@@ -1045,20 +1060,44 @@ int cosmo_compute_forces(cosmo_ctx *ctx, const float *px, const float *py, const
                          int64_t count, int64_t n_bodies, void *stream) {
   (void)top_nodes;
   (void)node_to_cache;
-  if (!ctx) return COSMO_E_INVALID;
+  if (!ctx || off < 0 || count < 0) return COSMO_E_INVALID;
   if (n_bodies > ctx->n_max) return COSMO_E_TOO_LARGE;
   if (n_bodies < 1) return COSMO_OK;
   const int rc = cosmo_pack_nodes(children, n_mass, n_com, n_min, n_max, 2 * n_bodies - 1, box,
                                   COSMO_MAC_REF, ctx->nodes, stream);
   if (rc != COSMO_OK) return rc;
-  // a launch of at most 1024 groups cannot fill the device with one
-  // wavefront per group: split each group's walk over the top tree
-  const int64_t todo = std::min(off + count, n_bodies) - off;
-  if (todo > 0 && todo <= std::min<int64_t>(ctx->n_max, 1024 * kWave))
-    return cosmo_traverse_split(ctx, px, py, pz, ctx->nodes, n_bodies, root_idx, force, theta,
-                                G, soft, box, off, count, n_bodies, nullptr, stream);
-  return cosmo_traverse(px, py, pz, ctx->nodes, n_bodies, root_idx, force, theta, nullptr, G,
-                        soft, box, off, count, n_bodies, COSMO_MAC_REF, 0, nullptr, stream);
+  return forces_from_ctx_records(ctx, px, py, pz, force, theta, G, soft, box, root_idx, off,
+                                 count, n_bodies, stream);
+}
+
+int cosmo_compute_forces_f32(cosmo_ctx *ctx, const float *px, const float *py, const float *pz,
+                             const int32_t *children, const float *n_mass, const float *n_com,
+                             const float *n_min, const float *n_max, float *force,
+                             double theta, double G, double soft, double box,
+                             const int32_t *root_idx, const int32_t *top_nodes,
+                             const int32_t *node_to_cache, int64_t off, int64_t count,
+                             int64_t n_bodies, void *stream) {
+  (void)top_nodes;
+  (void)node_to_cache;
+  if (!ctx || off < 0 || count < 0) return COSMO_E_INVALID;
+  if (n_bodies > ctx->n_max) return COSMO_E_TOO_LARGE;
+  if (n_bodies < 1) return COSMO_OK;
+  const int rc = cosmo_pack_nodes_f32(children, n_mass, n_com, n_min, n_max, 2 * n_bodies - 1,
+                                      box, COSMO_MAC_REF, ctx->nodes, stream);
+  if (rc != COSMO_OK) return rc;
+  return forces_from_ctx_records(ctx, px, py, pz, force, theta, G, soft, box, root_idx, off,
+                                 count, n_bodies, stream);
+}
+
+int cosmo_compute_forces_packed(cosmo_ctx *ctx, const float *px, const float *py,
+                                const float *pz, float *force, double theta, double G,
+                                double soft, double box, const int32_t *root_idx, int64_t off,
+                                int64_t count, int64_t n_bodies, void *stream) {
+  if (!ctx || off < 0 || count < 0) return COSMO_E_INVALID;
+  if (n_bodies > ctx->n_max) return COSMO_E_TOO_LARGE;
+  if (n_bodies < 1) return COSMO_OK;
+  return forces_from_ctx_records(ctx, px, py, pz, force, theta, G, soft, box, root_idx, off,
+                                 count, n_bodies, stream);
 }
 
 int cosmo_traverse_split(cosmo_ctx *ctx, const float *px, const float *py, const float *pz,

@@ -321,6 +321,37 @@ __global__ __launch_bounds__(256) void k_normalize_com(const double *__restrict_
   }
 }
 
+// tree_kernels.py:193-214 on the reference's float32 node arrays
+// (octree.py:37-38), in the reference's arithmetic: inv = 1.0 / m is FP64
+// (a Python float literal over a float32 element) and com *= inv rounds the
+// FP64 product back to the float32 array.
+__global__ __launch_bounds__(256) void k_normalize_com_f32(const float *__restrict__ n_mass,
+                                                           float *__restrict__ n_com,
+                                                           int64_t m) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const double nm = static_cast<double>(n_mass[i]);
+  if (nm > 0) {
+    const double inv = 1.0 / nm;
+    n_com[3 * i + 0] = static_cast<float>(static_cast<double>(n_com[3 * i + 0]) * inv);
+    n_com[3 * i + 1] = static_cast<float>(static_cast<double>(n_com[3 * i + 1]) * inv);
+    n_com[3 * i + 2] = static_cast<float>(static_cast<double>(n_com[3 * i + 2]) * inv);
+  }
+}
+
+// FP64 node mass and CoM sums rounded to the reference's float32 arrays.
+__global__ __launch_bounds__(256) void k_round_moments(const double *__restrict__ m64,
+                                                       const double *__restrict__ c64,
+                                                       float *__restrict__ n_mass,
+                                                       float *__restrict__ n_com, int64_t m) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  n_mass[i] = static_cast<float>(m64[i]);
+  n_com[3 * i + 0] = static_cast<float>(c64[3 * i + 0]);
+  n_com[3 * i + 1] = static_cast<float>(c64[3 * i + 1]);
+  n_com[3 * i + 2] = static_cast<float>(c64[3 * i + 2]);
+}
+
 // tree_kernels.py:216-234: the largest parentless index wins (deterministic).
 __global__ __launch_bounds__(256) void k_find_root(const int32_t *__restrict__ parents,
                                                    int32_t *__restrict__ root_idx, int64_t m) {
@@ -419,10 +450,12 @@ __device__ __forceinline__ void normalize_one(double nm, double &c0, double &c1,
 // far (the other end of each internal node's leaf range, from the build)
 // an AABB record of a node with at most `bucket` leaves also holds its leaf
 // range (first, count); count 0 marks every other node.
-template <int MAC, bool NORMALIZE>
+// T is double (this build's FP64 node arrays) or float (the reference's
+// float32 arrays, widened exactly); NORMALIZE needs double.
+template <int MAC, bool NORMALIZE, typename T = double>
 __global__ __launch_bounds__(256) void k_pack_nodes(
-    const int32_t *__restrict__ children, const double *__restrict__ n_mass,
-    double *__restrict__ n_com, const float *__restrict__ n_min,
+    const int32_t *__restrict__ children, const T *__restrict__ n_mass,
+    T *__restrict__ n_com, const float *__restrict__ n_min,
     const float *__restrict__ n_max, int64_t m, float boxf, void *__restrict__ out,
     const int32_t *__restrict__ far, int bucket) {
   const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -576,6 +609,37 @@ int cosmo_compute_multipoles(const float *px, const float *py, const float *pz,
   return COSMO_OK;
 }
 
+int cosmo_compute_multipoles_f32(cosmo_ctx *ctx, const float *px, const float *py,
+                                 const float *pz, const float *mass, const int32_t *children,
+                                 const int32_t *parents, float *n_mass, float *n_com,
+                                 float *n_min, float *n_max, int32_t *counts, int64_t n,
+                                 void *stream) {
+  if (!ctx || n < 0 || n > kMaxBodies) return COSMO_E_INVALID;
+  if (n == 0) return COSMO_OK;
+  if (n > ctx->n_max) return COSMO_E_TOO_LARGE;
+  if (!n_mass || !n_com) return COSMO_E_INVALID;
+  // FP64 sums in the context's record buffer (80 n_max >= 32 (2n - 1) bytes)
+  const int64_t m = 2 * n - 1;
+  double *m64 = reinterpret_cast<double *>(ctx->nodes);
+  double *c64 = m64 + m;
+  const int rc = cosmo_compute_multipoles(px, py, pz, mass, children, parents, m64, c64, n_min,
+                                          n_max, counts, n, stream);
+  if (rc != COSMO_OK) return rc;
+  k_round_moments<<<blocks_for(m, 256), 256, 0, as_stream(stream)>>>(m64, c64, n_mass, n_com, m);
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+int cosmo_normalize_com_f32(const float *n_mass, float *n_com, int64_t n_nodes, void *stream) {
+  if (n_nodes < 0 || n_nodes > 2 * kMaxBodies) return COSMO_E_INVALID;
+  if (n_nodes == 0) return COSMO_OK;
+  if (!n_mass || !n_com) return COSMO_E_INVALID;
+  k_normalize_com_f32<<<blocks_for(n_nodes, 256), 256, 0, as_stream(stream)>>>(n_mass, n_com,
+                                                                               n_nodes);
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
 int cosmo_normalize_com(const double *n_mass, double *n_com, int64_t n_nodes, void *stream) {
   if (n_nodes < 0 || n_nodes > 2 * kMaxBodies) return COSMO_E_INVALID;
   if (n_nodes == 0) return COSMO_OK;
@@ -630,6 +694,26 @@ static int pack_impl(const int32_t *children, const double *n_mass, double *n_co
     k_pack_nodes<COSMO_MAC_AABB, false><<<g, 256, 0, s>>>(children, n_mass, n_com, n_min, n_max, m, boxf, nodes, far, bucket);
   else
     return COSMO_E_INVALID;
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+int cosmo_pack_nodes_f32(const int32_t *children, const float *n_mass, const float *n_com,
+                         const float *n_min, const float *n_max, int64_t n_nodes, double box,
+                         int mac, void *nodes, void *stream) {
+  const int64_t m = n_nodes;
+  if (m < 0 || m > 2 * kMaxBodies || (m > 0 && (m & 1) == 0)) return COSMO_E_INVALID;
+  if (mac != COSMO_MAC_REF && mac != COSMO_MAC_AABB) return COSMO_E_INVALID;
+  if (m == 0) return COSMO_OK;
+  if (!children || !n_mass || !n_com || !n_min || !n_max || !nodes) return COSMO_E_INVALID;
+  const unsigned g = blocks_for(m, 256);
+  hipStream_t s = as_stream(stream);
+  const float boxf = static_cast<float>(box);
+  float *com = const_cast<float *>(n_com);  // not written without NORMALIZE
+  if (mac == COSMO_MAC_REF)
+    k_pack_nodes<COSMO_MAC_REF, false, float><<<g, 256, 0, s>>>(children, n_mass, com, n_min, n_max, m, boxf, nodes, nullptr, 0);
+  else
+    k_pack_nodes<COSMO_MAC_AABB, false, float><<<g, 256, 0, s>>>(children, n_mass, com, n_min, n_max, m, boxf, nodes, nullptr, 0);
   COSMO_CHECK_LAUNCH();
   return COSMO_OK;
 }

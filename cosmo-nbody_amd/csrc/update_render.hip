@@ -87,6 +87,18 @@ __global__ __launch_bounds__(256) void k_render_density_dev(const float *__restr
   splat_one(px[i], py[i], pz[i], m, res, scale, grid);
 }
 
+// grid[p] += bins[p] * 2^-frac_bits: the fixed-point splat added to a float32
+// density grid (camera.py:26).  The bin is exact in FP64 below 2^53 and the
+// product is rounded once to FP32 before the add.
+__global__ __launch_bounds__(256) void k_grid_add_fixed(const unsigned long long *__restrict__ bins,
+                                                        float *__restrict__ grid, int64_t cells,
+                                                        double scale) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= cells) return;
+  const unsigned long long b = bins[i];
+  if (b != 0ull) grid[i] += static_cast<float>(static_cast<double>(b) * scale);
+}
+
 // Colour mapping of a density frame (src/utils/visualizer.py:115-171), fused
 // after the splat: img = log1p(120 x) / (max(img) + 1e-9), val = img^2.2,
 // the four-band palette, x1.3, clipped to [0, 1]; x is the fixed-point bin
@@ -207,6 +219,17 @@ int cosmo_render_density(const float *px, const float *py, const float *pz, uint
   const float scale = ldexpf(1.0f, frac_bits);
   k_render_density<<<blocks_for(n, 256), 256, 0, as_stream(stream)>>>(
       px, py, pz, reinterpret_cast<unsigned long long *>(grid), m, res, scale, n);
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+int cosmo_grid_add_fixed(const uint64_t *bins, float *grid, int64_t cells, int frac_bits,
+                         void *stream) {
+  if (cells < 0 || frac_bits < 0 || frac_bits > 30) return COSMO_E_INVALID;
+  if (cells == 0) return COSMO_OK;
+  if (!bins || !grid) return COSMO_E_INVALID;
+  k_grid_add_fixed<<<blocks_for(cells, 256), 256, 0, as_stream(stream)>>>(
+      reinterpret_cast<const unsigned long long *>(bins), grid, cells, ldexp(1.0, -frac_bits));
   COSMO_CHECK_LAUNCH();
   return COSMO_OK;
 }

@@ -10,9 +10,12 @@ top-of-tree cache (build_top_tree_cache only feeds an on-chip copy of node
 data, tree_kernels.py:264-314), no host synchronisation inside the step,
 and normalize_com fused with packing of the node records.  ``run()``
 steps and captures the density grid; video output and NVML polling of the
-reference are not part of this package.
+reference are not part of this package.  With LEARNED_THETA the engine
+installs the trained per-group theta network named by THETA_NETWORK.
 """
 from __future__ import annotations
+
+import os
 
 import numpy as np
 import torch
@@ -25,12 +28,42 @@ from src.utils.cosmology import get_hubble_param
 from src.utils.visualizer import get_mvp_matrix
 
 
+NETWORK_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(
+    os.path.abspath(__file__)))), "networks")
+
+
 def mac_of(config) -> int:
     return MAC_AABB if str(getattr(config, "MAC", "ref")).lower() == "aabb" else MAC_REF
 
 
+def network_path(name: str) -> str:
+    """A trained theta network by file path, or by name in the package's
+    networks/ directory (e.g. "halos_128.npz")."""
+    if os.path.exists(name):
+        return name
+    p = os.path.join(NETWORK_DIR, name)
+    if os.path.exists(p):
+        return p
+    raise FileNotFoundError(f"trained theta network {name!r} not found (nor in {NETWORK_DIR})")
+
+
 class SimulationEngine:
-    def __init__(self, config, state, device="cuda"):
+    def __init__(self, config, state, device="cuda", theta_selector=None):
+        # LEARNED_THETA installs the network named by THETA_NETWORK (or the one
+        # given); it never falls back to fixed theta silently
+        if getattr(config, "LEARNED_THETA", False):
+            if mac_of(config) != MAC_AABB:
+                raise ValueError("LEARNED_THETA needs MAC='aabb': the network predicts the "
+                                 "errors of the AABB acceptance test")
+            if theta_selector is None:
+                name = str(getattr(config, "THETA_NETWORK", "") or "")
+                if not name:
+                    raise ValueError("LEARNED_THETA is set but no trained network is named: "
+                                     "set THETA_NETWORK (a file written by LearnedTheta.save) "
+                                     "or pass theta_selector")
+                path = network_path(name)
+                from src.core.learned_theta import LearnedTheta
+                theta_selector = LearnedTheta.load(path, device=device)
         self.config = config
         self.state = state
         self.tree_manager = OctreeManager(state.n, config.TPB, device=device)
@@ -49,7 +82,8 @@ class SimulationEngine:
         self.straddle_mass = (float(getattr(config, "STRADDLE_MASS", 0.0))
                               if self.mac == MAC_AABB else 0.0)
         self.theta_group = None     # per-group opening angles (learned theta), or None
-        self.theta_selector = None  # LearnedTheta: recomputes theta_group after each build
+        # LearnedTheta: recomputes theta_group after each build
+        self.theta_selector = theta_selector
         self.group_order = None     # launch order of the groups under per-group theta
         self._a_dev = None          # device scale factor while a step is being captured
         self.stats = None           # int64[3] interaction / visit / pop counters, or None

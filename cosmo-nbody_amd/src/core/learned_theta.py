@@ -353,7 +353,8 @@ def _engine_at(c, seed, steps):
     from src.core.state import SimulationState
     from src.utils.cosmology import generate_ics
     pos, vel = generate_ics(c, seed)
-    eng = SimulationEngine(c, SimulationState(len(pos[0]), pos, vel))
+    # the states that train and judge a network run the plain method
+    eng = SimulationEngine(c.replace(LEARNED_THETA=False), SimulationState(len(pos[0]), pos, vel))
     for _ in range(steps):
         eng.step()
     eng.build_tree()
@@ -421,11 +422,18 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
     on a log grid, the cheapest settings meeting the target by the cost model
     are timed and the fastest is kept.  Reports traversal and full-step
     times, interactions and the hindsight bound (per group the theta of least
-    cost + lambda * measured error)."""
+    cost + lambda * measured error).  theta* and lambda are tuned on one
+    sample of groups; both errors are then measured again on a disjoint
+    sample of as many groups (err_*_confirm)."""
     eng = _engine_at(c, *val)
     G = (eng.state.n + 63) // 64
     rng = np.random.default_rng(seed + 12345)
-    vgroups = np.sort(rng.choice(G, size=min(n_val_groups, G), replace=False))
+    # the tuning sample (theta*, lambda) and a disjoint confirmation sample on
+    # which both errors are measured again
+    drawn = rng.permutation(G)
+    nv = min(n_val_groups, G // 2)
+    vgroups = np.sort(drawn[:nv])
+    cgroups = np.sort(drawn[nv:2 * nv])
     probe = GroupProbe(eng, vgroups)
     err2_k, fixed_curve = [], []
     for t in thetas:
@@ -495,6 +503,13 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
     lt.lam = best[0]
     tg = lt.theta_groups(eng).clone()
     e_learned, i_learned = probe.rel_rms(0.5, tg.index_select(0, vg).contiguous())
+    cprobe = GroupProbe(eng, cgroups)
+    cg = torch.from_numpy(cgroups).to(eng.state.pos[0].device)
+    e_fixed_c, _ = cprobe.rel_rms(theta_star)
+    e_learned_c, _ = cprobe.rel_rms(0.5, tg.index_select(0, cg).contiguous())
+    log(f"confirmation sample ({len(cgroups)} other groups): fixed {e_fixed_c:.4e}, "
+        f"learned {e_learned_c:.4e}")
+    del cprobe
     t_learned_unordered = _time_traversal(eng, 0.5, tg)
     t_learned = _time_traversal(eng, 0.5, tg, group_order=lt.launch_order(tg).clone())
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -517,7 +532,9 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
            "hip_graph_steps": bool(graph),
            "theta_fixed": theta_star, "err_fixed": e_fixed,
            "inter_fixed": i_fixed, "lambda": lt.lam, "err_learned": e_learned,
-           "inter_learned": i_learned, "traversal_ms_fixed": t_fixed,
+           "inter_learned": i_learned, "confirm_groups": int(len(cgroups)),
+           "err_fixed_confirm": e_fixed_c, "err_learned_confirm": e_learned_c,
+           "traversal_ms_fixed": t_fixed,
            "traversal_ms_learned": t_learned,
            "traversal_ms_learned_unordered": t_learned_unordered, "select_ms": t_select,
            "speedup_traversal": t_fixed / t_learned,

@@ -53,6 +53,13 @@ SIGNATURES = {
     "cosmo_build_tree_spans": ([_p, _p, _p, _p, _i64, _p], _int),
     "cosmo_compute_multipoles": ([_p] * 11 + [_i64, _p], _int),
     "cosmo_normalize_com": ([_p, _p, _i64, _p], _int),
+    "cosmo_compute_multipoles_f32": ([_p] * 12 + [_i64, _p], _int),
+    "cosmo_normalize_com_f32": ([_p, _p, _i64, _p], _int),
+    "cosmo_pack_nodes_f32": ([_p] * 5 + [_i64, _f64, _int, _p, _p], _int),
+    "cosmo_compute_forces_f32": ([_p] * 10 + [_f64] * 4 + [_p, _p, _p, _i64, _i64, _i64, _p],
+                                 _int),
+    "cosmo_compute_forces_packed": ([_p] * 5 + [_f64] * 4 + [_p, _i64, _i64, _i64, _p], _int),
+    "cosmo_grid_add_fixed": ([_p, _p, _i64, _int, _p], _int),
     "cosmo_zeldovich_work_bytes": ([_i64], _i64),
     "cosmo_zeldovich_ics": ([_i64, _p, _f64, _f64, _p] + [_p] * 6 + [_p], _int),
     "cosmo_find_root": ([_p, _p, _i64, _p], _int),
@@ -155,6 +162,7 @@ class Context:
         check(load().cosmo_ctx_create(self.n_max, C.byref(h)), "cosmo_ctx_create")
         self.handle = h
         self.device = torch.cuda.current_device()
+        self.packed_key = None  # what the records in cosmo_ctx_nodes were packed from
 
     @property
     def nodes_ptr(self):
@@ -173,6 +181,20 @@ class Context:
 
 
 _shared_ctx = {}
+
+# Tree generation: bumped by every drop-in that writes tree arrays
+# (reset_tree_buffers, build_tree, compute_multipoles, normalize_com and the
+# packing variants).  compute_forces repacks its node records only when the
+# generation, the arrays or their torch versions changed since its last pack.
+_tree_generation = [0]
+
+
+def bump_tree_generation() -> None:
+    _tree_generation[0] += 1
+
+
+def tree_generation() -> int:
+    return _tree_generation[0]
 
 
 def shared_context(n: int) -> Context:

@@ -17,7 +17,7 @@ from __future__ import annotations
 import torch
 
 from ._hip import (MAC_AABB, MAC_REF, TRAV_COMPENSATED, TRAV_GROUP_STATS, TRAV_PAIRS, TRAV_STATS,
-                   check, kernel, load, ptr, shared_context, stream_handle)
+                   check, kernel, load, ptr, shared_context, stream_handle, tree_generation)
 
 __all__ = ["compute_forces", "integrate", "kick_drift", "traverse", "order_groups", "direct_sum",
            "MAC_REF", "MAC_AABB"]
@@ -30,26 +30,43 @@ def _xyz(pos_x, pos_y, pos_z, n):
                                                      (pos_z, "pos_z"))]
 
 
+def _packed_key(ctx, tensors, n, box):
+    """What decides the packed records: the arrays (storage, torch version),
+    the tree generation of the drop-ins that write them, n and the box."""
+    return (tree_generation(), n, float(box),
+            tuple((t.data_ptr(), t._version, t.dtype) for t in tensors))
+
+
 @kernel
 def compute_forces(pos_x, pos_y, pos_z, children, n_mass, n_com, n_min, n_max, force, theta, G,
                    soft, box, root, top_nodes, node_to_cache, off, _launch=None):
+    """Node arrays as the reference allocates them (float32, octree.py:37-40)
+    or this build's float64 mass and CoM.  The reference calls this once per
+    5000-particle batch on one tree (engine.py:91-111): the FP32 records are
+    packed into the shared context on the first call for a tree and reused
+    while the arrays are unchanged, so a batch only runs the walk."""
     n = pos_x.shape[0]
     off = int(off)
     count = (n - off) if _launch is None else _launch[0] * _launch[1]
     m = 2 * n - 1
-    if n_mass.dtype != torch.float64:  # reference-layout FP32 node arrays
-        n_mass = n_mass.to(torch.float64)
-        n_com = n_com.to(torch.float64)
     ctx = shared_context(n)
-    check(load().cosmo_compute_forces(
-        ctx.handle, *_xyz(pos_x, pos_y, pos_z, n),
-        ptr(children, torch.int32, "children", numel=2 * m),
-        ptr(n_mass, torch.float64, "n_mass", numel=m),
-        ptr(n_com, torch.float64, "n_com", numel=3 * m),
-        ptr(n_min, _F32, "n_min", numel=3 * m), ptr(n_max, _F32, "n_max", numel=3 * m),
-        ptr(force, _F32, "force", numel=3 * n), float(theta), float(G), float(soft),
-        float(box), ptr(root, torch.int32, "root"), None, None, off, count, n,
-        stream_handle()), "compute_forces")
+    L = load()
+    key = _packed_key(ctx, (children, n_mass, n_com, n_min, n_max), n, box)
+    if ctx.packed_key != key:
+        f = n_mass.dtype
+        if f not in (torch.float32, torch.float64) or n_com.dtype != f:
+            raise TypeError(f"compute_forces: n_mass/n_com must both be float32 or float64, "
+                            f"got {n_mass.dtype}/{n_com.dtype}")
+        pack = L.cosmo_pack_nodes_f32 if f == torch.float32 else L.cosmo_pack_nodes
+        check(pack(ptr(children, torch.int32, "children", numel=2 * m),
+                   ptr(n_mass, f, "n_mass", numel=m), ptr(n_com, f, "n_com", numel=3 * m),
+                   ptr(n_min, _F32, "n_min", numel=3 * m), ptr(n_max, _F32, "n_max", numel=3 * m),
+                   m, float(box), MAC_REF, ctx.nodes_ptr, stream_handle()), "compute_forces")
+        ctx.packed_key = key
+    check(L.cosmo_compute_forces_packed(
+        ctx.handle, *_xyz(pos_x, pos_y, pos_z, n), ptr(force, _F32, "force", numel=3 * n),
+        float(theta), float(G), float(soft), float(box), ptr(root, torch.int32, "root"), off,
+        count, n, stream_handle()), "compute_forces")
 
 
 def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0, count=None,

@@ -5,7 +5,8 @@ projects every particle with the MVP matrix and adds 2.5/w to its pixel
 (render_kernels.py:35-47).  The grid holds fixed-point uint64 bins (int64
 tensor of shape (res, res)): each visible particle adds
 rint(2.5/w * 2^frac_bits), so the grid is independent of arrival order.
-``to_float`` converts it back to the reference's float density.
+``to_float`` converts it back to the reference's float density.  A float32
+grid (the reference's own, camera.py:26) is filled from such bins.
 """
 from __future__ import annotations
 
@@ -25,16 +26,41 @@ def _mvp_host(mvp):
     return m
 
 
+_bins = {}
+
+
+def _bins_for(device, res):
+    """Fixed-point bins for a float32 grid, one buffer per device and size."""
+    key = (device, res)
+    b = _bins.get(key)
+    if b is None:
+        b = torch.zeros((res, res), dtype=torch.int64, device=device)
+        _bins[key] = b
+    return b
+
+
 @kernel
 def render_density(pos_x, pos_y, pos_z, grid, mvp, res, frac_bits=RENDER_FRAC_BITS,
                    _launch=None):
+    """grid: int64/uint64 fixed-point bins (res, res), or the reference's
+    float32 density grid (camera.py:26), which gains the splat rounded from
+    fixed-point bins (the same value whatever the arrival order)."""
     n = pos_x.shape[0] if _launch is None else min(pos_x.shape[0], _launch[0] * _launch[1])
     res = int(res)
     m = _mvp_host(mvp)
+    as_float = isinstance(grid, torch.Tensor) and grid.dtype == torch.float32
+    bins = _bins_for(grid.device, res) if as_float else grid
+    if as_float:
+        ptr(grid, torch.float32, "grid", numel=res * res)
+        bins.zero_()
     check(load().cosmo_render_density(
         ptr(pos_x, torch.float32, "pos_x", numel=n), ptr(pos_y, torch.float32, "pos_y", numel=n),
-        ptr(pos_z, torch.float32, "pos_z", numel=n), ptr(grid, U64, "grid", numel=res * res),
+        ptr(pos_z, torch.float32, "pos_z", numel=n), ptr(bins, U64, "grid", numel=res * res),
         m.ctypes.data, res, n, int(frac_bits), stream_handle()), "render_density")
+    if as_float:
+        check(load().cosmo_grid_add_fixed(ptr(bins, U64, "bins"), ptr(grid, torch.float32, "grid"),
+                                          res * res, int(frac_bits), stream_handle()),
+              "render_density")
 
 
 def to_float(grid, frac_bits=RENDER_FRAC_BITS):

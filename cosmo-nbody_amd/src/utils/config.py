@@ -10,14 +10,16 @@ a dataclass with the same attribute names plus the knobs this build adds;
   small_32            N = 32^3 Zel'dovich ICs, theta 0.5, eps 0.02/32, 5 steps
   repo_128            N = 128^3, theta 0.5, eps 0.02/128, 20 steps, 1024^2 splat
   clustered_128       N = 128^3 evolved 300 steps, theta swept (precision study)
-  learned_128         N = 128^3 Zel'dovich ICs, AABB test with 32-leaf buckets and
-                      compensated sums, for the learned per-group opening angle
-  large_256           N = 256^3, 50 steps, 2048^2 splat, learned theta
+  learned_128         N = 128^3 Zel'dovich ICs, AABB test with 32-leaf buckets,
+                      compensated sums and straddle mass 1e-4, with the learned
+                      per-group opening angle (network THETA_NETWORK)
+  large_256           N = 256^3, 50 steps, 2048^2 splat, learned theta (the same
+                      test, straddle mass scaled to the same 210 particle masses)
   halos_128           N = 128^3 heterogeneous test state (half in 64 Plummer
                       spheres, half uniform), AABB acceptance test with
                       32-leaf buckets, compensated sums, straddle mass 1e-4
-  halos_256           the same at N = 256^3 with the 2048^2 splat (the large
-                      clustered run with the learned theta)
+  halos_256           the same at N = 256^3 with the 2048^2 splat and straddle mass
+                      1.25e-5 (the large clustered run with the learned theta)
 
 Time stepping follows the reference schedule: da = (A_END - A_START) /
 N_STEPS with N_STEPS = 2500 (src/core/engine.py:46); RUN_STEPS is how many of
@@ -68,6 +70,8 @@ STRADDLE_MASS = 0.0        # AABB test: nodes lighter than STRADDLE_MASS pass th
                            # minimum-image guard (0 = exact guard)
 TRAVERSAL_PAIRS = False    # two particles per lane in the traversal (same results, slower at 128^3)
 LEARNED_THETA = False
+THETA_NETWORK = ""         # trained per-group theta network (LearnedTheta.save file, a path
+                           # or a name in cosmo-nbody_amd/networks/) for LEARNED_THETA
 ZA_AMPLITUDE = 0.3         # rms Zel'dovich displacement in grid spacings (cosmology.py:49)
 IC_KIND = "zeldovich"      # "zeldovich" (reference) or "halos" (heterogeneous test state)
 HALO_FRACTION = 0.5
@@ -112,6 +116,7 @@ class Config:
     STRADDLE_MASS: float = STRADDLE_MASS
     TRAVERSAL_PAIRS: bool = TRAVERSAL_PAIRS
     LEARNED_THETA: bool = LEARNED_THETA
+    THETA_NETWORK: str = THETA_NETWORK
     NAME: str = "reference_defaults"
 
     @property
@@ -142,14 +147,17 @@ PRESETS = {
     "repo_128": lambda: _plain(128, RUN_STEPS=20, RES=1024, NAME="repo_128"),
     "clustered_128": lambda: _plain(128, RUN_STEPS=300, NAME="clustered_128"),
     "learned_128": lambda: _plain(128, RUN_STEPS=20, LEARNED_THETA=True, MAC="aabb", BUCKET=32,
-                                  COMPENSATED=True, NAME="learned_128"),
-    "large_256": lambda: _plain(256, RUN_STEPS=50, RES=2048, LEARNED_THETA=True,
-                                NAME="large_256"),
+                                  COMPENSATED=True, STRADDLE_MASS=1e-4,
+                                  THETA_NETWORK="halos_128.npz", NAME="learned_128"),
+    "large_256": lambda: _plain(256, RUN_STEPS=50, RES=2048, LEARNED_THETA=True, MAC="aabb",
+                                BUCKET=32, COMPENSATED=True, STRADDLE_MASS=1.25e-5,
+                                THETA_NETWORK="halos_128.npz", NAME="large_256"),
     "halos_128": lambda: _plain(128, RUN_STEPS=20, MAC="aabb", IC_KIND="halos", BUCKET=32,
                                 COMPENSATED=True, STRADDLE_MASS=1e-4, NAME="halos_128"),
     "halos_256": lambda: _plain(256, RUN_STEPS=50, RES=2048, MAC="aabb", IC_KIND="halos",
-                                BUCKET=32, COMPENSATED=True, STRADDLE_MASS=1e-4,
-                                LEARNED_THETA=True, NAME="halos_256"),
+                                BUCKET=32, COMPENSATED=True, STRADDLE_MASS=1.25e-5,
+                                LEARNED_THETA=True, THETA_NETWORK="halos_128.npz",
+                                NAME="halos_256"),
 }
 
 

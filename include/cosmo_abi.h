@@ -166,6 +166,24 @@ int cosmo_compute_multipoles(const float *pos_x, const float *pos_y,
                              float *n_max, int32_t *counts, int64_t n,
                              void *stream);
 
+/* compute_multipoles on the reference's own float32 node arrays
+ * (octree.py:37-38): the same FP64 sums, rounded once to FP32 on store.
+ * The FP64 sums are formed in the context's packed-record buffer, so records
+ * packed there before (cosmo_ctx_nodes) are overwritten.  Requires n <= the
+ * context's capacity. */
+int cosmo_compute_multipoles_f32(cosmo_ctx *ctx, const float *pos_x, const float *pos_y,
+                                 const float *pos_z, const float *mass,
+                                 const int32_t *children, const int32_t *parents,
+                                 float *n_mass, float *n_com, float *n_min,
+                                 float *n_max, int32_t *counts, int64_t n,
+                                 void *stream);
+
+/* src/kernels/tree_kernels.py:193-214 normalize_com(n_mass, n_com, n_nodes)
+ * on the reference's float32 arrays, in the reference's arithmetic:
+ * inv = 1.0 / m in FP64, com = fl32(com * inv). */
+int cosmo_normalize_com_f32(const float *n_mass, float *n_com, int64_t n_nodes,
+                            void *stream);
+
 /* src/kernels/tree_kernels.py:193-214 normalize_com(n_mass, n_com, n_nodes). */
 int cosmo_normalize_com(const double *n_mass, double *n_com, int64_t n_nodes,
                         void *stream);
@@ -189,6 +207,13 @@ int cosmo_pack_nodes(const int32_t *children, const double *n_mass,
                      const double *n_com, const float *n_min,
                      const float *n_max, int64_t n_nodes, double box, int mac,
                      void *nodes, void *stream);
+
+/* cosmo_pack_nodes from the reference's float32 node arrays (mass, CoM
+ * already normalised by normalize_com). */
+int cosmo_pack_nodes_f32(const int32_t *children, const float *n_mass,
+                         const float *n_com, const float *n_min,
+                         const float *n_max, int64_t n_nodes, double box,
+                         int mac, void *nodes, void *stream);
 
 /* normalize_com and cosmo_pack_nodes fused in one pass. */
 int cosmo_normalize_pack(const int32_t *children, const double *n_mass,
@@ -293,6 +318,32 @@ int cosmo_compute_forces(cosmo_ctx *ctx, const float *pos_x,
                          const int32_t *node_to_cache, int64_t off,
                          int64_t count, int64_t n_bodies, void *stream);
 
+/* cosmo_compute_forces on the reference's float32 node arrays
+ * (octree.py:37-40; n_mass, n_com as normalize_com leaves them). */
+int cosmo_compute_forces_f32(cosmo_ctx *ctx, const float *pos_x,
+                             const float *pos_y, const float *pos_z,
+                             const int32_t *children, const float *n_mass,
+                             const float *n_com, const float *n_min,
+                             const float *n_max, float *force, double theta,
+                             double G, double soft, double box,
+                             const int32_t *root_idx, const int32_t *top_nodes,
+                             const int32_t *node_to_cache, int64_t off,
+                             int64_t count, int64_t n_bodies, void *stream);
+
+/* The traversal half of compute_forces for a batch loop that packs once:
+ * the same walk as cosmo_compute_forces over the reference-test records
+ * already in cosmo_ctx_nodes(ctx) (written by cosmo_pack_nodes or
+ * cosmo_pack_nodes_f32 with COSMO_MAC_REF for this n_bodies-leaf tree), for
+ * particles [off, off+count).  engine.py:91-111 calls compute_forces once
+ * per 5000-particle batch on one tree; packing once per tree and calling
+ * this per batch gives the same forces. */
+int cosmo_compute_forces_packed(cosmo_ctx *ctx, const float *pos_x,
+                                const float *pos_y, const float *pos_z,
+                                float *force, double theta, double G,
+                                double soft, double box,
+                                const int32_t *root_idx, int64_t off,
+                                int64_t count, int64_t n_bodies, void *stream);
+
 /* src/kernels/physics_kernels.py:155-203 integrate(pos_x, pos_y, pos_z,
  * vel_x, vel_y, vel_z, force, a, H, dt): c_drag = 1 - H*dt and
  * c_kick = c_drift = dt/a are formed in FP64 and rounded to FP32, then
@@ -322,6 +373,14 @@ int cosmo_kick_drift_dev(float *pos_x, float *pos_y, float *pos_z,
 int cosmo_render_density(const float *pos_x, const float *pos_y,
                          const float *pos_z, uint64_t *grid, const float *mvp,
                          int32_t res, int64_t n, int frac_bits, void *stream);
+
+/* The float32 density grid of the reference (camera.py:26) from fixed-point
+ * bins: grid[p] += fl32(bins[p] * 2^-frac_bits) for p < cells.  The
+ * render_density drop-in splats into bins with cosmo_render_density and
+ * adds them to a float32 grid with this, so the float grid does not depend
+ * on arrival order either. */
+int cosmo_grid_add_fixed(const uint64_t *bins, float *grid, int64_t cells,
+                         int frac_bits, void *stream);
 
 /* cosmo_render_density with the MVP matrix read from device memory
  * (mvp_dev: float[16], row-major), for captured HIP graphs. */
