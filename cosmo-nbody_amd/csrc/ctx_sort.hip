@@ -345,12 +345,12 @@ int cosmo_compute_codes(const float *px, const float *py, const float *pz,
   return COSMO_OK;
 }
 
-int cosmo_sort_by_code(cosmo_ctx *ctx, const uint64_t *codes_in, uint64_t *codes_out,
-                       uint32_t *perm, int64_t n, int key_bits, void *stream) {
-  if (!ctx || n < 0 || key_bits < 1 || key_bits > 64) return COSMO_E_INVALID;
-  if (n > ctx->n_max) return COSMO_E_TOO_LARGE;
-  if (n == 0) return COSMO_OK;
-  if (!codes_in || !codes_out || !perm) return COSMO_E_INVALID;
+// The LSD radix sort on a given workspace: keys_alt and vals_alt hold n
+// entries, hist kRadix * ceil(n / kSortTile), digit_tot kRadix.  Every pass
+// overwrites what it reads later, so the workspace needs no clearing.
+static int sort_with(uint64_t *keys_alt, uint32_t *vals_alt, uint32_t *hist, uint32_t *digit_tot,
+                     const uint64_t *codes_in, uint64_t *codes_out, uint32_t *perm, int64_t n,
+                     int key_bits, void *stream) {
   hipStream_t s = as_stream(stream);
   const int ntiles = static_cast<int>((n + kSortTile - 1) / kSortTile);
   int passes = (key_bits + kRadixBits - 1) / kRadixBits;
@@ -363,18 +363,28 @@ int cosmo_sort_by_code(cosmo_ctx *ctx, const uint64_t *codes_in, uint64_t *codes
   const uint32_t *vin = nullptr;
   for (int p = 0; p < passes; ++p) {
     const bool to_out = ((passes - 1 - p) & 1) == 0;
-    uint64_t *kout = to_out ? codes_out : ctx->keys_alt;
-    uint32_t *vout = to_out ? perm : ctx->vals_a;
+    uint64_t *kout = to_out ? codes_out : keys_alt;
+    uint32_t *vout = to_out ? perm : vals_alt;
     const int shift = p * kRadixBits;
-    k_radix_count<<<ntiles, kSortThreads, 0, s>>>(kin, n, shift, ctx->hist, ntiles);
-    k_radix_rowscan<<<kRadix, kSortThreads, 0, s>>>(ctx->hist, ntiles, ctx->digit_tot);
-    k_radix_scatter<<<ntiles, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, shift, ctx->hist,
-                                                    ctx->digit_tot, ntiles);
+    k_radix_count<<<ntiles, kSortThreads, 0, s>>>(kin, n, shift, hist, ntiles);
+    k_radix_rowscan<<<kRadix, kSortThreads, 0, s>>>(hist, ntiles, digit_tot);
+    k_radix_scatter<<<ntiles, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, shift, hist,
+                                                    digit_tot, ntiles);
     COSMO_CHECK_LAUNCH();
     kin = kout;
     vin = vout;
   }
   return COSMO_OK;
+}
+
+int cosmo_sort_by_code(cosmo_ctx *ctx, const uint64_t *codes_in, uint64_t *codes_out,
+                       uint32_t *perm, int64_t n, int key_bits, void *stream) {
+  if (!ctx || n < 0 || key_bits < 1 || key_bits > 64) return COSMO_E_INVALID;
+  if (n > ctx->n_max) return COSMO_E_TOO_LARGE;
+  if (n == 0) return COSMO_OK;
+  if (!codes_in || !codes_out || !perm) return COSMO_E_INVALID;
+  return sort_with(ctx->keys_alt, ctx->vals_a, ctx->hist, ctx->digit_tot, codes_in, codes_out,
+                   perm, n, key_bits, stream);
 }
 
 int cosmo_permute(const uint32_t *perm, int n_arrays, const float *const *src,

@@ -418,7 +418,8 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
              target_theta=None, lambdas=None, log=print, graph=False):
     """Fixed theta against learned per-group theta on the held-out state
     `val` at equal relative RMS error (FP64 direct sum on sampled groups):
-    the fixed theta meeting the target is found by bisection; lambda is swept
+    the fixed theta meeting the target is the largest candidate within it,
+    refined by bisection; lambda is swept
     on a log grid, the cheapest settings meeting the target by the cost model
     are timed and the fastest is kept.  Reports traversal and full-step
     times, interactions and the hindsight bound (per group the theta of least
@@ -445,11 +446,19 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
     if target_theta is not None:
         target = probe.rel_rms(target_theta)[0]
         log(f"target = error of fixed theta {target_theta}: {target:.4e}")
-    theta_star = _bisect(lambda t: probe.rel_rms(t)[0], min(thetas) * 0.5, max(thetas), target,
-                         iters=18)
-    if theta_star is None:
+    # the fixed theta meeting the target: the largest (cheapest) one whose
+    # error is within it, found on the candidate curve and refined by
+    # bisection up to the next candidate (the curve need not be monotone)
+    ok_c = [k for k, (_, e, _) in enumerate(fixed_curve) if e <= target]
+    if not ok_c:
         return {"reachable": False, "val_state": val, "target": target,
                 "fixed_curve": fixed_curve}
+    k_ok = max(ok_c)
+    theta_star = float(thetas[k_ok])
+    if k_ok + 1 < len(thetas):
+        t_b = _bisect(lambda t: probe.rel_rms(t)[0], thetas[k_ok], thetas[k_ok + 1], target,
+                      iters=14)
+        theta_star = t_b if t_b is not None else theta_star
     e_fixed, i_fixed = probe.rel_rms(theta_star)
     log(f"fixed theta meeting {target:.1e}: {theta_star:.4f} (error {e_fixed:.4e})")
     t_fixed = _time_traversal(eng, theta_star, None)

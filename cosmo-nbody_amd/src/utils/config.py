@@ -11,15 +11,15 @@ a dataclass with the same attribute names plus the knobs this build adds;
   repo_128            N = 128^3, theta 0.5, eps 0.02/128, 20 steps, 1024^2 splat
   clustered_128       N = 128^3 evolved 300 steps, theta swept (precision study)
   learned_128         N = 128^3 Zel'dovich ICs, AABB test with 32-leaf buckets,
-                      compensated sums and straddle mass 1e-4, with the learned
+                      compensated sums and straddle mass 5e-5, with the learned
                       per-group opening angle (network THETA_NETWORK)
   large_256           N = 256^3, 50 steps, 2048^2 splat, learned theta (the same
-                      test, straddle mass scaled to the same 210 particle masses)
+                      test, straddle mass scaled to the same 105 particle masses)
   halos_128           N = 128^3 heterogeneous test state (half in 64 Plummer
                       spheres, half uniform), AABB acceptance test with
-                      32-leaf buckets, compensated sums, straddle mass 1e-4
+                      32-leaf buckets, compensated sums, straddle mass 5e-5
   halos_256           the same at N = 256^3 with the 2048^2 splat and straddle mass
-                      1.25e-5 (the large clustered run with the learned theta)
+                      6.25e-6 (the large clustered run with the learned theta)
 
 Time stepping follows the reference schedule: da = (A_END - A_START) /
 N_STEPS with N_STEPS = 2500 (src/core/engine.py:46); RUN_STEPS is how many of
@@ -147,15 +147,15 @@ PRESETS = {
     "repo_128": lambda: _plain(128, RUN_STEPS=20, RES=1024, NAME="repo_128"),
     "clustered_128": lambda: _plain(128, RUN_STEPS=300, NAME="clustered_128"),
     "learned_128": lambda: _plain(128, RUN_STEPS=20, LEARNED_THETA=True, MAC="aabb", BUCKET=32,
-                                  COMPENSATED=True, STRADDLE_MASS=1e-4,
+                                  COMPENSATED=True, STRADDLE_MASS=5e-5,
                                   THETA_NETWORK="halos_128.npz", NAME="learned_128"),
     "large_256": lambda: _plain(256, RUN_STEPS=50, RES=2048, LEARNED_THETA=True, MAC="aabb",
-                                BUCKET=32, COMPENSATED=True, STRADDLE_MASS=1.25e-5,
+                                BUCKET=32, COMPENSATED=True, STRADDLE_MASS=6.25e-6,
                                 THETA_NETWORK="halos_128.npz", NAME="large_256"),
     "halos_128": lambda: _plain(128, RUN_STEPS=20, MAC="aabb", IC_KIND="halos", BUCKET=32,
-                                COMPENSATED=True, STRADDLE_MASS=1e-4, NAME="halos_128"),
+                                COMPENSATED=True, STRADDLE_MASS=5e-5, NAME="halos_128"),
     "halos_256": lambda: _plain(256, RUN_STEPS=50, RES=2048, MAC="aabb", IC_KIND="halos",
-                                BUCKET=32, COMPENSATED=True, STRADDLE_MASS=1.25e-5,
+                                BUCKET=32, COMPENSATED=True, STRADDLE_MASS=6.25e-6,
                                 LEARNED_THETA=True, THETA_NETWORK="halos_128.npz",
                                 NAME="halos_256"),
 }
