@@ -425,7 +425,8 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
     times, interactions and the hindsight bound (per group the theta of least
     cost + lambda * measured error).  theta* and lambda are tuned on one
     sample of groups; both errors are then measured again on a disjoint
-    sample of as many groups (err_*_confirm)."""
+    sample of as many groups (err_*_confirm).  Returns (result, LearnedTheta),
+    or (result, None) when the target cannot be met."""
     eng = _engine_at(c, *val)
     G = (eng.state.n + 63) // 64
     rng = np.random.default_rng(seed + 12345)
@@ -452,7 +453,7 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
     ok_c = [k for k, (_, e, _) in enumerate(fixed_curve) if e <= target]
     if not ok_c:
         return {"reachable": False, "val_state": val, "target": target,
-                "fixed_curve": fixed_curve}
+                "fixed_curve": fixed_curve}, None
     k_ok = max(ok_c)
     theta_star = float(thetas[k_ok])
     if k_ok + 1 < len(thetas):
@@ -477,7 +478,7 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
     # slowest wavefront.  All groups at theta_k take cost_k * G ms with
     # `concurrent` wavefronts resident at a time.
     props = torch.cuda.get_device_properties(eng.state.pos[0].device)
-    concurrent = props.multi_processor_count * 4 * WAVES_PER_SIMD
+    concurrent = min(props.multi_processor_count * 4 * WAVES_PER_SIMD, G)
     wave_ms = cost_arr * concurrent
     ok_k = [k for k in range(len(thetas)) if wave_ms[k] <= WAVE_BUDGET * t_fixed]
     theta_min = float(thetas[ok_k[0]]) if ok_k else float(thetas[-1])
@@ -501,7 +502,7 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
     ok = sorted((sw for sw in sweep if sw[1] <= target), key=lambda sw: sw[3])
     if not ok:
         return {"reachable": False, "val_state": val, "target": target,
-                "theta_fixed": theta_star, "sweep": sweep}
+                "theta_fixed": theta_star, "theta_min": theta_min, "sweep": sweep}, None
     best = None
     for lam, _, _, _ in ok[:5]:  # the cheapest settings by the cost model, timed
         tg = lt.theta_groups(eng, lam).clone()
@@ -573,9 +574,8 @@ def experiment(preset="repo_128", target=1e-3, mac="aabb", states=None, n_groups
     model, cost, info = train(c, train_states, n_groups, thetas, fit_steps, seed, log)
     out = []
     for v in val_states:
-        r = evaluate(c, model, cost, thetas, v, target, n_val_groups, seed, target_theta,
-                     lambdas, log)
-        res, lt = r if isinstance(r, tuple) else (r, None)
+        res, lt = evaluate(c, model, cost, thetas, v, target, n_val_groups, seed, target_theta,
+                           lambdas, log)
         res.update(info)
         out.append((res, lt))
     return out
@@ -636,9 +636,8 @@ def main():
               else DEFAULT_THETAS)
         cost = measure_costs(ce, vals[0], th, log)
         for v in vals:
-            r = evaluate(ce, model, cost, th, v, args.target, args.val_groups, log=log,
-                         graph=args.graph)
-            res = r[0] if isinstance(r, tuple) else r
+            res, _ = evaluate(ce, model, cost, th, v, args.target, args.val_groups, log=log,
+                              graph=args.graph)
             res.update(info)
             res["train_preset"] = args.preset
             print(json.dumps(res, default=float), flush=True)

@@ -5,7 +5,7 @@ The clustered state.  The reference's time schedule does not cluster the
 Zel'dovich ICs (G = 1, total mass 1, Hubble drag: after 300 steps
 counts-in-cells max/mean stays at 8), so the strongly non-uniform state
 these configurations ask for is the halo state: half of the particles in 64
-Plummer spheres, half uniform (IC_KIND = "halos"), here at N = 64^3.
+Plummer spheres, half uniform (IC_KIND = "halos").
 
   C2  theta in {0.3, 0.5, 0.7, 0.9} on the clustered state against the FP64
       direct sum on sampled targets (PCG64 seed 1234): the error grows with
@@ -19,6 +19,8 @@ Plummer spheres, half uniform (IC_KIND = "halos"), here at N = 64^3.
   C4  the learned step with the 2048^2 splat captured in HIP graphs: every
       replay equals the eager step bit for bit, and the error against the
       direct sum holds at the tuned level.
+
+C2 runs at N = 64^3; C3 and C4 at N = 128^3 (C4 is configured at 256^3).
 """
 import math
 
@@ -88,35 +90,36 @@ def test_c2_theta_sweep_on_clustered_state(orc, mac):
 
 @pytest.fixture(scope="module")
 def learned():
-    """A network trained on three 64^3 halo states (seeds 43-45), judged on
-    the held-out seed 42 at 1e-3: (result, LearnedTheta)."""
+    """C3 as configured (halos_128: N = 128^3, AABB test, 32-leaf buckets,
+    compensated sums, straddle mass 5e-5): a network trained on three
+    training states (seeds 43-45, 10240 sampled groups each, errors against
+    the FP64 direct sum), judged on the held-out seed 42 at 1e-3:
+    (config, result, LearnedTheta)."""
     from src.core import learned_theta as L
-    c = _halos()
-    states = [(43, 0), (44, 0), (45, 0)]
-    model, cost, info = L.train(c, states, 4096, L.DEFAULT_THETAS, 8000, log=print)
-    res, lt = L.evaluate(c, model, cost, L.DEFAULT_THETAS, (42, 0), 1e-3, n_val_groups=1024,
-                         log=print)
-    res.update(info)
-    return c, res, lt
+    from src.utils import config as cfg
+    out = L.experiment("halos_128", 1e-3, "aabb", n_groups=10240, log=print)
+    res, lt = out[0]
+    return cfg.preset("halos_128"), res, lt
 
 
 def test_c3_learned_theta_at_1e3(learned):
     c, res, lt = learned
+    assert res["reachable"], res
     print({k: res[k] for k in ("theta_fixed", "err_fixed", "err_learned", "err_fixed_confirm",
                                "err_learned_confirm", "inter_fixed", "inter_learned",
                                "step_ms_fixed", "step_ms_learned", "speedup_step")})
-    assert res["reachable"]
     assert res["err_fixed"] <= 1e-3 and res["err_learned"] <= 1e-3
     # the disjoint confirmation sample sees the same error level
     assert res["err_fixed_confirm"] <= 1.3e-3 and res["err_learned_confirm"] <= 1.3e-3
     assert res["err_learned_confirm"] <= res["err_fixed_confirm"] * 1.25
-    # fewer interactions than the fixed theta that meets the same error
+    # fewer interactions and a faster step than the fixed theta at that error
     assert res["inter_learned"] < res["inter_fixed"]
+    assert res["speedup_step"] > 1.0
     assert res["fallbacks"] == 0
 
 
 def test_c4_learned_graph_with_2048_splat(learned):
-    """C4 at test scale: LEARNED_THETA with the trained network installed
+    """C4 at N = 128^3: LEARNED_THETA with the trained network installed
     by the engine, the step with the 2048^2 splat replayed from HIP graphs
     equals the eager step bit for bit (theta_group, forces, particles,
     grid), and the learned forces of the last step stay within the tuned
@@ -128,6 +131,7 @@ def test_c4_learned_graph_with_2048_splat(learned):
     from src.kernels.physics_kernels import direct_sum
     from src.utils.cosmology import generate_ics
     c0, res, lt = learned
+    assert res["reachable"], res
     c = c0.replace(RES=2048, LEARNED_THETA=True, THETA_NETWORK="")
     from src.core.learned_theta import LearnedTheta
     pos, vel = generate_ics(c, 42)
