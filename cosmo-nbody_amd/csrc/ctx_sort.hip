@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "common.h"
 
@@ -263,7 +264,8 @@ const char *cosmo_status_string(int status) {
       return buf;
     }
     case COSMO_E_TOO_LARGE:
-      return "size exceeds the context capacity";
+      return "size exceeds the capacity (the context's n_max, or COSMO_SORT_NOALLOC_MAX "
+             "for the no-allocation sort)";
     case COSMO_E_MALFORMED_TREE:
       return "malformed tree";
     case COSMO_E_FFT:
@@ -375,6 +377,60 @@ static int sort_with(uint64_t *keys_alt, uint32_t *vals_alt, uint32_t *hist, uin
     vin = vout;
   }
   return COSMO_OK;
+}
+
+// ------------------------------------------- sort without allocation
+//
+// Workspace of cosmo_sort_by_code_noalloc in one file-scope device array:
+// alternate keys and values for COSMO_SORT_NOALLOC_MAX entries, the tile
+// histograms and the digit totals (24.5 MiB per device, held until the
+// process ends).  Host code never passes `scratch` itself: its device
+// address comes from hipGetSymbolAddress, once per device.
+constexpr int64_t kNoAllocMax = COSMO_SORT_NOALLOC_MAX;
+constexpr int64_t kNoAllocTiles = (kNoAllocMax + kSortTile - 1) / kSortTile;
+constexpr size_t kNoAllocKeysOfs = 0;
+constexpr size_t kNoAllocValsOfs = kNoAllocKeysOfs + sizeof(uint64_t) * kNoAllocMax;
+constexpr size_t kNoAllocHistOfs = kNoAllocValsOfs + sizeof(uint32_t) * kNoAllocMax;
+constexpr size_t kNoAllocTotOfs = kNoAllocHistOfs + sizeof(uint32_t) * kRadix * kNoAllocTiles;
+constexpr size_t kNoAllocBytes = kNoAllocTotOfs + sizeof(uint32_t) * kRadix;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call per device and kept).
+static int scratch_on_device(unsigned char **out) {
+  int dev = 0;
+  COSMO_CHECK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return COSMO_E_INVALID;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    COSMO_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return COSMO_E_HIP;
+    g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return COSMO_OK;
+}
+
+int cosmo_sort_by_code_noalloc(const uint64_t *codes_in, uint64_t *codes_out, uint32_t *perm,
+                               int64_t n, int key_bits, void *stream) {
+  if (n < 0 || key_bits < 1 || key_bits > 64) return COSMO_E_INVALID;
+  if (n > kNoAllocMax) return COSMO_E_TOO_LARGE;
+  if (n == 0) return COSMO_OK;
+  if (!codes_in || !codes_out || !perm) return COSMO_E_INVALID;
+  unsigned char *base = nullptr;
+  const int rc = scratch_on_device(&base);
+  if (rc != COSMO_OK) return rc;
+  return sort_with(reinterpret_cast<uint64_t *>(base + kNoAllocKeysOfs),
+                   reinterpret_cast<uint32_t *>(base + kNoAllocValsOfs),
+                   reinterpret_cast<uint32_t *>(base + kNoAllocHistOfs),
+                   reinterpret_cast<uint32_t *>(base + kNoAllocTotOfs), codes_in, codes_out, perm,
+                   n, key_bits, stream);
 }
 
 int cosmo_sort_by_code(cosmo_ctx *ctx, const uint64_t *codes_in, uint64_t *codes_out,

@@ -47,6 +47,7 @@ SIGNATURES = {
     "cosmo_tree_bytes": ([_i64, _int], _i64),
     "cosmo_compute_codes": ([_p, _p, _p, _p, _p, _i64, _int, _p], _int),
     "cosmo_sort_by_code": ([_p, _p, _p, _p, _i64, _int, _p], _int),
+    "cosmo_sort_by_code_noalloc": ([_p, _p, _p, _i64, _int, _p], _int),
     "cosmo_permute": ([_p, _int, C.POINTER(_p), C.POINTER(_p), _i64, _p], _int),
     "cosmo_reset_tree_buffers": ([_p, _p, _p, _p, _i64, _p], _int),
     "cosmo_build_tree": ([_p, _p, _p, _i64, _p], _int),

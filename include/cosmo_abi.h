@@ -133,6 +133,20 @@ int cosmo_sort_by_code(cosmo_ctx *ctx, const uint64_t *codes_in,
                        uint64_t *codes_out, uint32_t *perm, int64_t n,
                        int key_bits, void *stream);
 
+/* cosmo_sort_by_code without a context and without any allocation (no
+ * reference counterpart; for callers that cannot allocate device memory:
+ * code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The workspace is one static device array of 24.5 MiB per
+ * device, held until the process ends; n above COSMO_SORT_NOALLOC_MAX
+ * (2^21 = 128^3 codes) returns COSMO_E_TOO_LARGE.  Every call on a device
+ * shares that array: calls that overlap on one device (other streams,
+ * other host threads) race.  The first call on a device looks up the
+ * array's address (hipGetSymbolAddress); make it outside graph capture. */
+#define COSMO_SORT_NOALLOC_MAX (1 << 21)
+int cosmo_sort_by_code_noalloc(const uint64_t *codes_in, uint64_t *codes_out,
+                               uint32_t *perm, int64_t n, int key_bits,
+                               void *stream);
+
 /* src/core/state.py:37-40 the gathers x[sort_idx]: dst[a][k] = src[a][perm[k]]
  * for n_arrays (<= 8) float arrays; src/dst are host arrays of device
  * pointers; dst must not alias src. */
