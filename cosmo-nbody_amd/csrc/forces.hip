@@ -295,9 +295,13 @@ __global__ __launch_bounds__(256) void k_traverse(
     if (STATS) w_pop++;
     const int node = __builtin_amdgcn_readlane(stk, top);
     bool in[PPL];
+    uint64_t m_in[PPL];  // the entry's lanes as a scalar mask
 #pragma unroll
     for (int h = 0; h < PPL; ++h) {
       in[h] = depth[h] > top;
+      m_in[h] = __builtin_amdgcn_ballot_w64(in[h]);
+      // (keeps the compare ahead of the update, so the old depth needs no copy)
+      __builtin_amdgcn_sched_barrier(0);
       depth[h] = min(depth[h], top);
     }
     // Records are wave-uniform loads read through the constant address space
@@ -311,8 +315,10 @@ __global__ __launch_bounds__(256) void k_traverse(
     bool leaf;
     float4 A, B, C, D;
     if constexpr (MAC == COSMO_MAC_REF) {
+      // 32-bit byte offset: one scalar shift, and the load adds it to the
+      // base (traverse_impl refuses trees whose records pass 4 GiB)
       const ConstF *rec = (const ConstF *)(reinterpret_cast<const char *>(tree) +
-                                           int64_t(node) * int64_t(sizeof(Inode)));
+                                           uint32_t(node) * uint32_t(sizeof(Inode)));
       A = ld4(rec);
       B = ld4(rec + 4);
       C = D = A;
@@ -335,37 +341,67 @@ __global__ __launch_bounds__(256) void k_traverse(
     // `stack_top + 1 < 64` (physics_kernels.py:143) is uniform here
     const bool room = top + 1 < 64;
     bool open[PPL], hit[PPL], rej[PPL];
+    uint64_t om = 0;
+    if constexpr (MAC == COSMO_MAC_REF) {
+      // lanes may open the node only while the stack has room (:143)
+      const uint64_t m_room = room ? ~0ull : 0ull;
+      // The reference test with the lane sets kept as scalar masks: each
+      // compare is one ballot, the set logic runs on the scalar unit, and
+      // the divergent interaction runs under inverse_ballot(mask) (no lane
+      // bools materialised in vector registers).
+#pragma unroll
+      for (int h = 0; h < PPL; ++h) {
+        Disp d = displace(A.x, A.y, A.z, x[h], y[h], z[h]);
+        // ref_decide() on masks: accept = sx^2 < theta^2 d^2 and near_ok =
+        // d^2 > 1e-7 as the reference decides them in FP64; lanes of the
+        // entry inside the rounding band redo them in FP64
+        const float sx2 = B.x;
+        const float t = th2[h] * d.d2;
+        uint64_t m_acc = __builtin_amdgcn_ballot_w64(sx2 < t);
+        const float bnd = __builtin_fmaf(k1[h], B.y, k2[h]);
+        uint64_t m_band = __builtin_amdgcn_ballot_w64(fabsf(sx2 - t) <= bnd);
+        uint64_t m_near = ~0ull;
+        if (__float_as_int(B.x) < big_bits) {
+          // (the empty asm keeps this path a branch)
+          asm volatile("");
+          m_near = __builtin_amdgcn_ballot_w64(d.d2 > 1e-7f);
+          m_band |= __builtin_amdgcn_ballot_w64(fabsf(d.d2 - 1e-7f) <= 2e-10f);
+        }
+        m_band &= m_in[h];
+        if (__builtin_expect(m_band != 0, 0)) {
+          bool accept = __builtin_amdgcn_inverse_ballot_w64(m_acc);
+          bool near_ok = __builtin_amdgcn_inverse_ballot_w64(m_near);
+          if (__builtin_amdgcn_inverse_ballot_w64(m_band)) {
+            const double e2 = exact_d2(A.x, A.y, A.z, x[h], y[h], z[h]);
+            if (!leaf) {
+              const double sxd = static_cast<double>(pin(B.y));
+              accept = (sxd * sxd) < (th2d[h] * e2);
+            }
+            near_ok = e2 > 1e-7;
+          }
+          m_acc = __builtin_amdgcn_ballot_w64(accept);
+          m_near = __builtin_amdgcn_ballot_w64(near_ok);
+        }
+        const uint64_t m_hit = m_in[h] & m_acc & m_near;
+        const uint64_t m_open = m_in[h] & ~m_acc & m_room;
+        om |= m_open;
+        open[h] = __builtin_amdgcn_inverse_ballot_w64(m_open);
+        if (__builtin_amdgcn_inverse_ballot_w64(m_hit)) {
+          fix_ties(d, A.x, A.y, A.z, x[h], y[h], z[h]);
+          accumulate<COMP>(d, A.w, p.eps2, ax[h], ay[h], az[h], cx[h], cy[h], cz[h]);
+        }
+        if (STATS) {
+          w_vis += __builtin_popcountll(m_in[h]);
+          w_int += __builtin_popcountll(m_hit);
+        }
+      }
+    } else {
 #pragma unroll
     for (int h = 0; h < PPL; ++h) {
       // every lane evaluates the node; only the entry's lanes (in) use it
       Disp d = displace(A.x, A.y, A.z, x[h], y[h], z[h]);
       bool accept, near_ok;
-      if constexpr (MAC == COSMO_MAC_REF) {
-        // ref_decide() written out: the same decision, kept inline here because
-        // passing the lane masks through a function costs this loop extra
-        // VALU instructions (measured in the generated code)
-        const float sx2 = B.x;
-        const float t = th2[h] * d.d2;
-        accept = sx2 < t;
-        const float bnd = __builtin_fmaf(k1[h], B.y, k2[h]);
-        bool unsure;
-        if (__float_as_int(B.x) >= big_bits) {
-          near_ok = true;
-          unsure = in[h] & (fabsf(sx2 - t) <= bnd);
-        } else {
-          asm volatile("");
-          near_ok = d.d2 > 1e-7f;
-          unsure = in[h] & ((fabsf(d.d2 - 1e-7f) <= 2e-10f) | (fabsf(sx2 - t) <= bnd));
-        }
-        if (__builtin_expect(unsure, 0)) {
-          const double e2 = exact_d2(A.x, A.y, A.z, x[h], y[h], z[h]);
-          if (!leaf) {
-            const double sxd = static_cast<double>(pin(B.y));
-            accept = (sxd * sxd) < (th2d[h] * e2);
-          }
-          near_ok = e2 > 1e-7;
-        }
-      } else {
+      {
         // (s/theta + dl)^2 < d^2, and every particle of the node sits on the
         // same side of the minimum-image boundary as its CoM in each axis,
         // unless the node is lighter than straddle_mu (a node across the
@@ -387,11 +423,10 @@ __global__ __launch_bounds__(256) void k_traverse(
       rej[h] = in[h] & !accept;
       open[h] = rej[h] & room;
       if (hit[h]) {
-        fix_ties<MAC == COSMO_MAC_AABB>(d, A.x, A.y, A.z, x[h], y[h], z[h]);
+        fix_ties<true>(d, A.x, A.y, A.z, x[h], y[h], z[h]);
         accumulate<COMP>(d, A.w, p.eps2, ax[h], ay[h], az[h], cx[h], cy[h], cz[h]);
       }
     }
-    uint64_t om = 0;
 #pragma unroll
     for (int h = 0; h < PPL; ++h) {
       om |= __ballot(open[h]);
@@ -399,6 +434,7 @@ __global__ __launch_bounds__(256) void k_traverse(
         w_vis += __builtin_popcountll(__ballot(in[h]));
         w_int += __builtin_popcountll(__ballot(hit[h]));
       }
+    }
     }
     // AABB records of small nodes carry their leaf range (count > 0): the
     // lanes that reject such a node sum its leaves directly, in leaf order,
@@ -457,7 +493,7 @@ __global__ __launch_bounds__(256) void k_traverse(
       }
     } else if (om != 0) {
       // internal nodes have two children (build_tree): push c2, then c1
-      // (physics_kernels.py:143-149)
+      // (physics_kernels.py:143-149); without room the reference drops them
       stk = write_lane(stk, __float_as_int(B.w), top);
       stk = write_lane(stk, __float_as_int(B.z), top + 1);
 #pragma unroll
@@ -955,6 +991,9 @@ static int traverse_impl(const float *px, const float *py, const float *pz, cons
       n_tree > kMaxBodies)
     return COSMO_E_INVALID;
   if (mac != COSMO_MAC_REF && mac != COSMO_MAC_AABB) return COSMO_E_INVALID;
+  // the reference-test walk addresses its records with 32-bit byte offsets
+  if (mac == COSMO_MAC_REF && (2 * n_tree - 1) * int64_t(sizeof(cosmo_inode_ref)) > (int64_t(1) << 32))
+    return COSMO_E_TOO_LARGE;
   const bool stats_on = (flags & (COSMO_TRAV_STATS | COSMO_TRAV_GROUP_STATS)) != 0;
   if (stats_on && !stats) return COSMO_E_INVALID;
   const int64_t lo = off;
