@@ -31,6 +31,7 @@ struct TravParams {
   float eps2;
   float half_box;
   float straddle_mu;  // AABB: nodes lighter than this pass the minimum-image guard
+  const float *straddle_group;  // AABB: each group's own straddle mass (nullable)
   int wave_stats;   // STATS: also store each wavefront's counts
   int big_bits;     // REF: bits of 2e-7 theta^2 (fixed theta; see k_traverse)
   const int32_t *order;  // launch order of the groups (nullable)
@@ -287,6 +288,12 @@ __global__ __launch_bounds__(256) void k_traverse(
     for (int h = 1; h < PPL; ++h) th2max = fmaxf(th2max, th2[h]);
     big_bits = __builtin_amdgcn_readfirstlane(__float_as_int(th2max * 2e-7f));
   }
+  // the straddle mass of this wavefront's group (one group per wavefront
+  // when it is given per group)
+  float straddle_mu = p.straddle_mu;
+  if (MAC == COSMO_MAC_AABB && p.straddle_group)
+    straddle_mu =
+        __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(p.straddle_group[wave])));
   int stk = (lane == 0) ? root : 0;  // lane k holds stack slot k
   int top = 1;
 
@@ -412,7 +419,7 @@ __global__ __launch_bounds__(256) void k_traverse(
         // only internal nodes at or above the straddle mass must pass the
         // guard (a wave-uniform condition); the others skip it
         bool inside = true;
-        if (!leaf && !(A.w < p.straddle_mu)) {
+        if (!leaf && !(A.w < straddle_mu)) {
           asm volatile("");
           inside = (d.dx + C.w < hb) && (d.dx - C.x > -hb) && (d.dy + D.x < hb) &&
                    (d.dy - C.y > -hb) && (d.dz + D.y < hb) && (d.dz - C.z > -hb);
@@ -984,8 +991,11 @@ static int traverse_impl(const float *px, const float *py, const float *pz, cons
                          const float *theta_group, double G, double soft, double box,
                          int64_t off, int64_t count, int64_t n_bodies, int mac, int flags,
                          uint64_t *stats, const int32_t *order, double straddle_mass,
-                         void *stream) {
+                         const float *straddle_group, void *stream) {
   if (!(straddle_mass >= 0.0) || (straddle_mass > 0.0 && mac != COSMO_MAC_AABB))
+    return COSMO_E_INVALID;
+  // per-group straddle masses: AABB test, one 64-particle group per wavefront
+  if (straddle_group && (mac != COSMO_MAC_AABB || (flags & COSMO_TRAV_PAIRS)))
     return COSMO_E_INVALID;
   if (n_bodies < 0 || n_bodies > kMaxBodies || off < 0 || count < 0 || n_tree < 1 ||
       n_tree > kMaxBodies)
@@ -1015,6 +1025,7 @@ static int traverse_impl(const float *px, const float *py, const float *pz, cons
   p.eps2 = sf * sf;
   p.half_box = static_cast<float>(box) * 0.5f;
   p.straddle_mu = static_cast<float>(straddle_mass);
+  p.straddle_group = straddle_group;
   p.wave_stats = (flags & COSMO_TRAV_GROUP_STATS) != 0;
   p.order = order;
   p.n_waves = (hi - lo + kWave * (pairs_of(flags) ? 2 : 1) - 1) / (kWave * (pairs_of(flags) ? 2 : 1));
@@ -1043,7 +1054,8 @@ int cosmo_traverse(const float *px, const float *py, const float *pz, const void
                    int64_t count, int64_t n_bodies, int mac, int flags, uint64_t *stats,
                    void *stream) {
   return traverse_impl(px, py, pz, nodes, n_tree, root_idx, force, theta, theta_group, G, soft,
-                       box, off, count, n_bodies, mac, flags, stats, nullptr, 0.0, stream);
+                       box, off, count, n_bodies, mac, flags, stats, nullptr, 0.0, nullptr,
+                       stream);
 }
 
 int cosmo_traverse_ordered(const float *px, const float *py, const float *pz, const void *nodes,
@@ -1053,18 +1065,20 @@ int cosmo_traverse_ordered(const float *px, const float *py, const float *pz, co
                            uint64_t *stats, const int32_t *group_order, void *stream) {
   if (!group_order || (flags & COSMO_TRAV_PAIRS)) return COSMO_E_INVALID;
   return traverse_impl(px, py, pz, nodes, n_tree, root_idx, force, theta, theta_group, G, soft,
-                       box, off, count, n_bodies, mac, flags, stats, group_order, 0.0, stream);
+                       box, off, count, n_bodies, mac, flags, stats, group_order, 0.0, nullptr,
+                       stream);
 }
 
 int cosmo_traverse_ex(const float *px, const float *py, const float *pz, const void *nodes,
                       int64_t n_tree, const int32_t *root_idx, float *force, double theta,
                       const float *theta_group, double G, double soft, double box, int64_t off,
                       int64_t count, int64_t n_bodies, int mac, int flags, uint64_t *stats,
-                      const int32_t *group_order, double straddle_mass, void *stream) {
+                      const int32_t *group_order, double straddle_mass,
+                      const float *straddle_group, void *stream) {
   if (group_order && (flags & COSMO_TRAV_PAIRS)) return COSMO_E_INVALID;
   return traverse_impl(px, py, pz, nodes, n_tree, root_idx, force, theta, theta_group, G, soft,
                        box, off, count, n_bodies, mac, flags, stats, group_order, straddle_mass,
-                       stream);
+                       straddle_group, stream);
 }
 
 int cosmo_order_groups(const float *theta_group, int64_t groups, int32_t *order, void *stream) {

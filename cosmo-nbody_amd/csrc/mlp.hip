@@ -25,7 +25,7 @@ namespace cosmo {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kIn = COSMO_MLP_IN_PAD;  // 16 (10 used)
+constexpr int kIn = COSMO_MLP_IN_PAD;  // 16 (11 used)
 constexpr int kHid = COSMO_MLP_HIDDEN; // 64
 constexpr int kOffW1 = 0;
 constexpr int kOffB1 = kOffW1 + kIn * kHid;
@@ -551,36 +551,47 @@ __global__ __launch_bounds__(256) void k_group_features(
   f[8] = log2f(1.0f + s1 * 2.3283064365386963e-10f);
 }
 
-// Rows g*K + k of the MLP input: [theta_k, features of g..., 0 pad] in BF16.
+// Rows g*K + k of the MLP input: [theta_k, features of g..., cand2_k, 0 pad]
+// in BF16 (cand2: the second coordinate of candidate k, or 0).  With fidx
+// the features of candidate k come from block fidx[k] of feat ([blocks][groups][9]).
 __global__ __launch_bounds__(256) void k_assemble_rows(const float *__restrict__ feat,
                                                        int64_t groups,
-                                                       const float *__restrict__ thetas, int K,
+                                                       const float *__restrict__ thetas,
+                                                       const float *__restrict__ cand2,
+                                                       const int32_t *__restrict__ fidx, int K,
                                                        uint16_t *__restrict__ X) {
   const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (r >= groups * K) return;
   const int64_t g = r / K;
   const int k = static_cast<int>(r % K);
+  const int64_t blk = fidx ? fidx[k] : 0;
+  const float *f = feat + (blk * groups + g) * COSMO_MLP_GROUP_FEATURES;
   uint16_t *x = X + r * kIn;
   x[0] = f2bf(thetas[k]);
 #pragma unroll
-  for (int j = 0; j < COSMO_MLP_GROUP_FEATURES; ++j)
-    x[1 + j] = f2bf(feat[g * COSMO_MLP_GROUP_FEATURES + j]);
+  for (int j = 0; j < COSMO_MLP_GROUP_FEATURES; ++j) x[1 + j] = f2bf(f[j]);
+  x[1 + COSMO_MLP_GROUP_FEATURES] = cand2 ? f2bf(cand2[k]) : uint16_t(0);
 #pragma unroll
-  for (int j = 1 + COSMO_MLP_GROUP_FEATURES; j < kIn; ++j) x[j] = 0;
+  for (int j = 2 + COSMO_MLP_GROUP_FEATURES; j < kIn; ++j) x[j] = 0;
 }
 
-// theta_g = thetas[argmin_k cost[k] + lambda * 2^pred[g*K+k]]; any non-finite
-// prediction of the group selects the fixed fallback theta.
+// k* = argmin_k cost[k] + lambda * 2^pred[g*K+k]: theta_group[g] =
+// thetas[k*] and, when given, mu_group[g] = mus[k*]; any non-finite
+// prediction of the group selects the fixed fallbacks.
 __global__ __launch_bounds__(256) void k_select_theta(const float *__restrict__ pred,
                                                       int64_t groups,
                                                       const float *__restrict__ thetas,
+                                                      const float *__restrict__ mus,
                                                       const float *__restrict__ cost, int K,
                                                       float lambda, float fallback,
+                                                      float fallback_mu,
                                                       float *__restrict__ theta_group,
+                                                      float *__restrict__ mu_group,
                                                       unsigned long long *__restrict__ n_fallback) {
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (g >= groups) return;
-  float best = INFINITY, th = fallback;
+  float best = INFINITY;
+  int kb = -1;
   bool ok = true;
   for (int k = 0; k < K; ++k) {
     const float pk = pred[g * K + k];
@@ -588,14 +599,18 @@ __global__ __launch_bounds__(256) void k_select_theta(const float *__restrict__ 
     const float score = cost[k] + lambda * exp2f(pk);
     if (score < best) {
       best = score;
-      th = thetas[k];
+      kb = k;
     }
   }
-  if (!ok || !isfinite(best)) {
-    th = fallback;
+  float th = fallback, mu = fallback_mu;
+  if (!ok || !isfinite(best) || kb < 0) {
     if (n_fallback) atomicAdd(n_fallback, 1ull);
+  } else {
+    th = thetas[kb];
+    if (mus) mu = mus[kb];
   }
   theta_group[g] = th;
+  if (mu_group) mu_group[g] = mu;
 }
 
 }  // namespace cosmo
@@ -707,25 +722,29 @@ int cosmo_group_features_dev(const float *px, const float *py, const float *pz,
                              0.0, a, n_grid, profile, nbins, feat, stream);
 }
 
-int cosmo_assemble_rows(const float *feat, int64_t groups, const float *thetas, int K,
-                        uint16_t *X, void *stream) {
+int cosmo_assemble_rows(const float *feat, int64_t groups, const float *thetas,
+                        const float *cand2, const int32_t *fidx, int K, uint16_t *X,
+                        void *stream) {
   if (groups < 0 || K < 1 || (groups && (!feat || !thetas || !X))) return COSMO_E_INVALID;
   if (groups == 0) return COSMO_OK;
-  k_assemble_rows<<<blocks_for(groups * K, 256), 256, 0, as_stream(stream)>>>(feat, groups,
-                                                                             thetas, K, X);
+  k_assemble_rows<<<blocks_for(groups * K, 256), 256, 0, as_stream(stream)>>>(
+      feat, groups, thetas, cand2, fidx, K, X);
   COSMO_CHECK_LAUNCH();
   return COSMO_OK;
 }
 
 int cosmo_select_theta(const float *pred, int64_t groups, const float *thetas,
-                       const float *cost, int K, double lambda, double fallback,
-                       float *theta_group, uint64_t *n_fallback, void *stream) {
-  if (groups < 0 || K < 1 || (groups && (!pred || !thetas || !cost || !theta_group)))
+                       const float *mus, const float *cost, int K, double lambda,
+                       double fallback, double fallback_mu, float *theta_group,
+                       float *mu_group, uint64_t *n_fallback, void *stream) {
+  if (groups < 0 || K < 1 || (groups && (!pred || !thetas || !cost || !theta_group)) ||
+      (mu_group && !mus))
     return COSMO_E_INVALID;
   if (groups == 0) return COSMO_OK;
   k_select_theta<<<blocks_for(groups, 256), 256, 0, as_stream(stream)>>>(
-      pred, groups, thetas, cost, K, static_cast<float>(lambda), static_cast<float>(fallback),
-      theta_group, reinterpret_cast<unsigned long long *>(n_fallback));
+      pred, groups, thetas, mus, cost, K, static_cast<float>(lambda),
+      static_cast<float>(fallback), static_cast<float>(fallback_mu), theta_group, mu_group,
+      reinterpret_cast<unsigned long long *>(n_fallback));
   COSMO_CHECK_LAUNCH();
   return COSMO_OK;
 }

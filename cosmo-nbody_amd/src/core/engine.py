@@ -85,6 +85,7 @@ class SimulationEngine:
         # LearnedTheta: recomputes theta_group after each build
         self.theta_selector = theta_selector
         self.group_order = None     # launch order of the groups under per-group theta
+        self.straddle_group = None  # per-group straddle masses chosen with theta_group
         self._a_dev = None          # device scale factor while a step is being captured
         self.stats = None           # int64[3] interaction / visit / pop counters, or None
         self.pairs = bool(getattr(config, "TRAVERSAL_PAIRS", False))
@@ -102,6 +103,7 @@ class SimulationEngine:
                                        self.bucket)
         if self.theta_selector is not None:
             self.theta_group = self.theta_selector.theta_groups(self, a=self._a_dev)
+            self.straddle_group = getattr(self.theta_selector, "mu_group", None)
             self.group_order = self.theta_selector.launch_order(self.theta_group)
 
     def compute_forces(self):
@@ -110,6 +112,7 @@ class SimulationEngine:
                  c.G, c.SOFTENING, c.BOX_SIZE, mac=self.mac, theta_group=self.theta_group,
                  compensated=bool(getattr(c, "COMPENSATED", False)), stats=self.stats,
                  pairs=self.pairs, straddle_mass=self.straddle_mass,
+                 straddle_group=self.straddle_group if self.theta_group is not None else None,
                  group_order=(self.group_order if self.theta_group is not None and not self.pairs
                               else None))
 

@@ -36,6 +36,50 @@ WAVE_BUDGET = 0.5      # largest single-wavefront time allowed, as a fraction of
 DEFAULT_THETAS = (0.04, 0.05, 0.06, 0.07, 0.08, 0.1, 0.12, 0.15, 0.2, 0.25, 0.3, 0.4, 0.5, 0.6,
                   0.8, 1.0, 1.4, 2.0)
 LABEL_FLOOR = -40.0   # log2 of the smallest squared-error share a label keeps
+# Candidates of the joint selection: opening angles x straddle masses (the
+# configuration's straddle mass times these factors).
+PAIR_THETAS = (0.1, 0.12, 0.15, 0.2, 0.25, 0.3, 0.4, 0.5, 0.6, 0.8, 1.0, 1.4)
+MU_FACTORS = (0.25, 0.5, 1.0, 2.0, 4.0)
+
+
+def candidates(c, thetas=None, pairs=True):
+    """(thetas, mus) of the candidates for configuration c: every theta with
+    every straddle mass c.STRADDLE_MASS * MU_FACTORS when pairs is set and the
+    AABB test has a straddle mass, else each theta with c.STRADDLE_MASS."""
+    mu0 = float(getattr(c, "STRADDLE_MASS", 0.0))
+    if pairs and mu0 > 0:
+        th = PAIR_THETAS if thetas is None else tuple(thetas)
+        return ([t for _ in MU_FACTORS for t in th],
+                [mu0 * f for f in MU_FACTORS for _ in th])
+    th = DEFAULT_THETAS if thetas is None else tuple(thetas)
+    return list(th), [mu0] * len(th)
+
+
+def feature_blocks(engine, mus_l, feat=None, prof=None, a=None):
+    """Group features computed with the straddle profile of each distinct
+    straddle mass among the candidates' (mus_l): (feat float[J, G, 9], fidx
+    list of the block of each candidate).  feat and prof (int64 [J, 2,
+    6 * PROFILE_BINS]) are reused when given."""
+    st, tm, c = engine.state, engine.tree_manager, engine.config
+    G = (st.n + 63) // 64
+    distinct = sorted(set(float(m) for m in mus_l))
+    dev = st.pos[0].device
+    if feat is None:
+        feat = torch.empty((len(distinct), G, K.GROUP_FEATURES), dtype=torch.float32,
+                           device=dev)
+    if prof is None:
+        prof = torch.empty((len(distinct), 2, 6 * K.PROFILE_BINS), dtype=torch.int64,
+                           device=dev)
+    for j, m in enumerate(distinct):
+        profile = K.straddle_profile(tm, m, prof[j, 0], prof[j, 1]) if m > 0 else None
+        K.group_features(st.pos, tm, engine.a if a is None else a, c.N_GRID, feat[j],
+                         profile=profile)
+    return feat, [distinct.index(float(m)) for m in mus_l]
+
+
+def mu_code(mu, n):
+    """MLP input of a straddle mass: log2(1 + mass in particle masses)."""
+    return math.log2(1.0 + float(mu) * float(n))
 
 
 def init_params(seed=0, device="cuda"):
@@ -165,46 +209,53 @@ class GroupProbe:
         self._force = torch.empty((self.T, 3), dtype=torch.float32, device=dev)
         self._stats = torch.zeros(3, dtype=torch.int64, device=dev)
 
-    def errors(self, theta=0.5, theta_group=None):
-        """(squared-error sum per group, interactions per target)."""
+    def errors(self, theta=0.5, theta_group=None, straddle=None, straddle_group=None):
+        """(squared-error sum per group, interactions per target); straddle:
+        the straddle mass (default the engine's), straddle_group per group."""
         e, tm, c = self.engine, self.engine.tree_manager, self.engine.config
         self._stats.zero_()
         traverse(*self.tpos, tm.nodes, tm.root_idx, self._force, theta, c.G, c.SOFTENING,
                  c.BOX_SIZE, mac=e.mac, theta_group=theta_group, stats=self._stats,
                  n_nodes=tm.n_nodes, compensated=bool(getattr(c, "COMPENSATED", False)),
-                 straddle_mass=e.straddle_mass)
+                 straddle_mass=e.straddle_mass if straddle is None else straddle,
+                 straddle_group=straddle_group)
         d = torch.zeros((self.T, 3), dtype=torch.float64, device=self.a_ref.device)
         d[self.vmask] = self._force[self.vmask].double() - self.a_ref
         err2 = (d ** 2).sum(1).reshape(len(self.groups), 64).sum(1)
         return err2, float(self._stats[0].item()) / float(self.counts.sum())
 
-    def rel_rms(self, theta=0.5, theta_group=None):
-        err2, inter = self.errors(theta, theta_group)
+    def rel_rms(self, theta=0.5, theta_group=None, straddle=None, straddle_group=None):
+        err2, inter = self.errors(theta, theta_group, straddle, straddle_group)
         return math.sqrt(float(err2.sum().item()) / self.norm), inter
 
 
-def sample(engine, n_groups, thetas=DEFAULT_THETAS, seed=0, cost_kind="time", cost=None):
-    """Training samples from the engine's current tree (built).  cost_kind:
-    "time" (ms of the whole-state traversal at each fixed theta per group)
-    or "interactions" (per particle, on the sampled groups); a given `cost`
-    array is used as is."""
+def sample(engine, n_groups, thetas=DEFAULT_THETAS, seed=0, cost_kind="time", cost=None,
+           mus=None):
+    """Training samples from the engine's current tree (built) for the
+    candidates (thetas[k], mus[k]) (mus None: the engine's straddle mass).
+    cost_kind: "time" (ms of the whole-state traversal at each candidate per
+    group) or "interactions" (per particle, on the sampled groups); a given
+    `cost` array is used as is."""
     st, tm, c = engine.state, engine.tree_manager, engine.config
     G = (st.n + 63) // 64
+    mus = [engine.straddle_mass] * len(thetas) if mus is None else list(mus)
     rng = np.random.default_rng(seed)
     groups = np.sort(rng.choice(G, size=min(n_groups, G), replace=False))
-    prof = (K.straddle_profile(tm, engine.straddle_mass) if engine.straddle_mass > 0
-            else None)
-    feat = K.group_features(st.pos, tm, engine.a, c.N_GRID, profile=prof)
+    feat, fidx = feature_blocks(engine, mus)
     th = torch.tensor(thetas, dtype=torch.float32, device=feat.device)
-    X = K.assemble_rows(feat[torch.from_numpy(groups).to(feat.device)].contiguous(), th)
+    codes = torch.tensor([mu_code(m, st.n) for m in mus], dtype=torch.float32,
+                         device=feat.device)
+    fi = torch.tensor(fidx, dtype=torch.int32, device=feat.device)
+    X = K.assemble_rows(feat[:, torch.from_numpy(groups).to(feat.device)].contiguous(), th,
+                        cand2=codes, fidx=fi)
     probe = GroupProbe(engine, groups)
     cols, measured = [], []
-    for t in thetas:
-        e2, inter = probe.errors(t)
+    for t, m in zip(thetas, mus):
+        e2, inter = probe.errors(t, straddle=m)
         cols.append(e2)
         if cost is None:
             measured.append(inter if cost_kind == "interactions"
-                            else _time_traversal(engine, t, None, reps=2) / G)
+                            else _time_traversal(engine, t, None, reps=2, straddle=m) / G)
     cost = np.asarray(measured if cost is None else cost, np.float64)
     err2 = torch.stack(cols, 1)
     mean_norm = probe.norm / probe.counts.sum()
@@ -218,54 +269,81 @@ def sample(engine, n_groups, thetas=DEFAULT_THETAS, seed=0, cost_kind="time", co
 
 
 class LearnedTheta:
-    """Selects engine.theta_group each step from the model's predictions.
-    theta_min drops the candidates below it: one group at a very small theta
-    is a single wavefront whose duration bounds the whole traversal."""
+    """Selects engine.theta_group (and, with straddle-mass candidates,
+    mu_group) each step from the model's predictions.  Candidate k is the
+    opening angle thetas[k] with the straddle mass mu_factors[k] times the
+    engine's (all 1 when mu_factors is None: opening angles only), so a
+    network keeps its meaning at another N when the configuration scales
+    its straddle mass with the particle mass.  keep (or theta_min) drops
+    candidates: one group at a very small theta is a single wavefront whose
+    duration bounds the whole traversal."""
 
-    def __init__(self, model, thetas, cost, lam, fallback, theta_min=0.0):
-        keep = [k for k, t in enumerate(thetas) if t >= theta_min]
+    def __init__(self, model, thetas, cost, lam, fallback, theta_min=0.0, mu_factors=None,
+                 keep=None):
+        factors = [1.0] * len(thetas) if mu_factors is None else list(mu_factors)
+        keep = ([k for k, t in enumerate(thetas) if t >= theta_min] if keep is None
+                else list(keep))
         self.model = model
-        self.thetas_list = [thetas[k] for k in keep]
+        self.thetas_list = [float(thetas[k]) for k in keep]
+        self.mu_factors = [float(factors[k]) for k in keep]
         self.cost_np = np.asarray(cost, np.float32)[keep]
         self.lam = float(lam)
         self.fallback = float(fallback)
+        # straddle masses are chosen per group only when the candidates differ
+        self.joint = len(set(self.mu_factors)) > 1
+        self.mu_group = None
         self._dev = None
 
-    def _ensure(self, device, G):
-        if self._dev is None or self._dev[0] != G:
+    def _ensure(self, device, G, n, mu_engine):
+        mus_l = [mu_engine * f for f in self.mu_factors]
+        if self._dev is None or self._dev[0] != G or self._dev[13] != mus_l:
             th = torch.tensor(self.thetas_list, dtype=torch.float32, device=device)
+            mus = torch.tensor(mus_l, dtype=torch.float32, device=device)
+            codes = torch.tensor([mu_code(m, n) for m in mus_l], dtype=torch.float32,
+                                 device=device)
             cost = torch.from_numpy(self.cost_np).to(device)
             Kc = len(self.thetas_list)
             X = torch.empty((G * Kc, K.IN_PAD), dtype=torch.int16, device=device)
             Y = torch.empty(G * Kc, dtype=torch.float32, device=device)
             tg = torch.empty(G, dtype=torch.float32, device=device)
+            mg = torch.empty(G, dtype=torch.float32, device=device)
             nf = torch.zeros(1, dtype=torch.int64, device=device)
-            feat = torch.empty((G, K.GROUP_FEATURES), dtype=torch.float32, device=device)
+            J = len(set(mus_l))
+            feat = torch.empty((J, G, K.GROUP_FEATURES), dtype=torch.float32, device=device)
             order = torch.empty(G, dtype=torch.int32, device=device)
-            prof = torch.empty((2, 6 * K.PROFILE_BINS), dtype=torch.int64, device=device)
-            self._dev = (G, th, cost, X, Y, tg, nf, feat, order, prof)
+            prof = torch.empty((J, 2, 6 * K.PROFILE_BINS), dtype=torch.int64, device=device)
+            distinct = sorted(set(mus_l))
+            fidx = torch.tensor([distinct.index(m) for m in mus_l], dtype=torch.int32,
+                                device=device)
+            self._dev = (G, th, cost, X, Y, tg, nf, feat, order, prof, mus, codes, mg, mus_l,
+                         fidx)
         return self._dev
 
     def theta_groups(self, engine, lam=None, a=None):
-        """Per-group theta for the engine's current tree (device float[G]).
+        """Per-group theta for the engine's current tree (device float[G]);
+        the per-group straddle masses, when chosen, are left in mu_group.
         a: the scale factor (default engine.a; a device tensor inside a
         captured step).  Allocates nothing after the first call."""
         st, tm, c = engine.state, engine.tree_manager, engine.config
         G = (st.n + 63) // 64
-        _, th, cost, X, Y, tg, nf, feat, _, prof = self._ensure(st.pos[0].device, G)
-        profile = None
-        if engine.straddle_mass > 0:
-            profile = K.straddle_profile(tm, engine.straddle_mass, prof[0], prof[1])
-        K.group_features(st.pos, tm, engine.a if a is None else a, c.N_GRID, feat,
-                         profile=profile)
-        K.assemble_rows(feat, th, X)
+        _, th, cost, X, Y, tg, nf, feat, _, prof, mus, codes, mg, mus_l, fidx = self._ensure(
+            st.pos[0].device, G, st.n, float(engine.straddle_mass))
+        feature_blocks(engine, mus_l, feat, prof, a)
+        K.assemble_rows(feat, th, X, cand2=codes, fidx=fidx)
         self.model.predict(X, Y)
-        K.select_theta(Y, G, th, cost, self.lam if lam is None else lam, self.fallback, tg, nf)
+        if self.joint:
+            K.select_theta(Y, G, th, cost, self.lam if lam is None else lam, self.fallback, tg,
+                           nf, mus=mus, fallback_mu=float(engine.straddle_mass), mu_out=mg)
+            self.mu_group = mg
+        else:
+            K.select_theta(Y, G, th, cost, self.lam if lam is None else lam, self.fallback, tg,
+                           nf)
+            self.mu_group = None
         return tg
 
     def launch_order(self, theta_group):
         """Groups by increasing theta: the traversal's launch order."""
-        order = self._ensure(theta_group.device, theta_group.shape[0])[8]
+        order = self._dev[8]
         return order_groups(theta_group, order)
 
     @property
@@ -274,7 +352,8 @@ class LearnedTheta:
 
     def save(self, path):
         np.savez(path, params=self.model.params.cpu().numpy(), thetas=np.array(self.thetas_list),
-                 cost=self.cost_np, lam=self.lam, fallback=self.fallback)
+                 mu_factors=np.array(self.mu_factors), cost=self.cost_np, lam=self.lam,
+                 fallback=self.fallback)
 
     @classmethod
     def load(cls, path, device="cuda"):
@@ -284,22 +363,26 @@ class LearnedTheta:
             model = Ensemble([ThetaModel(q, device=device) for q in p])
         else:
             model = ThetaModel(p, device=device)
-        return cls(model, d["thetas"].tolist(), d["cost"], float(d["lam"]), float(d["fallback"]))
+        factors = d["mu_factors"].tolist() if "mu_factors" in d.files else None
+        return cls(model, d["thetas"].tolist(), d["cost"], float(d["lam"]), float(d["fallback"]),
+                   mu_factors=factors)
 
 
-def _time_traversal(engine, theta, theta_group, reps=5, group_order=None):
+def _time_traversal(engine, theta, theta_group, reps=5, group_order=None, straddle=None,
+                    straddle_group=None):
     """Average HIP-event time (ms) of the full traversal (all particles)."""
     st, tm, c = engine.state, engine.tree_manager, engine.config
     comp = bool(getattr(c, "COMPENSATED", False))
+    mu = engine.straddle_mass if straddle is None else straddle
     traverse(*st.pos, tm.nodes, tm.root_idx, st.force, theta, c.G, c.SOFTENING, c.BOX_SIZE,
              mac=engine.mac, theta_group=theta_group, group_order=group_order,
-             compensated=comp, straddle_mass=engine.straddle_mass)
+             compensated=comp, straddle_mass=mu, straddle_group=straddle_group)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(reps):
         traverse(*st.pos, tm.nodes, tm.root_idx, st.force, theta, c.G, c.SOFTENING, c.BOX_SIZE,
                  mac=engine.mac, theta_group=theta_group, group_order=group_order,
-                 compensated=comp, straddle_mass=engine.straddle_mass)
+                 compensated=comp, straddle_mass=mu, straddle_group=straddle_group)
     e1.record()
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps
@@ -370,14 +453,17 @@ def default_states(c):
     return [(c.SEED, 0), (c.SEED, 10), (c.SEED, 20)], (c.SEED, 30)
 
 
-def train(c, states, n_groups, thetas, fit_steps, seed=0, log=print, n_models=N_MODELS):
-    """Samples from the training states (cost of each theta timed on the
-    first), trains n_models networks (an Ensemble when more than one);
-    returns (model, cost, info)."""
+def train(c, states, n_groups, thetas, fit_steps, seed=0, log=print, n_models=N_MODELS,
+          mus=None):
+    """Samples from the training states for the candidates (thetas[k],
+    mus[k]) (cost of each candidate timed on the first state), trains
+    n_models networks (an Ensemble when more than one); returns (model,
+    cost, info)."""
     Xs, Ts, costs = [], [], []
     for k, (sd, steps) in enumerate(states):
         eng = _engine_at(c, sd, steps)
-        smp = sample(eng, n_groups, thetas, seed=seed + k, cost=costs[0] if costs else None)
+        smp = sample(eng, n_groups, thetas, seed=seed + k, cost=costs[0] if costs else None,
+                     mus=mus)
         Xs.append(smp.X)
         Ts.append(smp.T)
         costs.append(smp.cost)
@@ -403,32 +489,37 @@ def train(c, states, n_groups, thetas, fit_steps, seed=0, log=print, n_models=N_
     return model, np.mean(np.stack(costs), 0), info
 
 
-def measure_costs(c, state, thetas, log=print):
-    """ms per group of a whole traversal of `state` at each fixed theta."""
+def measure_costs(c, state, thetas, log=print, mus=None):
+    """ms per group of a whole traversal of `state` at each candidate."""
     eng = _engine_at(c, *state)
     G = (eng.state.n + 63) // 64
+    mus = [eng.straddle_mass] * len(thetas) if mus is None else list(mus)
     out = []
-    for t in thetas:
-        out.append(_time_traversal(eng, t, None, reps=2) / G)
-        log(f"cost at theta {t}: {out[-1] * G:.1f} ms")
+    for t, m in zip(thetas, mus):
+        out.append(_time_traversal(eng, t, None, reps=2, straddle=m) / G)
+        log(f"cost at theta {t}, straddle {m:.3g}: {out[-1] * G:.1f} ms")
     return np.asarray(out)
 
 
 def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0,
-             target_theta=None, lambdas=None, log=print, graph=False):
-    """Fixed theta against learned per-group theta on the held-out state
-    `val` at equal relative RMS error (FP64 direct sum on sampled groups):
-    the fixed theta meeting the target is the largest candidate within it,
-    refined by bisection; lambda is swept
-    on a log grid, the cheapest settings meeting the target by the cost model
+             target_theta=None, lambdas=None, log=print, graph=False, mus=None):
+    """Fixed theta against the learned per-group selection on the held-out
+    state `val` at equal relative RMS error (FP64 direct sum on sampled
+    groups).  The candidates are (thetas[k], mus[k]) (mus None: the
+    configuration's straddle mass for all).  The fixed method keeps the
+    configuration's straddle mass; its theta is the largest one within the
+    target on the candidate thetas, refined by bisection.  lambda is swept on
+    a log grid, the cheapest settings meeting the target by the cost model
     are timed and the fastest is kept.  Reports traversal and full-step
-    times, interactions and the hindsight bound (per group the theta of least
-    cost + lambda * measured error).  theta* and lambda are tuned on one
-    sample of groups; both errors are then measured again on a disjoint
+    times, interactions and the hindsight bound (per group the candidate of
+    least cost + lambda * measured error).  theta* and lambda are tuned on
+    one sample of groups; both errors are then measured again on a disjoint
     sample of as many groups (err_*_confirm).  Returns (result, LearnedTheta),
     or (result, None) when the target cannot be met."""
     eng = _engine_at(c, *val)
     G = (eng.state.n + 63) // 64
+    mu0 = eng.straddle_mass
+    mus = [mu0] * len(thetas) if mus is None else list(mus)
     rng = np.random.default_rng(seed + 12345)
     # the tuning sample (theta*, lambda) and a disjoint confirmation sample on
     # which both errors are measured again
@@ -437,11 +528,15 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
     vgroups = np.sort(drawn[:nv])
     cgroups = np.sort(drawn[nv:2 * nv])
     probe = GroupProbe(eng, vgroups)
-    err2_k, fixed_curve = [], []
-    for t in thetas:
-        e2, inter = probe.errors(t)
+    err2_k = []
+    for t, m in zip(thetas, mus):
+        e2, _ = probe.errors(t, straddle=m)
         err2_k.append(e2.cpu().numpy())
-        fixed_curve.append((t, math.sqrt(float(e2.sum().item()) / probe.norm), inter))
+    th_fixed = sorted(set(float(t) for t in thetas))
+    fixed_curve = []
+    for t in th_fixed:
+        e, inter = probe.rel_rms(t, straddle=mu0)
+        fixed_curve.append((t, e, inter))
     log("fixed theta curve (theta, rel_rms, interactions): " +
         ", ".join(f"({t:.2f}, {e:.2e}, {i:.0f})" for t, e, i in fixed_curve))
     if target_theta is not None:
@@ -455,9 +550,9 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
         return {"reachable": False, "val_state": val, "target": target,
                 "fixed_curve": fixed_curve}, None
     k_ok = max(ok_c)
-    theta_star = float(thetas[k_ok])
-    if k_ok + 1 < len(thetas):
-        t_b = _bisect(lambda t: probe.rel_rms(t)[0], thetas[k_ok], thetas[k_ok + 1], target,
+    theta_star = float(th_fixed[k_ok])
+    if k_ok + 1 < len(th_fixed):
+        t_b = _bisect(lambda t: probe.rel_rms(t)[0], th_fixed[k_ok], th_fixed[k_ok + 1], target,
                       iters=14)
         theta_star = t_b if t_b is not None else theta_star
     e_fixed, i_fixed = probe.rel_rms(theta_star)
@@ -475,28 +570,47 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
             bound = (float(lam), mc)
     # Candidates whose average single-wavefront time would exceed half the
     # fixed-theta traversal are dropped: the traversal cannot end before its
-    # slowest wavefront.  All groups at theta_k take cost_k * G ms with
+    # slowest wavefront.  All groups at candidate k take cost_k * G ms with
     # `concurrent` wavefronts resident at a time.
     props = torch.cuda.get_device_properties(eng.state.pos[0].device)
     concurrent = min(props.multi_processor_count * 4 * WAVES_PER_SIMD, G)
     wave_ms = cost_arr * concurrent
-    ok_k = [k for k in range(len(thetas)) if wave_ms[k] <= WAVE_BUDGET * t_fixed]
-    theta_min = float(thetas[ok_k[0]]) if ok_k else float(thetas[-1])
-    lt = LearnedTheta(model, thetas, cost, 1.0, theta_star, theta_min=theta_min)
+    keep = [k for k in range(len(thetas)) if wave_ms[k] <= WAVE_BUDGET * t_fixed]
+    if not keep:
+        keep = [int(np.argmin(wave_ms))]
+    theta_min = float(min(thetas[k] for k in keep))
+    # the same bound over the candidates the selection keeps
+    bound_kept = None
+    Ek, ck = E[:, keep], cost_arr[keep]
+    for lam in np.logspace(-6, 16, 221):
+        k = np.argmin(ck[None, :] + lam * Ek / probe.norm, axis=1)
+        e = math.sqrt(Ek[np.arange(len(k)), k].sum() / probe.norm)
+        mc = float(ck[k].mean())
+        if e <= target and (bound_kept is None or mc < bound_kept[1]):
+            bound_kept = (float(lam), mc)
+    lt = LearnedTheta(model, thetas, cost, 1.0, theta_star, keep=keep,
+                      mu_factors=[m / mu0 for m in mus] if mu0 > 0 else None)
     vg = torch.from_numpy(vgroups).to(eng.state.pos[0].device)
-    th_np = np.asarray(lt.thetas_list, np.float32)
     cost_sel = np.asarray(lt.cost_np, np.float64)
+    pairs_sel = list(zip(lt.thetas_list, [mu0 * f for f in lt.mu_factors]))
 
-    def model_cost(tg):
-        k = np.clip(np.searchsorted(th_np, tg.cpu().numpy()), 0, len(th_np) - 1)
-        return float(cost_sel[k].mean())
+    def sub(t):
+        return None if t is None else t.index_select(0, vg).contiguous()
+
+    def model_cost(tg, mg):
+        tg = tg.cpu().numpy()
+        mg = np.full_like(tg, mu0) if mg is None else mg.cpu().numpy()
+        idx = {(np.float32(a), np.float32(b)): k for k, (a, b) in enumerate(pairs_sel)}
+        return float(np.mean([cost_sel[idx.get((np.float32(a), np.float32(b)), 0)]
+                              for a, b in zip(tg, mg)]))
 
     lambdas = np.logspace(-2, 14, 129) if lambdas is None else lambdas
     sweep = []
     for j, lam in enumerate(lambdas):
         tg = lt.theta_groups(eng, float(lam))
-        e, inter = probe.rel_rms(0.5, tg.index_select(0, vg).contiguous())
-        sweep.append((float(lam), e, inter, model_cost(tg)))
+        mg = lt.mu_group
+        e, inter = probe.rel_rms(0.5, sub(tg), straddle=mu0, straddle_group=sub(mg))
+        sweep.append((float(lam), e, inter, model_cost(tg, mg)))
         if j % 32 == 31:
             log(f"lambda sweep {j + 1}/{len(lambdas)}")
     ok = sorted((sw for sw in sweep if sw[1] <= target), key=lambda sw: sw[3])
@@ -506,22 +620,28 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
     best = None
     for lam, _, _, _ in ok[:5]:  # the cheapest settings by the cost model, timed
         tg = lt.theta_groups(eng, lam).clone()
-        t = _time_traversal(eng, 0.5, tg, group_order=lt.launch_order(tg).clone())
+        mg = None if lt.mu_group is None else lt.mu_group.clone()
+        t = _time_traversal(eng, 0.5, tg, group_order=lt.launch_order(tg).clone(),
+                            straddle_group=mg)
         log(f"lambda {lam:.3g}: learned traversal {t:.1f} ms")
         if best is None or t < best[1]:
             best = (lam, t)
     lt.lam = best[0]
     tg = lt.theta_groups(eng).clone()
-    e_learned, i_learned = probe.rel_rms(0.5, tg.index_select(0, vg).contiguous())
+    mg = None if lt.mu_group is None else lt.mu_group.clone()
+    e_learned, i_learned = probe.rel_rms(0.5, sub(tg), straddle=mu0, straddle_group=sub(mg))
     cprobe = GroupProbe(eng, cgroups)
     cg = torch.from_numpy(cgroups).to(eng.state.pos[0].device)
     e_fixed_c, _ = cprobe.rel_rms(theta_star)
-    e_learned_c, _ = cprobe.rel_rms(0.5, tg.index_select(0, cg).contiguous())
+    e_learned_c, _ = cprobe.rel_rms(
+        0.5, tg.index_select(0, cg).contiguous(), straddle=mu0,
+        straddle_group=None if mg is None else mg.index_select(0, cg).contiguous())
     log(f"confirmation sample ({len(cgroups)} other groups): fixed {e_fixed_c:.4e}, "
         f"learned {e_learned_c:.4e}")
     del cprobe
-    t_learned_unordered = _time_traversal(eng, 0.5, tg)
-    t_learned = _time_traversal(eng, 0.5, tg, group_order=lt.launch_order(tg).clone())
+    t_learned_unordered = _time_traversal(eng, 0.5, tg, straddle_group=mg)
+    t_learned = _time_traversal(eng, 0.5, tg, group_order=lt.launch_order(tg).clone(),
+                                straddle_group=mg)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(5):
@@ -533,11 +653,18 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
     log(f"fixed-theta step: {step_fixed:.1f} ms")
     step_learned = _time_steps(c, val, lt, graph=graph)
     log(f"learned-theta step: {step_learned:.1f} ms")
-    hist_t = np.histogram(tg.cpu().numpy(), bins=np.concatenate([th_np - 1e-6, [np.inf]]))[0]
+    tg_np = tg.cpu().numpy()
+    hist_t = {float(t): int(np.sum(np.isclose(tg_np, t))) for t in sorted(set(lt.thetas_list))}
+    hist_mu = None
+    if mg is not None:
+        mg_np = mg.cpu().numpy()
+        hist_mu = {float(mu0 * f): int(np.sum(np.isclose(mg_np, mu0 * f)))
+                   for f in sorted(set(lt.mu_factors))}
     res = {"reachable": True, "preset": c.NAME, "mac": c.MAC, "target": target,
            "bucket": int(getattr(c, "BUCKET", 0)),
            "compensated": bool(getattr(c, "COMPENSATED", False)),
            "straddle_mass": float(getattr(c, "STRADDLE_MASS", 0.0)),
+           "joint_straddle": bool(lt.joint), "candidates": len(lt.thetas_list),
            "n": eng.state.n, "val_state": val, "val_groups": int(len(vgroups)),
            "hip_graph_steps": bool(graph),
            "theta_fixed": theta_star, "err_fixed": e_fixed,
@@ -552,30 +679,34 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
            "step_ms_fixed": step_fixed, "step_ms_learned": step_learned,
            "speedup_step": step_fixed / step_learned,
            "hindsight_bound_speedup": ((t_fixed / G) / bound[1] if bound else None),
+           "hindsight_bound_kept": ((t_fixed / G) / bound_kept[1] if bound_kept else None),
            "interaction_ratio": i_fixed / i_learned,
            "theta_group_mean": float(tg.mean().item()), "theta_min": theta_min,
-           "theta_group_histogram": dict(zip([float(t) for t in lt.thetas_list],
-                                             hist_t.tolist())),
+           "theta_group_histogram": hist_t, "straddle_group_histogram": hist_mu,
            "fallbacks": lt.n_fallback}
     return res, lt
 
 
 def experiment(preset="repo_128", target=1e-3, mac="aabb", states=None, n_groups=2048,
-               n_val_groups=1024, fit_steps=20000, seed=0, thetas=DEFAULT_THETAS,
-               target_theta=None, lambdas=None, log=print, overrides=None, val_states=None):
+               n_val_groups=1024, fit_steps=20000, seed=0, thetas=None,
+               target_theta=None, lambdas=None, log=print, overrides=None, val_states=None,
+               pairs=True):
     """Trains the network on the GPU from samples the method generates on the
     training states, then evaluates it (see evaluate) on each held-out
-    state.  Returns [(result, LearnedTheta or None)] per held-out state and
-    the training info."""
+    state.  The candidates are opening angles, and with pairs (and a
+    straddle mass) opening angles x straddle masses (see candidates()).
+    Returns [(result, LearnedTheta or None)] per held-out state and the
+    training info."""
     from src.utils import config as cfg
     c = cfg.preset(preset, MAC=mac, **(overrides or {}))
+    th, mus = candidates(c, thetas, pairs)
     train_states, val = default_states(c) if states is None else states
     val_states = [val] if val_states is None else list(val_states)
-    model, cost, info = train(c, train_states, n_groups, thetas, fit_steps, seed, log)
+    model, cost, info = train(c, train_states, n_groups, th, fit_steps, seed, log, mus=mus)
     out = []
     for v in val_states:
-        res, lt = evaluate(c, model, cost, thetas, v, target, n_val_groups, seed, target_theta,
-                           lambdas, log)
+        res, lt = evaluate(c, model, cost, th, v, target, n_val_groups, seed, target_theta,
+                           lambdas, log, mus=mus)
         res.update(info)
         out.append((res, lt))
     return out
@@ -609,6 +740,8 @@ def main():
                     help="comma-separated candidate thetas for --eval-preset")
     ap.add_argument("--graph", action="store_true",
                     help="time full steps replayed from HIP graphs (with the density splat)")
+    ap.add_argument("--theta-only", action="store_true",
+                    help="candidates are opening angles only (no per-group straddle mass)")
     ap.add_argument("--save", default="")
     args = ap.parse_args()
     over = {}
@@ -626,18 +759,21 @@ def main():
     vals = [val] + [(int(x), 0) for x in args.val_seeds.split(",") if x]
     if args.eval_preset:
         log = lambda m: print(m, flush=True)  # noqa: E731
-        model, _, info = train(c, train_states, args.groups, DEFAULT_THETAS, args.fit_steps,
-                               log=log)
+        pairs = not args.theta_only
+        th_t, mus_t = candidates(c, None, pairs)
+        model, _, info = train(c, train_states, args.groups, th_t, args.fit_steps, log=log,
+                               mus=mus_t)
         over_e = dict(over)
         if args.eval_straddle is not None:
             over_e["STRADDLE_MASS"] = args.eval_straddle
         ce = cfg.preset(args.eval_preset, MAC=args.mac, **over_e)
         th = (tuple(float(x) for x in args.eval_thetas.split(",")) if args.eval_thetas
-              else DEFAULT_THETAS)
-        cost = measure_costs(ce, vals[0], th, log)
+              else None)
+        th, mus = candidates(ce, th, pairs)
+        cost = measure_costs(ce, vals[0], th, log, mus=mus)
         for v in vals:
             res, _ = evaluate(ce, model, cost, th, v, args.target, args.val_groups, log=log,
-                              graph=args.graph)
+                              graph=args.graph, mus=mus)
             res.update(info)
             res["train_preset"] = args.preset
             print(json.dumps(res, default=float), flush=True)
@@ -645,7 +781,8 @@ def main():
     out = experiment(args.preset, args.target, args.mac, fit_steps=args.fit_steps,
                      n_groups=args.groups, n_val_groups=args.val_groups,
                      target_theta=args.target_theta, log=lambda m: print(m, flush=True),
-                     overrides=over, val_states=vals, states=(train_states, val))
+                     overrides=over, val_states=vals, states=(train_states, val),
+                     pairs=not args.theta_only)
     for res, lt in out:
         print(json.dumps(res, default=float), flush=True)
     if args.save and out[0][1] is not None:

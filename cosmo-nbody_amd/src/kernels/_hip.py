@@ -75,7 +75,7 @@ SIGNATURES = {
     "cosmo_traverse_ordered": ([_p] * 4 + [_i64, _p, _p, _f64, _p, _f64, _f64, _f64, _i64, _i64,
                                             _i64, _int, _int, _p, _p, _p], _int),
     "cosmo_traverse_ex": ([_p] * 4 + [_i64, _p, _p, _f64, _p, _f64, _f64, _f64, _i64, _i64,
-                                       _i64, _int, _int, _p, _p, _f64, _p], _int),
+                                       _i64, _int, _int, _p, _p, _f64, _p, _p], _int),
     "cosmo_order_groups": ([_p, _i64, _p, _p], _int),
     "cosmo_compute_forces": ([_p] * 10 + [_f64] * 4 + [_p, _p, _p, _i64, _i64, _i64, _p], _int),
     "cosmo_integrate": ([_p] * 7 + [_f64, _f64, _f64, _i64, _p], _int),
@@ -90,12 +90,13 @@ SIGNATURES = {
     "cosmo_straddle_profile": ([_p] * 4 + [_i64, _f64, _p, _p, _int, _p], _int),
     "cosmo_group_features": ([_p] * 9 + [_i64, _f64, _f64, _p, _int, _p, _p], _int),
     "cosmo_group_features_dev": ([_p] * 9 + [_i64, _p, _f64, _p, _int, _p, _p], _int),
-    "cosmo_assemble_rows": ([_p, _i64, _p, _int, _p, _p], _int),
+    "cosmo_assemble_rows": ([_p, _i64, _p, _p, _p, _int, _p, _p], _int),
     "cosmo_mlp_forward": ([_p, _i64, _p, _p, _p, _p], _int),
     "cosmo_mlp_grad": ([_p, _p, _i64, _p, _p, _p, _int, _p, _p], _int),
     "cosmo_mlp_adam": ([_p, _p, _p, _p, _f32, _f32, _f32, _f32, _int, _p, _p], _int),
     "cosmo_mlp_to_bf16": ([_p, _p, _p], _int),
-    "cosmo_select_theta": ([_p, _i64, _p, _p, _int, _f64, _f64, _p, _p, _p], _int),
+    "cosmo_select_theta": ([_p, _i64, _p, _p, _p, _int, _f64, _f64, _f64, _p, _p, _p, _p],
+                           _int),
 }
 
 _lib = None

@@ -10,7 +10,7 @@ import torch
 
 from ._hip import check, load, ptr, stream_handle
 
-IN, IN_PAD, HIDDEN, GROUP_FEATURES = 10, 16, 64, 9
+IN, IN_PAD, HIDDEN, GROUP_FEATURES = 11, 16, 64, 9
 PARAMS = 16 * 64 + 64 + 64 * 64 + 64 + 64 + 1
 BF16_WEIGHTS = 16 * 64 + 64 * 64 + 64
 GRAD_BLOCKS = 64
@@ -69,13 +69,21 @@ def group_features(pos, tree, a, n_grid, out=None, profile=None, nbins=PROFILE_B
     return out
 
 
-def assemble_rows(feat, thetas, out=None):
-    G = feat.shape[0]
+def assemble_rows(feat, thetas, out=None, cand2=None, fidx=None):
+    """MLP input rows [theta_k, features, cand2_k] for every group and
+    candidate k (cand2: the candidates' straddle-mass codes, or None: 0).
+    feat: float32 [groups, 9], or [blocks, groups, 9] with fidx (int32 [K]):
+    candidate k takes its features from block fidx[k]."""
+    G = feat.shape[-2]
     K = thetas.shape[0]
     out = torch.empty((G * K, IN_PAD), dtype=_I16, device=feat.device) if out is None else out
-    check(load().cosmo_assemble_rows(ptr(feat, _F32, "feat"), G, ptr(thetas, _F32, "thetas"), K,
-                                     ptr(out, _I16, "X", numel=G * K * IN_PAD), stream_handle()),
-          "assemble_rows")
+    c2 = None if cand2 is None else ptr(cand2, _F32, "cand2", numel=K)
+    fi = None if fidx is None else ptr(fidx, torch.int32, "fidx", numel=K)
+    if fidx is None and feat.dim() != 2:
+        raise ValueError("assemble_rows: blocks of features need fidx")
+    check(load().cosmo_assemble_rows(ptr(feat, _F32, "feat"), G, ptr(thetas, _F32, "thetas"), c2,
+                                     fi, K, ptr(out, _I16, "X", numel=G * K * IN_PAD),
+                                     stream_handle()), "assemble_rows")
     return out
 
 
@@ -119,12 +127,21 @@ def to_bf16(params, wbf):
           "mlp_to_bf16")
 
 
-def select_theta(pred, groups, thetas, cost, lam, fallback, out, n_fallback=None):
+def select_theta(pred, groups, thetas, cost, lam, fallback, out, n_fallback=None, mus=None,
+                 fallback_mu=0.0, mu_out=None):
+    """Per-group theta (and, with mus and mu_out, straddle mass) of the
+    candidate minimising cost + lam * 2^pred."""
     K = thetas.shape[0]
+    if (mus is None) != (mu_out is None):
+        raise ValueError("select_theta: mus and mu_out go together")
     check(load().cosmo_select_theta(ptr(pred, _F32, "pred", numel=groups * K), int(groups),
-                                    ptr(thetas, _F32, "thetas"), ptr(cost, _F32, "cost", numel=K),
-                                    K, float(lam), float(fallback),
+                                    ptr(thetas, _F32, "thetas", numel=K),
+                                    None if mus is None else ptr(mus, _F32, "mus", numel=K),
+                                    ptr(cost, _F32, "cost", numel=K), K, float(lam),
+                                    float(fallback), float(fallback_mu),
                                     ptr(out, _F32, "theta_group", numel=groups),
+                                    None if mu_out is None else
+                                    ptr(mu_out, _F32, "mu_group", numel=groups),
                                     None if n_fallback is None else
                                     ptr(n_fallback, (torch.int64, torch.uint64), "n_fallback"),
                                     stream_handle()), "select_theta")

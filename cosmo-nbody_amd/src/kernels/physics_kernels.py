@@ -71,7 +71,8 @@ def compute_forces(pos_x, pos_y, pos_z, children, n_mass, n_com, n_min, n_max, f
 
 def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0, count=None,
              mac=MAC_REF, theta_group=None, compensated=False, stats=None, n_nodes=None,
-             pairs=False, group_stats=False, group_order=None, straddle_mass=0.0):
+             pairs=False, group_stats=False, group_order=None, straddle_mass=0.0,
+             straddle_group=None):
     """Traversal over packed node records; stats: int64[3] tensor accumulating
     (interactions, node visits, wavefront stack pops) when given; with
     group_stats it has 3 + 3*W entries and entries 3+3w.. hold wavefront group
@@ -80,7 +81,8 @@ def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0
     lane (128-particle groups).  group_order: int32 launch order of the
     64-particle groups (a permutation, e.g. from order_groups).
     straddle_mass (AABB test): nodes lighter than straddle_mass pass the
-    minimum-image guard (0 = exact guard)."""
+    minimum-image guard (0 = exact guard); straddle_group (float32 per
+    64-particle group) gives each group its own."""
     n = pos_x.shape[0]
     n_nodes = 2 * n - 1 if n_nodes is None else int(n_nodes)
     n_tree = (n_nodes + 1) // 2
@@ -101,7 +103,9 @@ def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0
             int(mac), flags, st)
     order = None if group_order is None else ptr(group_order, torch.int32, "group_order",
                                                  numel=max(groups, 0))
-    check(load().cosmo_traverse_ex(*args, order, float(straddle_mass), stream_handle()),
+    sg = None if straddle_group is None else ptr(straddle_group, _F32, "straddle_group",
+                                                 numel=max(groups, 0))
+    check(load().cosmo_traverse_ex(*args, order, float(straddle_mass), sg, stream_handle()),
           "traverse")
 
 

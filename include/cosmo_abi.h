@@ -301,14 +301,17 @@ int cosmo_traverse_ordered(const float *pos_x, const float *pos_y,
  * minimum-image boundary still passes the test when its FP32 mass is below
  * fl(straddle_mass) (0 = the exact guard of cosmo_traverse).  Such a node
  * puts at most its mass on the wrong image, so the limit bounds the error
- * it adds. */
+ * it adds.  straddle_group (nullable, device float[ceil(count/64)], AABB
+ * test, not with COSMO_TRAV_PAIRS) gives each group its own straddle mass
+ * in place of straddle_mass. */
 int cosmo_traverse_ex(const float *pos_x, const float *pos_y, const float *pos_z,
                       const void *nodes, int64_t n_tree, const int32_t *root_idx,
                       float *force, double theta, const float *theta_group,
                       double G, double soft, double box, int64_t off,
                       int64_t count, int64_t n_bodies, int mac, int flags,
                       uint64_t *stats, const int32_t *group_order,
-                      double straddle_mass, void *stream);
+                      double straddle_mass, const float *straddle_group,
+                      void *stream);
 
 /* order[0..groups) = the groups sorted by increasing theta_group (64
  * logarithmic bins; arbitrary order inside a bin): the launch order for

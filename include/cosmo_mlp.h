@@ -5,15 +5,16 @@
  *
  * A group is 64 Morton-consecutive particles (one wavefront of the
  * traversal).  For every group, cosmo_group_features computes nine
- * features; the MLP input row is [theta, features..., 0-padding to 16].
- * The network 10 -> 64 -> 64 -> 1 (ReLU, BF16 weights, FP32 accumulation on
+ * features; the MLP input row is [theta, features..., mu, 0-padding to 16]
+ * for a candidate (theta, mu) of opening angle and straddle mass code.
+ * The network 11 -> 64 -> 64 -> 1 (ReLU, BF16 weights, FP32 accumulation on
  * MFMA) predicts log2 of the group's mean squared acceleration error
  * relative to the mean squared acceleration of all particles.
- * cosmo_select_theta picks, per group, the candidate theta minimising
- * cost[k] + lambda * 2^prediction, with a fixed theta as fallback.
+ * cosmo_select_theta picks, per group, the candidate minimising
+ * cost[k] + lambda * 2^prediction, with fixed values as fallback.
  *
  * Parameters: one FP32 vector of COSMO_MLP_PARAMS values laid out as
- * W1[16][64] (rows >= 10 zero), b1[64], W2[64][64], b2[64], W3[64], b3, and
+ * W1[16][64] (rows >= 11 zero), b1[64], W2[64][64], b2[64], W3[64], b3, and
  * a BF16 copy of W1|W2|W3 (COSMO_MLP_BF16_WEIGHTS values) read by the
  * kernels.  Weights are row-major [inputs][outputs].
  */
@@ -26,7 +27,7 @@
 extern "C" {
 #endif
 
-#define COSMO_MLP_IN 10
+#define COSMO_MLP_IN 11
 #define COSMO_MLP_IN_PAD 16
 #define COSMO_MLP_HIDDEN 64
 #define COSMO_MLP_GROUP_FEATURES 9
@@ -73,9 +74,13 @@ int cosmo_group_features_dev(const float *pos_x, const float *pos_y,
                              double n_grid, const uint64_t *profile, int nbins,
                              float *feat, void *stream);
 
-/* X[g*K + k][0..15] = BF16 [thetas[k], feat[g][0..8], 0...]. */
+/* X[g*K + k][0..15] = BF16 [thetas[k], F[g][0..8], cand2[k], 0...] with
+ * F = feat (float[groups][9]), or, when fidx is given, block fidx[k] of feat
+ * (float[blocks][groups][9]: the features computed with each candidate's
+ * straddle profile).  cand2 nullable: 0. */
 int cosmo_assemble_rows(const float *feat, int64_t groups,
-                        const float *thetas, int K, uint16_t *X, void *stream);
+                        const float *thetas, const float *cand2,
+                        const int32_t *fidx, int K, uint16_t *X, void *stream);
 
 /* Y[r] = MLP(X[r]) for rows r < rows; X is BF16 [rows][16]. */
 int cosmo_mlp_forward(const uint16_t *X, int64_t rows, const uint16_t *wbf,
@@ -98,12 +103,15 @@ int cosmo_mlp_adam(float *params, float *m, float *v, const float *grad,
 /* BF16 copy of the weights of an FP32 parameter vector. */
 int cosmo_mlp_to_bf16(const float *params, uint16_t *wbf, void *stream);
 
-/* theta_group[g] = thetas[argmin_k cost[k] + lambda * 2^pred[g*K+k]]; a
- * group with any non-finite prediction gets `fallback` and increments
- * *n_fallback (nullable, device uint64). */
+/* With k* = argmin_k cost[k] + lambda * 2^pred[g*K+k]: theta_group[g] =
+ * thetas[k*] and, when mu_group is given, mu_group[g] = mus[k*] (the
+ * candidates' straddle masses).  A group with any non-finite prediction
+ * gets `fallback` and `fallback_mu` and increments *n_fallback (nullable,
+ * device uint64). */
 int cosmo_select_theta(const float *pred, int64_t groups, const float *thetas,
-                       const float *cost, int K, double lambda, double fallback,
-                       float *theta_group, uint64_t *n_fallback, void *stream);
+                       const float *mus, const float *cost, int K, double lambda,
+                       double fallback, double fallback_mu, float *theta_group,
+                       float *mu_group, uint64_t *n_fallback, void *stream);
 
 #ifdef __cplusplus
 }

@@ -628,8 +628,9 @@ void orc_compute_forces(const float *px, const float *py, const float *pz,
                         double G, double soft, double box, int32_t root,
                         int64_t off, int64_t count, int64_t n, int precision,
                         int mac, const float *theta_g, int bucket,
-                        double straddle_mass, const int64_t *targets,
-                        int64_t *n_inter, int64_t *n_visit, double *force64) {
+                        double straddle_mass, const float *straddle_g,
+                        const int64_t *targets, int64_t *n_inter, int64_t *n_visit,
+                        double *force64) {
   int64_t hi = off + count;
   if (hi > n) hi = n;
   if (targets) { /* the listed particles; output row t is targets[t] */
@@ -651,14 +652,14 @@ void orc_compute_forces(const float *px, const float *py, const float *pz,
     if (precision == 64) {
       const double th = theta_g ? (double)theta_g[i] : theta;
       bh_one_f64(px, py, pz, children, n_mass, n_com, n_min, n_max, mac, th, G,
-                 soft, box, root, i, bfirst, bcount, bucket, (float)straddle_mass,
+                 soft, box, root, i, bfirst, bcount, bucket, straddle_g ? straddle_g[i] : (float)straddle_mass,
                  a, &ni, &nv);
     } else {
       const float th = theta_g ? theta_g[i] : (float)theta;
       const double th2d = theta_g ? (double)th * (double)th : theta * theta;
       bh_one_f32(px, py, pz, children, n_mass, n_com, n_min, n_max, mac, th,
                  th2d, (float)G, (float)soft, (float)box, root, i,
-                 precision == 31, bfirst, bcount, bucket, (float)straddle_mass,
+                 precision == 31, bfirst, bcount, bucket, straddle_g ? straddle_g[i] : (float)straddle_mass,
                  a, &ni, &nv);
     }
     force[3 * t + 0] = (float)a[0];

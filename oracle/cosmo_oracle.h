@@ -95,6 +95,7 @@ int orc_validate_tree(const int32_t *children, const int32_t *parents,
  * bucket: ORC_MAC_AABB only (0 = off): a rejected internal node of at most
  *   `bucket` leaves is not opened; its leaves are visited in leaf order
  *   (no reference counterpart).
+ * straddle_g: optional per-particle straddle mass (in place of straddle_mass).
  * targets: optional list of count particle indices (off is then ignored);
  *   output row t (force, counts, force64) belongs to particle targets[t].
  * n_inter / n_visit: optional per-particle counts of accepted interactions
@@ -106,8 +107,9 @@ void orc_compute_forces(const float *px, const float *py, const float *pz,
                         double G, double soft, double box, int32_t root,
                         int64_t off, int64_t count, int64_t n, int precision,
                         int mac, const float *theta_g, int bucket,
-                        double straddle_mass, const int64_t *targets,
-                        int64_t *n_inter, int64_t *n_visit, double *force64);
+                        double straddle_mass, const float *straddle_g,
+                        const int64_t *targets, int64_t *n_inter, int64_t *n_visit,
+                        double *force64);
 
 /* FP64 direct sum over all n sources for the listed targets, with the same
  * minimum image, d2 > 1e-7 cut and Plummer law as compute_forces.  Output

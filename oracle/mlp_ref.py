@@ -16,7 +16,7 @@ from __future__ import annotations
 
 import numpy as np
 
-IN, IN_PAD, HID = 10, 16, 64
+IN, IN_PAD, HID = 11, 16, 64
 OFF_W1 = 0
 OFF_B1 = OFF_W1 + IN_PAD * HID
 OFF_W2 = OFF_B1 + HID

@@ -53,7 +53,7 @@ def lib():
             "orc_validate_tree": [_p, _p, _i64],
             "orc_compute_forces": [_p] * 9 + [_f64] * 4 + [_i32, _i64, _i64, _i64,
                                                            C.c_int, C.c_int, _p, C.c_int, _f64,
-                                                           _p, _p, _p, _p],
+                                                           _p, _p, _p, _p, _p],
             "orc_direct_sum": [_p, _p, _p, _p, _i64, _p, _i64, _f64, _f64, _p],
             "orc_kick_drift": [_p] * 7 + [_f32] * 3 + [_i64],
             "orc_integrate_coeffs": [_f64, _f64, _f64, _p, _p, _p],
@@ -165,7 +165,8 @@ def validate_tree(children, parents, n):
 
 def compute_forces(px, py, pz, children, n_mass, n_com, n_min, n_max, theta, G, soft,
                    box, root, off=0, count=None, precision=64, mac=MAC_REF, theta_g=None,
-                   counts=False, want_f64=False, bucket=0, straddle_mass=0.0, targets=None):
+                   counts=False, want_f64=False, bucket=0, straddle_mass=0.0, targets=None,
+                   straddle_g=None):
     """src/kernels/physics_kernels.py:14-153.
 
     Node arrays may be FP64 (as produced by the FP64 upward pass); they are
@@ -177,6 +178,7 @@ def compute_forces(px, py, pz, children, n_mass, n_com, n_min, n_max, theta, G, 
     leaves are summed leaf by leaf in leaf order instead of being opened.
     straddle_mass (AABB test only): nodes lighter than straddle_mass pass the
     minimum-image guard.
+    straddle_g: optional per-particle straddle mass (float32[n]).
     targets: optional particle indices; then every output has one row per
     target (row t belongs to particle targets[t]) and off/count are ignored.
     """
@@ -198,6 +200,7 @@ def compute_forces(px, py, pz, children, n_mass, n_com, n_min, n_max, theta, G, 
         off = 0
     force = np.zeros((rows, 3), np.float32)
     tg = None if theta_g is None else _c(theta_g, np.float32)
+    sg = None if straddle_g is None else _c(straddle_g, np.float32)
     ni = np.zeros(rows, np.int64) if counts else None
     nv = np.zeros(rows, np.int64) if counts else None
     f64 = np.zeros((rows, 3), np.float64) if want_f64 else None
@@ -205,7 +208,7 @@ def compute_forces(px, py, pz, children, n_mass, n_com, n_min, n_max, theta, G, 
                              _ptr(nmin), _ptr(nmax), _ptr(force), float(theta), float(G),
                              float(soft), float(box), int(root), int(off), int(count), n,
                              int(precision), int(mac), _ptr(tg), int(bucket),
-                             float(straddle_mass), _ptr(targets), _ptr(ni), _ptr(nv),
+                             float(straddle_mass), _ptr(sg), _ptr(targets), _ptr(ni), _ptr(nv),
                              _ptr(f64))
     out = [force]
     if counts:

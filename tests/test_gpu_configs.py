@@ -136,7 +136,8 @@ def test_c4_learned_graph_with_2048_splat(learned):
     from src.core.learned_theta import LearnedTheta
     pos, vel = generate_ics(c, 42)
     # one selector per engine (each keeps its own device buffers), same network
-    twin = LearnedTheta(lt.model, lt.thetas_list, lt.cost_np, lt.lam, lt.fallback)
+    twin = LearnedTheta(lt.model, lt.thetas_list, lt.cost_np, lt.lam, lt.fallback,
+                        mu_factors=lt.mu_factors)
     eager, graph = [SimulationEngine(c, SimulationState(len(pos[0]), pos, vel), theta_selector=s)
                     for s in (lt, twin)]
     assert eager.theta_selector is lt
@@ -148,6 +149,8 @@ def test_c4_learned_graph_with_2048_splat(learned):
         graph.step_graph()
         torch.cuda.synchronize()
         assert torch.equal(eager.theta_group, graph.theta_group), f"step {k}"
+        if lt.joint:
+            assert torch.equal(eager.straddle_group, graph.straddle_group), f"step {k}"
         for a, b in zip(eager.state.pos + eager.state.vel, graph.state.pos + graph.state.vel):
             assert torch.equal(a, b), f"step {k}"
         assert torch.equal(eager.state.force, graph.state.force), f"step {k}"
@@ -165,3 +168,32 @@ def test_c4_learned_graph_with_2048_splat(learned):
     print(f"learned step error vs direct sum: {e:.3e} (tuned {res['err_learned']:.3e})")
     assert e <= 1.5e-3
     assert lt.n_fallback == 0 and twin.n_fallback == 0
+
+
+def test_learned_preset_installs_named_network():
+    """A LEARNED_THETA preset loads the network named by THETA_NETWORK (the
+    committed networks/halos_128.npz) and selects a theta and a straddle mass
+    per group after every tree build; steps run with them."""
+    import torch
+
+    from src.core.engine import SimulationEngine
+    from src.core.state import SimulationState
+    from src.utils import config as cfg
+    from src.utils.cosmology import generate_ics
+    c = cfg.preset("learned_128", N_GRID=32, SOFTENING=0.02 / 32, STRADDLE_MASS=5e-5 * 64,
+                   IC_KIND="halos", N_HALOS=8)
+    pos, vel = generate_ics(c, 42)
+    eng = SimulationEngine(c, SimulationState(len(pos[0]), pos, vel))
+    assert eng.theta_selector is not None and eng.theta_selector.joint
+    eng.build_tree()
+    G = (eng.state.n + 63) // 64
+    tg, mg = eng.theta_group, eng.straddle_group
+    assert tg is not None and tg.shape == (G,) and bool(torch.isfinite(tg).all())
+    assert mg is not None and mg.shape == (G,)
+    allowed = torch.tensor([c.STRADDLE_MASS * f for f in eng.theta_selector.mu_factors],
+                           dtype=torch.float32, device=mg.device)
+    assert bool(torch.isin(mg, allowed).all())
+    for _ in range(2):
+        eng.step()
+    assert bool(torch.isfinite(eng.state.force).all())
+    assert eng.theta_selector.n_fallback == 0

@@ -439,6 +439,51 @@ def test_traverse_buckets(orc, torch_mod, ics32, bucket):
         assert rel_rms(host(force), f32) <= 2e-6
 
 
+def test_traverse_straddle_group(orc, torch_mod, ics32):
+    """Per-group straddle masses (AABB test, 16-leaf buckets): each group's
+    interactions and visits equal the oracle's with that group's straddle
+    mass for its particles, forces to FP32 rounding; constant per-group
+    values give exactly the scalar limit's results."""
+    torch = torch_mod
+    from src.kernels.physics_kernels import traverse
+    from src.kernels.tree_kernels import build_tree_spans, normalize_pack_buckets
+    pos, mass, ch, nm, com, lo, hi = _case(orc, ics32)
+    _, codes = sorted_case(orc, ics32)
+    n = len(mass)
+    m = 2 * n - 1
+    children = torch.full((m, 2), -1, dtype=torch.int32, device="cuda")
+    parents = torch.full((m,), -1, dtype=torch.int32, device="cuda")
+    far = torch.zeros(n - 1, dtype=torch.int32, device="cuda")
+    build_tree_spans(dev(codes, torch), children, parents, far)
+    _, csum, _, _ = orc.compute_multipoles(*pos, mass, ch, orc.build_tree(codes)[1])
+    nodes = normalize_pack_buckets(children, far, dev(nm, torch), dev(csum, torch),
+                                   dev(lo, torch), dev(hi, torch), 1.0, 16)
+    root = torch.tensor([n], dtype=torch.int32, device="cuda")
+    P = [dev(p, torch) for p in pos]
+    W = (n + 63) // 64
+    soft = 0.02 / 32
+    sg = np.random.default_rng(8).choice([0.0, 2e-4, 1e-3, 4e-3], W).astype(np.float32)
+    for theta, group_mu in ((0.5, sg), (0.7, np.full(W, 1e-3, np.float32))):
+        force = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+        gs = torch.zeros(3 + 3 * W, dtype=torch.int64, device="cuda")
+        traverse(*P, nodes, root, force, theta, 1.0, soft, 1.0, mac=1, stats=gs,
+                 group_stats=True, straddle_group=dev(group_mu, torch))
+        f32, ni, nv = orc.compute_forces(*pos, ch, nm, com, lo, hi, theta, 1.0, soft, 1.0, n,
+                                         precision=32, mac=1, counts=True, bucket=16,
+                                         straddle_g=np.repeat(group_mu, 64)[:n])
+        g = host(gs)
+        per = g[3:].reshape(W, 3)
+        pad = np.zeros(W * 64 - n, dtype=np.int64)
+        assert np.array_equal(per[:, 0], np.concatenate([ni, pad]).reshape(W, 64).sum(1))
+        assert np.array_equal(per[:, 1], np.concatenate([nv, pad]).reshape(W, 64).sum(1))
+        assert rel_rms(host(force), f32) <= 2e-6
+        if np.all(group_mu == group_mu[0]):
+            f2 = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+            traverse(*P, nodes, root, f2, theta, 1.0, soft, 1.0, mac=1,
+                     straddle_mass=float(group_mu[0]))
+            assert torch.equal(force, f2)
+
+
 @pytest.mark.parametrize("mac", [0, 1])
 def test_traverse_pairs_identical(orc, torch_mod, ics32, mac):
     """Two particles per lane (128-particle groups) gives the same forces bit

@@ -168,3 +168,41 @@ def test_select_theta_and_nan_fallback():
     pred[7, 2] = np.nan
     K.select_theta(_dev(torch, pred.reshape(-1), np.float32), G, thetas, cost, 3.0, 0.5, out, nf)
     assert out[7].item() == pytest.approx(0.5) and int(nf.item()) == 1
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a HIP device")
+def test_select_pairs_and_rows():
+    """Joint candidates (theta, straddle mass): the selection returns the
+    theta and the straddle mass of the same best candidate (and both
+    fallbacks for a non-finite prediction); the MLP rows carry the
+    candidate's straddle code in input 10."""
+    import torch
+    from src.kernels import mlp_kernels as K
+    G = 500
+    th = np.array([0.3, 0.5, 0.3, 0.5, 0.7], np.float32)
+    mus = np.array([1e-5, 1e-5, 4e-5, 4e-5, 2e-5], np.float32)
+    cost = np.array([4.0, 2.0, 3.0, 1.5, 0.5], np.float32)
+    rng = np.random.default_rng(2)
+    pred = rng.uniform(-8, 2, (G, len(th))).astype(np.float32)
+    pred[11, 3] = np.inf
+    dv = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    out = torch.empty(G, dtype=torch.float32, device="cuda")
+    mout = torch.empty(G, dtype=torch.float32, device="cuda")
+    nf = torch.zeros(1, dtype=torch.int64, device="cuda")
+    K.select_theta(dv(pred.reshape(-1)), G, dv(th), dv(cost), 3.0, 0.45, out, nf, mus=dv(mus),
+                   fallback_mu=3e-5, mu_out=mout)
+    k = np.argmin(cost[None, :] + 3.0 * np.exp2(pred), 1)
+    o, mo = out.cpu().numpy(), mout.cpu().numpy()
+    ok = np.ones(G, bool)
+    ok[11] = False
+    assert np.array_equal(o[ok], th[k][ok]) and np.array_equal(mo[ok], mus[k][ok])
+    assert o[11] == np.float32(0.45) and mo[11] == np.float32(3e-5) and int(nf.item()) == 1
+    feat = dv(rng.standard_normal((7, K.GROUP_FEATURES)).astype(np.float32))
+    codes = dv(np.log2(1.0 + mus * 2 ** 21).astype(np.float32))
+    X = K.assemble_rows(feat, dv(th), cand2=codes).cpu().numpy().view(np.uint16)
+    bf = lambda a: (np.asarray(a, np.float32).view(np.uint32) + 0x7FFF +  # noqa: E731
+                    ((np.asarray(a, np.float32).view(np.uint32) >> 16) & 1)) >> 16
+    assert np.array_equal(X[:, 10], np.tile(bf(codes.cpu().numpy()), 7).astype(np.uint16))
+    assert np.array_equal(X[:, 0], np.tile(bf(th), 7).astype(np.uint16))
+    assert not X[:, 11:].any()
