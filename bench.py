@@ -279,9 +279,9 @@ def main():
     # Work counters come from one extra traversal of the final state (the
     # counting kernel variant is not the one timed above).
     eng.build_tree()
-    eng.stats = torch.zeros(3, dtype=torch.int64, device="cuda")
+    eng.stats = torch.zeros(4, dtype=torch.int64, device="cuda")
     eng.compute_forces()
-    inter, visits, pops = (int(x) for x in eng.stats.cpu().tolist())
+    inter, visits, pops, drops = (int(x) for x in eng.stats.cpu().tolist())
     eng.stats = None
     shards = None
     if args.shards > 1 and eng.theta_group is None:

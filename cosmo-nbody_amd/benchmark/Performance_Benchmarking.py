@@ -17,10 +17,10 @@ Deliberate differences, each printed in the CSV's companion columns:
 - render_density: the grid is the fixed-point int64 grid of the drop-in,
   cleared inside the interval as the reference clears its float grid.
 - compute_forces keeps the reference's 5000-particle batch loop (:281-296);
-  each drop-in call re-packs the node records from the reference-layout
-  arrays, because the caller may have changed them.  The extra columns
-  ``traverse_theta_*`` time the step path: one pack and one launch over all
-  particles.
+  the drop-in packs the node records from the reference-layout arrays on
+  the first call for a tree and reuses them while the arrays are unchanged.
+  The extra columns ``traverse_theta_*`` time the step path: one pack and
+  one launch over all particles.
 - positions are drawn from a seeded generator (the reference's are
   unseeded).
 
@@ -102,8 +102,9 @@ class BenchState:
         self.child = torch.full((M, 2), -1, dtype=torch.int32, device=dev)
         self.parent = torch.full((M,), -1, dtype=torch.int32, device=dev)
         self.root_idx = torch.zeros(1, dtype=torch.int32, device=dev)
-        self.n_mass = torch.zeros(M, dtype=torch.float64, device=dev)
-        self.n_com = torch.zeros((M, 3), dtype=torch.float64, device=dev)
+        # float32 node arrays, as the reference allocates them (octree.py:37-38)
+        self.n_mass = torch.zeros(M, dtype=torch.float32, device=dev)
+        self.n_com = torch.zeros((M, 3), dtype=torch.float32, device=dev)
         self.n_min = torch.zeros((M, 3), dtype=torch.float32, device=dev)
         self.n_max = torch.zeros((M, 3), dtype=torch.float32, device=dev)
         self.counters = torch.zeros(M, dtype=torch.int32, device=dev)

@@ -248,7 +248,7 @@ __global__ __launch_bounds__(256) void k_traverse(
   float ax[PPL], ay[PPL], az[PPL], cx[PPL], cy[PPL], cz[PPL];
   int depth[PPL];
   bool active[PPL];
-  uint32_t w_int = 0, w_vis = 0, w_pop = 0;  // wave totals (uniform)
+  uint32_t w_int = 0, w_vis = 0, w_pop = 0, w_drop = 0;  // wave totals (uniform)
   const int root = root_idx[0];
 #pragma unroll
   for (int h = 0; h < PPL; ++h) {
@@ -350,8 +350,12 @@ __global__ __launch_bounds__(256) void k_traverse(
     bool open[PPL], hit[PPL], rej[PPL];
     uint64_t om = 0;
     if constexpr (MAC == COSMO_MAC_REF) {
-      // lanes may open the node only while the stack has room (:143)
+      // lanes may open the node only while the stack has room (:143), and
+      // only an internal node has children to push (c1 != -1, :144): a leaf
+      // whose test fails on a non-finite distance is used as the reference
+      // uses it (no interaction), never opened
       const uint64_t m_room = room ? ~0ull : 0ull;
+      const uint64_t m_kids = __float_as_int(B.z) >= 0 ? ~0ull : 0ull;
       // The reference test with the lane sets kept as scalar masks: each
       // compare is one ballot, the set logic runs on the scalar unit, and
       // the divergent interaction runs under inverse_ballot(mask) (no lane
@@ -390,7 +394,7 @@ __global__ __launch_bounds__(256) void k_traverse(
           m_near = __builtin_amdgcn_ballot_w64(near_ok);
         }
         const uint64_t m_hit = m_in[h] & m_acc & m_near;
-        const uint64_t m_open = m_in[h] & ~m_acc & m_room;
+        const uint64_t m_open = m_in[h] & ~m_acc & m_room & m_kids;
         om |= m_open;
         open[h] = __builtin_amdgcn_inverse_ballot_w64(m_open);
         if (__builtin_amdgcn_inverse_ballot_w64(m_hit)) {
@@ -400,6 +404,8 @@ __global__ __launch_bounds__(256) void k_traverse(
         if (STATS) {
           w_vis += __builtin_popcountll(m_in[h]);
           w_int += __builtin_popcountll(m_hit);
+          // lanes whose children the full stack drops (:143)
+          w_drop += __builtin_popcountll(m_in[h] & ~m_acc & ~m_room & m_kids);
         }
       }
     } else {
@@ -449,6 +455,10 @@ __global__ __launch_bounds__(256) void k_traverse(
     // records are wave-uniform scalar loads.
     int bcount = 0;
     if constexpr (MAC == COSMO_MAC_AABB) bcount = leaf ? 0 : __float_as_int(D.w);
+    if (STATS && MAC == COSMO_MAC_AABB && bcount == 0 && !room) {
+#pragma unroll
+      for (int h = 0; h < PPL; ++h) w_drop += __builtin_popcountll(__ballot(rej[h]));
+    }
     if (bcount > 0) {
       uint64_t bm = 0;  // lanes that reject the node (stack room is not needed)
 #pragma unroll
@@ -522,8 +532,9 @@ __global__ __launch_bounds__(256) void k_traverse(
     atomicAdd(&stats[0], static_cast<unsigned long long>(w_int));
     atomicAdd(&stats[1], static_cast<unsigned long long>(w_vis));
     atomicAdd(&stats[2], static_cast<unsigned long long>(w_pop));
+    if (w_drop) atomicAdd(&stats[3], static_cast<unsigned long long>(w_drop));
     if (p.wave_stats) {
-      unsigned long long *ws = stats + 3 + 3 * wave;
+      unsigned long long *ws = stats + 4 + 3 * wave;
       ws[0] = w_int;
       ws[1] = w_vis;
       ws[2] = w_pop;
@@ -791,7 +802,9 @@ __global__ __launch_bounds__(256) void k_traverse_split(
           w_vis += __builtin_popcountll(__ballot(in));
           w_int += __builtin_popcountll(__ballot(hit));
         }
-        if (__ballot(open) != 0) {
+        // only an internal node has children to push (c1 != -1,
+        // physics_kernels.py:144)
+        if (__ballot(open) != 0 && __float_as_int(B.z) >= 0) {
           stk = write_lane(stk, __float_as_int(B.w), top);
           stk = write_lane(stk, c1, top + 1);
           depth = open ? top + 2 : depth;

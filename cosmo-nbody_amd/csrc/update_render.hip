@@ -7,6 +7,8 @@
 // same way and the results are bit-identical.
 #include <math.h>
 
+#include <algorithm>
+
 #include "common.h"
 
 namespace cosmo {
@@ -85,6 +87,18 @@ __global__ __launch_bounds__(256) void k_render_density_dev(const float *__restr
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   splat_one(px[i], py[i], pz[i], m, res, scale, grid);
+}
+
+// count[0] += number of non-finite values among v[0..n) (NaN or +-inf).
+__global__ __launch_bounds__(256) void k_count_nonfinite(const float *__restrict__ v, int64_t n,
+                                                         unsigned long long *__restrict__ count) {
+  unsigned long long c = 0;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    c += isfinite(v[i]) ? 0ull : 1ull;
+  // wave sum, one atomic per wavefront that saw any
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(count, c);
 }
 
 // grid[p] += bins[p] * 2^-frac_bits: the fixed-point splat added to a float32
@@ -230,6 +244,17 @@ int cosmo_grid_add_fixed(const uint64_t *bins, float *grid, int64_t cells, int f
   if (!bins || !grid) return COSMO_E_INVALID;
   k_grid_add_fixed<<<blocks_for(cells, 256), 256, 0, as_stream(stream)>>>(
       reinterpret_cast<const unsigned long long *>(bins), grid, cells, ldexp(1.0, -frac_bits));
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+int cosmo_count_nonfinite(const float *v, int64_t n, uint64_t *count, void *stream) {
+  if (n < 0 || !count) return COSMO_E_INVALID;
+  if (n == 0) return COSMO_OK;
+  if (!v) return COSMO_E_INVALID;
+  const unsigned g = static_cast<unsigned>(std::min<int64_t>(blocks_for(n, 256), 4096));
+  k_count_nonfinite<<<g, 256, 0, as_stream(stream)>>>(
+      v, n, reinterpret_cast<unsigned long long *>(count));
   COSMO_CHECK_LAUNCH();
   return COSMO_OK;
 }

@@ -23,7 +23,7 @@ import torch
 from src.core.camera import Camera
 from src.core.octree import OctreeManager
 from src.kernels import _hip
-from src.kernels.physics_kernels import MAC_AABB, MAC_REF, integrate, traverse
+from src.kernels.physics_kernels import MAC_AABB, MAC_REF, count_nonfinite, integrate, traverse
 from src.utils.cosmology import get_hubble_param
 from src.utils.visualizer import get_mvp_matrix
 
@@ -48,7 +48,7 @@ def network_path(name: str) -> str:
 
 
 class SimulationEngine:
-    def __init__(self, config, state, device="cuda", theta_selector=None):
+    def __init__(self, config, state, device="cuda", theta_selector=None, health=False):
         # LEARNED_THETA installs the network named by THETA_NETWORK (or the one
         # given); it never falls back to fixed theta silently
         if getattr(config, "LEARNED_THETA", False):
@@ -87,9 +87,17 @@ class SimulationEngine:
         self.group_order = None     # launch order of the groups under per-group theta
         self.straddle_group = None  # per-group straddle masses chosen with theta_group
         self._a_dev = None          # device scale factor while a step is being captured
-        self.stats = None           # int64[3] interaction / visit / pop counters, or None
+        self.stats = None           # int64[4] interaction / visit / pop / drop counters, or None
         self.pairs = bool(getattr(config, "TRAVERSAL_PAIRS", False))
         self.steps_done = 0
+        # failure detection (health=True): the traversal's counters (its
+        # overflow drops among them) and the non-finite accelerations,
+        # accumulated on the device over the steps, read by health_report()
+        self.health = bool(health)
+        self._health_stats = (torch.zeros(4, dtype=torch.int64, device=device) if health
+                              else None)
+        self._nonfinite = torch.zeros(1, dtype=torch.int64, device=device) if health else None
+        self._health_steps = 0
 
     def timestep(self):
         """(H, dt) at the current scale factor (engine.py:63-64)."""
@@ -108,13 +116,30 @@ class SimulationEngine:
 
     def compute_forces(self):
         s, c = self.state, self.config
+        stats = self._health_stats if self.health and self.stats is None else self.stats
         traverse(*s.pos, self.tree_manager.nodes, self.tree_manager.root_idx, s.force, c.THETA,
                  c.G, c.SOFTENING, c.BOX_SIZE, mac=self.mac, theta_group=self.theta_group,
-                 compensated=bool(getattr(c, "COMPENSATED", False)), stats=self.stats,
+                 compensated=bool(getattr(c, "COMPENSATED", False)), stats=stats,
                  pairs=self.pairs, straddle_mass=self.straddle_mass,
                  straddle_group=self.straddle_group if self.theta_group is not None else None,
                  group_order=(self.group_order if self.theta_group is not None and not self.pairs
                               else None))
+        if self.health:
+            count_nonfinite(s.force, self._nonfinite)
+            self._health_steps += 1
+
+    def health_report(self):
+        """Counters of the steps run with health=True (synchronises): overflow
+        drops (nodes a particle's full 64-entry stack could not open, whose
+        children the reference drops), non-finite acceleration components,
+        per-group theta fallbacks of the learned selection."""
+        if not self.health:
+            raise RuntimeError("the engine was created without health=True")
+        st = [int(x) for x in self._health_stats.cpu().tolist()]
+        return {"steps": self._health_steps, "interactions": st[0], "visits": st[1],
+                "overflow_drops": st[3], "nonfinite_force_components": int(self._nonfinite.item()),
+                "theta_fallbacks": (self.theta_selector.n_fallback if self.theta_selector
+                                    is not None else 0)}
 
     def step(self):
         H, dt = self.timestep()

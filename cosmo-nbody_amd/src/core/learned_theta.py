@@ -207,7 +207,7 @@ class GroupProbe:
         self.norm = float((self.a_ref ** 2).sum().item())
         self.counts = np.minimum(64, st.n - self.groups * 64).astype(np.float64)
         self._force = torch.empty((self.T, 3), dtype=torch.float32, device=dev)
-        self._stats = torch.zeros(3, dtype=torch.int64, device=dev)
+        self._stats = torch.zeros(4, dtype=torch.int64, device=dev)
 
     def errors(self, theta=0.5, theta_group=None, straddle=None, straddle_group=None):
         """(squared-error sum per group, interactions per target); straddle:

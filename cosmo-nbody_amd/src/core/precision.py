@@ -50,7 +50,7 @@ def sweep(engine, thetas, n_targets=16384, seed=1234,
     t_direct = time.perf_counter() - t0
     out = []
     force = torch.empty((len(idx), 3), dtype=torch.float32, device=ref.device)
-    stats = torch.zeros(3, dtype=torch.int64, device=ref.device)
+    stats = torch.zeros(4, dtype=torch.int64, device=ref.device)
     own = (engine.mac, engine.bucket)
     for name, bucket, mu in variants:
         mac = MAC_REF if name == "ref" else MAC_AABB

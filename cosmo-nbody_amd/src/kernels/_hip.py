@@ -61,6 +61,7 @@ SIGNATURES = {
                                  _int),
     "cosmo_compute_forces_packed": ([_p] * 5 + [_f64] * 4 + [_p, _i64, _i64, _i64, _p], _int),
     "cosmo_grid_add_fixed": ([_p, _p, _i64, _int, _p], _int),
+    "cosmo_count_nonfinite": ([_p, _i64, _p, _p], _int),
     "cosmo_zeldovich_work_bytes": ([_i64], _i64),
     "cosmo_zeldovich_ics": ([_i64, _p, _f64, _f64, _p] + [_p] * 6 + [_p], _int),
     "cosmo_find_root": ([_p, _p, _i64, _p], _int),

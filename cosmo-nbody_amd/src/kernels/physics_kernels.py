@@ -20,7 +20,7 @@ from ._hip import (MAC_AABB, MAC_REF, TRAV_COMPENSATED, TRAV_GROUP_STATS, TRAV_P
                    check, kernel, load, ptr, shared_context, stream_handle, tree_generation)
 
 __all__ = ["compute_forces", "integrate", "kick_drift", "traverse", "order_groups", "direct_sum",
-           "MAC_REF", "MAC_AABB"]
+           "count_nonfinite", "MAC_REF", "MAC_AABB"]
 
 _F32 = torch.float32
 
@@ -73,10 +73,10 @@ def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0
              mac=MAC_REF, theta_group=None, compensated=False, stats=None, n_nodes=None,
              pairs=False, group_stats=False, group_order=None, straddle_mass=0.0,
              straddle_group=None):
-    """Traversal over packed node records; stats: int64[3] tensor accumulating
-    (interactions, node visits, wavefront stack pops) when given; with
-    group_stats it has 3 + 3*W entries and entries 3+3w.. hold wavefront group
-    w's own counts.  The targets (pos_*) need not be the tree's particles:
+    """Traversal over packed node records; stats: int64[4] tensor accumulating
+    (interactions, node visits, wavefront stack pops, overflow drops) when
+    given; with group_stats it has 4 + 3*W entries and entries 4+3w.. hold
+    wavefront group w's own interactions, visits and pops.  The targets (pos_*) need not be the tree's particles:
     n_nodes gives the tree size when they differ.  pairs: two particles per
     lane (128-particle groups).  group_order: int32 launch order of the
     64-particle groups (a permutation, e.g. from order_groups).
@@ -96,7 +96,7 @@ def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0
     tg = None if theta_group is None else ptr(theta_group, _F32, "theta_group",
                                               numel=max(groups, 0))
     st = None if stats is None else ptr(stats, (torch.int64, torch.uint64), "stats",
-                                        numel=3 + (3 * max(waves, 0) if group_stats else 0))
+                                        numel=4 + (3 * max(waves, 0) if group_stats else 0))
     args = (*_xyz(pos_x, pos_y, pos_z, n), ptr(nodes, torch.uint8, "nodes"), n_tree,
             ptr(root, torch.int32, "root"), ptr(force, _F32, "force", numel=3 * n),
             float(theta), tg, float(G), float(soft), float(box), int(off), int(count), n,
@@ -166,3 +166,11 @@ def direct_sum(pos_x, pos_y, pos_z, mass, targets, G, soft, out=None):
                                   float(soft), ptr(out, torch.float64, "out", numel=3 * nt),
                                   stream_handle()), "direct_sum")
     return out
+
+
+def count_nonfinite(values, count):
+    """count[0] += number of non-finite entries of the float32 tensor values
+    (count: int64[1] on the device; no host synchronisation)."""
+    check(load().cosmo_count_nonfinite(ptr(values, _F32, "values"), values.numel(),
+                                       ptr(count, (torch.int64, torch.uint64), "count", numel=1),
+                                       stream_handle()), "count_nonfinite")

@@ -250,12 +250,14 @@ int cosmo_normalize_pack_buckets(const int32_t *children, const int32_t *far,
  * wavefront each),
  * writing force[i][0..2] (accelerations).  theta_group (nullable, device
  * float[ceil(count/64)]) gives each group's opening angle in place of
- * theta.  stats (nullable, device uint64[3]) accumulates interactions,
- * node visits (per particle) and stack pops (per wavefront) when flags has
- * COSMO_TRAV_STATS.  With COSMO_TRAV_GROUP_STATS stats holds 3 + 3*W entries
- * for W = ceil(count / (64 * particles per lane)) wavefront groups: the
- * totals accumulate in stats[0..2] and group w's own counts are written to
- * stats[3+3w .. 3+3w+2]. */
+ * theta.  stats (nullable, device uint64[4]) accumulates interactions,
+ * node visits (per particle), stack pops (per wavefront) and overflow drops
+ * (per particle: nodes it would have opened while its stack held 63
+ * entries, whose children the reference drops, physics_kernels.py:143) when
+ * flags has COSMO_TRAV_STATS.  With COSMO_TRAV_GROUP_STATS stats holds
+ * 4 + 3*W entries for W = ceil(count / (64 * particles per lane))
+ * wavefront groups: the totals accumulate in stats[0..3] and group w's own
+ * interactions, visits and pops are written to stats[4+3w .. 4+3w+2]. */
 int cosmo_traverse(const float *pos_x, const float *pos_y, const float *pos_z,
                    const void *nodes, int64_t n_tree, const int32_t *root_idx,
                    float *force, double theta, const float *theta_group,
@@ -415,6 +417,12 @@ int cosmo_render_density_dev(const float *pos_x, const float *pos_y,
 int cosmo_colorize_frame(const uint64_t *grid, int32_t res, int frac_bits,
                          uint64_t *max_scratch, float *frame, uint8_t *rgb,
                          void *stream);
+
+/* Failure detection (no reference counterpart; the reference has no checks):
+ * count[0] += the number of non-finite values (NaN, +-inf) among v[0..n),
+ * e.g. the 3n components of the accelerations after a traversal.  count:
+ * device uint64[1]. */
+int cosmo_count_nonfinite(const float *v, int64_t n, uint64_t *count, void *stream);
 
 /* FP64 direct sum (no reference counterpart; the error yardstick): for each
  * target t, acc[t][k] = sum_j G m_j (d^2+eps^2)^-3/2 d_k over all n sources

@@ -384,12 +384,12 @@ static void bh_one_f64(const float *px, const float *py, const float *pz,
                        double soft, double box, int32_t root, int64_t i,
                        const int32_t *bfirst, const int32_t *bcount, int bucket,
                        float straddle_lim, double out[3], int64_t *n_int,
-                       int64_t *n_vis) {
+                       int64_t *n_vis, int64_t *n_drop) {
   const double x = px[i], y = py[i], z = pz[i];
   double ax = 0.0, ay = 0.0, az = 0.0;
   int32_t stack[64];
   int top = 0;
-  int64_t ni = 0, nv = 0;
+  int64_t ni = 0, nv = 0, nd = 0;
   stack[top++] = root;
   const double theta_sq = theta * theta;
   const double inv_theta = 1.0 / theta;
@@ -463,6 +463,8 @@ static void bh_one_f64(const float *px, const float *py, const float *pz,
     } else if (top + 1 < 64) {
       if (c1 != -1) stack[top++] = c2;
       if (c2 != -1) stack[top++] = c1;
+    } else {
+      nd++; /* the full stack drops the children (physics_kernels.py:143) */
     }
   }
   out[0] = ax;
@@ -470,6 +472,7 @@ static void bh_one_f64(const float *px, const float *py, const float *pz,
   out[2] = az;
   if (n_int) *n_int = ni;
   if (n_vis) *n_vis = nv;
+  if (n_drop) *n_drop = nd;
 }
 
 /* Minimum image of c - p in FP32 with the image chosen from the exact
@@ -513,13 +516,14 @@ static void bh_one_f32(const float *px, const float *py, const float *pz,
                        float Gf, float softf, float boxf, int32_t root,
                        int64_t i, int compensated, const int32_t *bfirst,
                        const int32_t *bcount, int bucket, float straddle_lim,
-                       double out[3], int64_t *n_int, int64_t *n_vis) {
+                       double out[3], int64_t *n_int, int64_t *n_vis,
+                       int64_t *n_drop) {
   const float x = px[i], y = py[i], z = pz[i];
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   float cx = 0.0f, cy = 0.0f, cz = 0.0f; /* Neumaier compensation terms */
   int32_t stack[64];
   int top = 0;
-  int64_t ni = 0, nv = 0;
+  int64_t ni = 0, nv = 0, nd = 0;
   stack[top++] = root;
   const float inv_theta = 1.0f / thf;
   const float eps2 = softf * softf;
@@ -612,6 +616,8 @@ static void bh_one_f32(const float *px, const float *py, const float *pz,
     } else if (top + 1 < 64) {
       if (c1 != -1) stack[top++] = c2;
       if (c2 != -1) stack[top++] = c1;
+    } else {
+      nd++; /* the full stack drops the children (physics_kernels.py:143) */
     }
   }
   out[0] = ax + cx;
@@ -619,6 +625,7 @@ static void bh_one_f32(const float *px, const float *py, const float *pz,
   out[2] = az + cz;
   if (n_int) *n_int = ni;
   if (n_vis) *n_vis = nv;
+  if (n_drop) *n_drop = nd;
 }
 
 void orc_compute_forces(const float *px, const float *py, const float *pz,
@@ -630,7 +637,7 @@ void orc_compute_forces(const float *px, const float *py, const float *pz,
                         int mac, const float *theta_g, int bucket,
                         double straddle_mass, const float *straddle_g,
                         const int64_t *targets, int64_t *n_inter, int64_t *n_visit,
-                        double *force64) {
+                        int64_t *n_drop, double *force64) {
   int64_t hi = off + count;
   if (hi > n) hi = n;
   if (targets) { /* the listed particles; output row t is targets[t] */
@@ -648,19 +655,19 @@ void orc_compute_forces(const float *px, const float *py, const float *pz,
   for (int64_t t = off; t < hi; ++t) {
     const int64_t i = targets ? targets[t] : t;
     double a[3];
-    int64_t ni = 0, nv = 0;
+    int64_t ni = 0, nv = 0, nd = 0;
     if (precision == 64) {
       const double th = theta_g ? (double)theta_g[i] : theta;
       bh_one_f64(px, py, pz, children, n_mass, n_com, n_min, n_max, mac, th, G,
                  soft, box, root, i, bfirst, bcount, bucket, straddle_g ? straddle_g[i] : (float)straddle_mass,
-                 a, &ni, &nv);
+                 a, &ni, &nv, &nd);
     } else {
       const float th = theta_g ? theta_g[i] : (float)theta;
       const double th2d = theta_g ? (double)th * (double)th : theta * theta;
       bh_one_f32(px, py, pz, children, n_mass, n_com, n_min, n_max, mac, th,
                  th2d, (float)G, (float)soft, (float)box, root, i,
                  precision == 31, bfirst, bcount, bucket, straddle_g ? straddle_g[i] : (float)straddle_mass,
-                 a, &ni, &nv);
+                 a, &ni, &nv, &nd);
     }
     force[3 * t + 0] = (float)a[0];
     force[3 * t + 1] = (float)a[1];
@@ -672,6 +679,7 @@ void orc_compute_forces(const float *px, const float *py, const float *pz,
     }
     if (n_inter) n_inter[t] = ni;
     if (n_visit) n_visit[t] = nv;
+    if (n_drop) n_drop[t] = nd;
   }
   free(bfirst);
   free(bcount);

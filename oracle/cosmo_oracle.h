@@ -99,7 +99,9 @@ int orc_validate_tree(const int32_t *children, const int32_t *parents,
  * targets: optional list of count particle indices (off is then ignored);
  *   output row t (force, counts, force64) belongs to particle targets[t].
  * n_inter / n_visit: optional per-particle counts of accepted interactions
- * (d2 > 1e-7) and of node visits (MAC evaluations and bucket leaves).     */
+ * (d2 > 1e-7) and of node visits (MAC evaluations and bucket leaves).
+ * n_drop: optional per-particle count of rejected internal nodes whose
+ * children the full 64-entry stack drops (physics_kernels.py:143).       */
 void orc_compute_forces(const float *px, const float *py, const float *pz,
                         const int32_t *children, const float *n_mass,
                         const float *n_com, const float *n_min,
@@ -109,7 +111,7 @@ void orc_compute_forces(const float *px, const float *py, const float *pz,
                         int mac, const float *theta_g, int bucket,
                         double straddle_mass, const float *straddle_g,
                         const int64_t *targets, int64_t *n_inter, int64_t *n_visit,
-                        double *force64);
+                        int64_t *n_drop, double *force64);
 
 /* FP64 direct sum over all n sources for the listed targets, with the same
  * minimum image, d2 > 1e-7 cut and Plummer law as compute_forces.  Output

@@ -53,7 +53,7 @@ def lib():
             "orc_validate_tree": [_p, _p, _i64],
             "orc_compute_forces": [_p] * 9 + [_f64] * 4 + [_i32, _i64, _i64, _i64,
                                                            C.c_int, C.c_int, _p, C.c_int, _f64,
-                                                           _p, _p, _p, _p, _p],
+                                                           _p, _p, _p, _p, _p, _p],
             "orc_direct_sum": [_p, _p, _p, _p, _i64, _p, _i64, _f64, _f64, _p],
             "orc_kick_drift": [_p] * 7 + [_f32] * 3 + [_i64],
             "orc_integrate_coeffs": [_f64, _f64, _f64, _p, _p, _p],
@@ -166,14 +166,15 @@ def validate_tree(children, parents, n):
 def compute_forces(px, py, pz, children, n_mass, n_com, n_min, n_max, theta, G, soft,
                    box, root, off=0, count=None, precision=64, mac=MAC_REF, theta_g=None,
                    counts=False, want_f64=False, bucket=0, straddle_mass=0.0, targets=None,
-                   straddle_g=None):
+                   straddle_g=None, drops=False):
     """src/kernels/physics_kernels.py:14-153.
 
     Node arrays may be FP64 (as produced by the FP64 upward pass); they are
     rounded to FP32, the storage type of the reference's node arrays
     (src/core/octree.py:37-40), before the traversal.
     Returns force float32[n,3] (rows outside [off, off+count) are zero) and,
-    if counts, (n_inter, n_visit) int64[n]; if want_f64 also the FP64 sums.
+    if counts, (n_inter, n_visit) int64[n]; if drops, the per-particle
+    overflow drops int64[n]; if want_f64 also the FP64 sums.
     bucket (AABB test only): rejected internal nodes of at most `bucket`
     leaves are summed leaf by leaf in leaf order instead of being opened.
     straddle_mass (AABB test only): nodes lighter than straddle_mass pass the
@@ -204,15 +205,18 @@ def compute_forces(px, py, pz, children, n_mass, n_com, n_min, n_max, theta, G, 
     ni = np.zeros(rows, np.int64) if counts else None
     nv = np.zeros(rows, np.int64) if counts else None
     f64 = np.zeros((rows, 3), np.float64) if want_f64 else None
+    nd = np.zeros(rows, np.int64) if drops else None
     lib().orc_compute_forces(_ptr(px), _ptr(py), _ptr(pz), _ptr(children), _ptr(nm), _ptr(nc),
                              _ptr(nmin), _ptr(nmax), _ptr(force), float(theta), float(G),
                              float(soft), float(box), int(root), int(off), int(count), n,
                              int(precision), int(mac), _ptr(tg), int(bucket),
                              float(straddle_mass), _ptr(sg), _ptr(targets), _ptr(ni), _ptr(nv),
-                             _ptr(f64))
+                             _ptr(nd), _ptr(f64))
     out = [force]
     if counts:
         out += [ni, nv]
+    if drops:
+        out.append(nd)
     if want_f64:
         out.append(f64)
     return out[0] if len(out) == 1 else tuple(out)

@@ -108,7 +108,7 @@ def gpu_stages(pos, vel, n_grid, force_in, mvp, res, theta):
     for mac, tag in ((0, "ref"), (1, "aabb")):
         nodes = pack_nodes(ch, nm, nc, lo, hi, 1.0, mac)
         f = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
-        stats = torch.zeros(3, dtype=torch.int64, device="cuda")
+        stats = torch.zeros(4, dtype=torch.int64, device="cuda")
         traverse(*ps, nodes, root, f, theta, 1.0, soft, 1.0, mac=mac, stats=stats)
         out[f"force32_{tag}"] = h(f)
         out[f"stats_{tag}"] = h(stats)

@@ -69,7 +69,7 @@ def test_c2_theta_sweep_on_clustered_state(orc, mac):
     for theta in (0.3, 0.5, 0.7, 0.9):
         for comp in (False, True):
             f = torch.zeros((len(tg), 3), dtype=torch.float32, device="cuda")
-            s = torch.zeros(3, dtype=torch.int64, device="cuda")
+            s = torch.zeros(4, dtype=torch.int64, device="cuda")
             traverse(*tpos, tm.nodes, tm.root_idx, f, theta, c.G, c.SOFTENING, c.BOX_SIZE,
                      mac=eng.mac, compensated=comp, stats=s, n_nodes=tm.n_nodes)
             g = f.cpu().numpy()

@@ -31,7 +31,7 @@ def test_reference_mac_known_answers_gpu(orc, name):
     nodes = pack_nodes(_dev(ch), _dev(nm), _dev(com), _dev(lo), _dev(hi), 1.0, 0)
     root = torch.tensor([3], dtype=torch.int32, device="cuda")
     f = torch.zeros((3, 3), dtype=torch.float32, device="cuda")
-    st = torch.zeros(3, dtype=torch.int64, device="cuda")
+    st = torch.zeros(4, dtype=torch.int64, device="cuda")
     traverse(*P, nodes, root, f, case["theta"], K.G, K.SOFT, 1.0, off=0, count=1, stats=st)
     s = st.cpu().numpy()
     assert int(s[0]) == case["interactions"] and int(s[1]) == case["visits"]

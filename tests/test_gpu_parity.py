@@ -333,6 +333,83 @@ def test_traverse_split_stack_overflow(orc, torch_mod):
 
 
 @pytest.mark.parametrize("mac", [0, 1])
+def test_traverse_overflow_drops(orc, torch_mod, mac):
+    """The code chain of 63 levels at theta = 0: every particle's walk finds
+    the full 64-entry stack at the pair's parent and drops its children
+    (physics_kernels.py:143).  The traversal's drop counter equals the
+    oracle's per-particle drops summed (one per particle), and its
+    interaction and visit counts and forces equal the oracle's."""
+    torch = torch_mod
+    from test_oracle_stages import _code_chain
+
+    from src.kernels.physics_kernels import traverse
+    from src.kernels.tree_kernels import pack_nodes
+    pos, mass, ch, nm, com, lo, hi = _case(orc, _code_chain(63))
+    n = len(mass)
+    nodes = pack_nodes(dev(ch, torch), dev(nm, torch), dev(com, torch), dev(lo, torch),
+                       dev(hi, torch), 1.0, mac)
+    root = torch.tensor([n], dtype=torch.int32, device="cuda")
+    force = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+    stats = torch.zeros(4, dtype=torch.int64, device="cuda")
+    traverse(*[dev(p, torch) for p in pos], nodes, root, force, 0.0, 1.0, 1e-3, 1.0, mac=mac,
+             stats=stats)
+    f32, ni, nv, nd = orc.compute_forces(*pos, ch, nm, com, lo, hi, 0.0, 1.0, 1e-3, 1.0, n,
+                                         precision=32, mac=mac, counts=True, drops=True)
+    s = host(stats)
+    assert int(nd.sum()) == n
+    assert int(s[3]) == int(nd.sum())
+    assert int(s[0]) == int(ni.sum()) and int(s[1]) == int(nv.sum())
+    assert rel_rms(host(force), f32) <= 2e-6
+    # a stack that never fills drops nothing
+    f2 = torch.zeros_like(force)
+    st2 = torch.zeros(4, dtype=torch.int64, device="cuda")
+    traverse(*[dev(p, torch) for p in pos], nodes, root, f2, 0.5, 1.0, 1e-3, 1.0, mac=mac,
+             stats=st2)
+    assert int(host(st2)[3]) == int(orc.compute_forces(*pos, ch, nm, com, lo, hi, 0.5, 1.0,
+                                                       1e-3, 1.0, n, precision=32, mac=mac,
+                                                       drops=True)[1].sum())
+
+
+@pytest.mark.parametrize("mac", ["ref", "aabb"])
+def test_engine_health_counts_nonfinite(torch_mod, mac):
+    """Health mode on a clean state, on a state with one NaN mass and on one
+    with a mass of 3e38.  A NaN mass makes the CoM of the leaf and of every
+    node above it NaN: those nodes fail every comparison, so they are opened
+    (internal) or skipped (the leaf), the walk still ends, and no
+    acceleration is touched -- the reference method's own outcome, which
+    the oracle shows as well.  A mass of 3e38 keeps every CoM finite and
+    sends every other particle's acceleration to infinity.  The device
+    counter equals the number of non-finite components in the force array
+    summed over the two steps, and no stack overflows."""
+    torch = torch_mod
+    from src.core.engine import SimulationEngine
+    from src.core.state import SimulationState
+    from src.utils import config as cfg
+    from src.utils.cosmology import generate_ics
+    c = cfg.preset("small_32", MAC=mac, BUCKET=16 if mac == "aabb" else 0)
+    pos, vel = generate_ics(c, 7)
+    n = len(pos[0])
+    for bad in (None, float("nan"), 3e38):
+        st = SimulationState(n, pos, vel)
+        if bad is not None:
+            st.mass[1234] = bad
+        eng = SimulationEngine(c, st, health=True)
+        for _ in range(2):
+            eng.build_tree()
+            eng.compute_forces()
+        torch.cuda.synchronize()
+        rep = eng.health_report()
+        assert rep["steps"] == 2 and rep["overflow_drops"] == 0
+        last = int((~torch.isfinite(eng.state.force)).sum().item())
+        assert rep["nonfinite_force_components"] == 2 * last, bad
+        if bad == 3e38:
+            assert last == 3 * (n - 1)
+        else:
+            assert last == 0
+            assert rep["interactions"] > 0 and rep["visits"] >= rep["interactions"]
+
+
+@pytest.mark.parametrize("mac", [0, 1])
 def test_traverse_interaction_counts(orc, torch_mod, ics32, mac):
     torch = torch_mod
     from src.kernels.physics_kernels import traverse
@@ -343,7 +420,7 @@ def test_traverse_interaction_counts(orc, torch_mod, ics32, mac):
     nodes = pack_nodes(dev(ch, torch), dev(nm, torch), dev(com, torch), dev(lo, torch),
                        dev(hi, torch), 1.0, mac)
     force = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
-    stats = torch.zeros(3, dtype=torch.int64, device="cuda")
+    stats = torch.zeros(4, dtype=torch.int64, device="cuda")
     root = torch.tensor([n], dtype=torch.int32, device="cuda")
     traverse(*[dev(p, torch) for p in pos], nodes, root, force, 0.5, 1.0, soft, 1.0, mac=mac,
              stats=stats)
@@ -356,15 +433,15 @@ def test_traverse_interaction_counts(orc, torch_mod, ics32, mac):
     # per-group counts: each wavefront group's interactions and visits are the
     # oracle's per-particle counts summed over its 64 particles
     W = (n + 63) // 64
-    gs = torch.zeros(3 + 3 * W, dtype=torch.int64, device="cuda")
+    gs = torch.zeros(4 + 3 * W, dtype=torch.int64, device="cuda")
     traverse(*[dev(p, torch) for p in pos], nodes, root, force, 0.5, 1.0, soft, 1.0, mac=mac,
              stats=gs, group_stats=True)
     g = host(gs)
-    per = g[3:].reshape(W, 3)
+    per = g[4:].reshape(W, 3)
     pad = np.zeros(W * 64 - n, dtype=np.int64)
     assert np.array_equal(per[:, 0], np.concatenate([ni, pad]).reshape(W, 64).sum(1))
     assert np.array_equal(per[:, 1], np.concatenate([nv, pad]).reshape(W, 64).sum(1))
-    assert np.array_equal(g[:3], s) and int(per[:, 2].sum()) == int(s[2])
+    assert np.array_equal(g[:4], s) and int(per[:, 2].sum()) == int(s[2])
 
 
 def _leaf_ranges(ch, n):
@@ -421,7 +498,7 @@ def test_traverse_buckets(orc, torch_mod, ics32, bucket):
                                          (0.5, tg, False, 0.0), (0.3, None, True, 0.0),
                                          (0.6, None, True, 4e-3), (0.5, tg, False, 1e-3)):
         force = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
-        gs = torch.zeros(3 + 3 * W, dtype=torch.int64, device="cuda")
+        gs = torch.zeros(4 + 3 * W, dtype=torch.int64, device="cuda")
         traverse(*P, nodes, root, force, theta, 1.0, soft, 1.0, mac=1, stats=gs,
                  group_stats=True, compensated=comp, straddle_mass=mu,
                  theta_group=None if group_theta is None else dev(group_theta, torch))
@@ -432,7 +509,7 @@ def test_traverse_buckets(orc, torch_mod, ics32, bucket):
                                          straddle_mass=mu)
         g = host(gs)
         assert int(g[0]) == int(ni.sum()) and int(g[1]) == int(nv.sum())
-        per = g[3:].reshape(W, 3)
+        per = g[4:].reshape(W, 3)
         pad = np.zeros(W * 64 - n, dtype=np.int64)
         assert np.array_equal(per[:, 0], np.concatenate([ni, pad]).reshape(W, 64).sum(1))
         assert np.array_equal(per[:, 1], np.concatenate([nv, pad]).reshape(W, 64).sum(1))
@@ -465,14 +542,14 @@ def test_traverse_straddle_group(orc, torch_mod, ics32):
     sg = np.random.default_rng(8).choice([0.0, 2e-4, 1e-3, 4e-3], W).astype(np.float32)
     for theta, group_mu in ((0.5, sg), (0.7, np.full(W, 1e-3, np.float32))):
         force = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
-        gs = torch.zeros(3 + 3 * W, dtype=torch.int64, device="cuda")
+        gs = torch.zeros(4 + 3 * W, dtype=torch.int64, device="cuda")
         traverse(*P, nodes, root, force, theta, 1.0, soft, 1.0, mac=1, stats=gs,
                  group_stats=True, straddle_group=dev(group_mu, torch))
         f32, ni, nv = orc.compute_forces(*pos, ch, nm, com, lo, hi, theta, 1.0, soft, 1.0, n,
                                          precision=32, mac=1, counts=True, bucket=16,
                                          straddle_g=np.repeat(group_mu, 64)[:n])
         g = host(gs)
-        per = g[3:].reshape(W, 3)
+        per = g[4:].reshape(W, 3)
         pad = np.zeros(W * 64 - n, dtype=np.int64)
         assert np.array_equal(per[:, 0], np.concatenate([ni, pad]).reshape(W, 64).sum(1))
         assert np.array_equal(per[:, 1], np.concatenate([nv, pad]).reshape(W, 64).sum(1))
@@ -503,7 +580,7 @@ def test_traverse_pairs_identical(orc, torch_mod, ics32, mac):
         res = []
         for pairs in (False, True):
             f = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
-            st = torch.zeros(3, dtype=torch.int64, device="cuda")
+            st = torch.zeros(4, dtype=torch.int64, device="cuda")
             # an odd-sized slice exercises partial groups and half-empty waves
             traverse(*P, nodes, root, f, 0.5, 1.0, 0.02 / 32, 1.0, off=37, count=n - 37 - 70,
                      mac=mac, theta_group=None if group_theta is None else group_theta,
@@ -537,7 +614,7 @@ def test_traverse_launch_order(orc, torch_mod, ics32):
     res = []
     for go in (None, order):
         f = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
-        st = torch.zeros(3 + 3 * W, dtype=torch.int64, device="cuda")
+        st = torch.zeros(4 + 3 * W, dtype=torch.int64, device="cuda")
         traverse(*P, nodes, root, f, 0.5, 1.0, 0.02 / 32, 1.0, mac=1, theta_group=tg, stats=st,
                  group_stats=True, group_order=go)
         res.append((host(f), host(st)))
@@ -561,7 +638,7 @@ def test_traverse_group_theta_and_compensated(orc, torch_mod, ics32):
     out = {}
     for comp in (False, True):
         force = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
-        stats = torch.zeros(3, dtype=torch.int64, device="cuda")
+        stats = torch.zeros(4, dtype=torch.int64, device="cuda")
         traverse(*P, nodes, root, force, 0.5, 1.0, soft, 1.0, mac=1, theta_group=dev(tg, torch),
                  compensated=comp, stats=stats)
         out[comp] = (host(force), host(stats))

@@ -366,7 +366,9 @@ def test_stack_overflow_drops_children(orc):
     drops children (physics_kernels.py:143): the particles that reach it
     lose exactly the two interactions with the pair.  With m = 40 the stack
     stays below 64 and every particle interacts with every other one beyond
-    the 1e-7 cut, as in the direct sum."""
+    the 1e-7 cut, as in the direct sum.  The oracle's drop counter reads
+    exactly one dropped node per particle (the pair's parent) at m = 63 and
+    none at m = 40, for both acceptance tests."""
     for m, drops in ((63, True), (40, False)):
         pos = _code_chain(m)
         codes, _ = orc.compute_codes(*pos)
@@ -384,6 +386,13 @@ def test_stack_overflow_drops_children(orc):
             assert set(np.unique(lost)) == {0, 2} and np.count_nonzero(lost) > 10
         else:
             assert np.array_equal(ni, beyond)
+        for mac in (0, 1):
+            _, ni2, _, nd = orc.compute_forces(*p, ch, nm, com, nmin, nmax, 0.0, 1.0, 1e-3, 1.0,
+                                               m + 2, precision=64, mac=mac, counts=True,
+                                               drops=True)
+            assert np.array_equal(nd, np.full(m + 2, 1 if drops else 0))
+            if mac == 0:
+                assert np.array_equal(ni2, ni)
 
 
 def test_direct_sum_three_body(orc):
