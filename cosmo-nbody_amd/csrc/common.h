@@ -54,11 +54,24 @@ constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 
 // Split traversal (small launches): at most this many top-tree subtrees per
-// wavefront group, top-tree paths of at most kSplitMaxPath nodes, and at
-// most kSplitMaxCount particles per launch.
-constexpr int kSplitMaxFrontier = 64;
+// wavefront group (one per thread of the frontier builder), top-tree paths
+// of at most kSplitMaxPath nodes, and at most kSplitMaxCount particles per
+// launch.  A launch of G groups gets F = min(kSplitMaxFrontier,
+// kSplitWaveBudget / G) subtrees per group, so F * G <= kSplitWaveBudget
+// wavefronts (G <= kSplitMaxCount / 64 = 1024 keeps F >= 16) and the
+// per-subtree partial sums fit kSplitWaveBudget * 64 * 3 floats.
+constexpr int kSplitMaxFrontier = 256;
 constexpr int kSplitMaxPath = 24;
 constexpr int64_t kSplitMaxCount = 65536;
+constexpr int64_t kSplitWaveBudget = 16384;
+static_assert(kSplitWaveBudget / (kSplitMaxCount / 64) >= 1, "one subtree per group at least");
+static_assert(kSplitWaveBudget >= kSplitMaxFrontier * 64, "partial buffer bound");
+
+__host__ __device__ inline int split_fanout(int64_t count) {
+  const int64_t groups = (count + 63) / 64;
+  const int64_t f = kSplitWaveBudget / groups;
+  return static_cast<int>(f < 1 ? 1 : (f > kSplitMaxFrontier ? kSplitMaxFrontier : f));
+}
 
 struct SplitFrontier {
   int count;
@@ -93,5 +106,5 @@ struct cosmo_ctx {
   void *nodes;          // packed records, cosmo_tree_bytes(n_max, AABB)
   int32_t *scratch_i32; // [4 n_max + 4] validator counters
   void *split_frontier; // cosmo::SplitFrontier of the split traversal
-  float *split_partial; // [kSplitMaxFrontier][min(n_max, kSplitMaxCount)][3]
+  float *split_partial; // [F][count][3] <= [kSplitWaveBudget * 64][3] (see split_fanout)
 };

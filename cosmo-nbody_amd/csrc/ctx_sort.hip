@@ -293,8 +293,10 @@ int cosmo_ctx_create(int64_t n_max, cosmo_ctx **out) {
   if (e == hipSuccess) e = hipMalloc(&c->scratch_i32, sizeof(int32_t) * (4 * n_max + 4));
   if (e == hipSuccess) e = hipMalloc(&c->split_frontier, sizeof(SplitFrontier));
   if (e == hipSuccess)
-    e = hipMalloc(&c->split_partial, sizeof(float) * 3 * kSplitMaxFrontier *
-                                         std::min<int64_t>(n_max, kSplitMaxCount));
+    e = hipMalloc(&c->split_partial,
+                  sizeof(float) * 3 *
+                      std::min<int64_t>(kSplitWaveBudget * 64,
+                                        kSplitMaxFrontier * std::min<int64_t>(n_max, kSplitMaxCount)));
   if (e != hipSuccess) {
     set_hip_error(e);
     cosmo_ctx_destroy(c);

@@ -561,47 +561,53 @@ __global__ __launch_bounds__(256) void k_traverse(
 // the subtree sums in subtree order: the same terms, associated
 // differently.  Reference test only (packed 32-byte records per node id).
 
-// The frontier: a cut of the top tree into at most `target` subtrees (one
-// wavefront, one entry per lane), kept in depth-first order (c1 before c2).
-// Children come from the records (c1, c2 at words 6 and 7).  Each entry
-// carries its leaf range: the root of a tree built by k_build_tree is
-// internal node 0 (id n_tree) over [0, n_tree - 1], and an internal node's
-// c1 ends its range at the split index (c1's leaf id, or its internal index
-// = id - n_tree).  Subtrees are split largest-first by weight = leaves,
-// times `focus` for a subtree that holds some of the launch's own
-// particles [lo, hi): the near field of every group lies there, and its
-// walks are the longest of the launch.  Each round expands the entries
-// within a factor 2 of the heaviest one (heaviest first when they exceed
-// the budget), so the frontier takes a few dependent record loads, not one
-// per expansion.  Any cut gives the same visits and interactions per
-// particle; the weight only shapes the work per wavefront.  With another
-// root the ranges are unknown and the weight falls back to 2^-depth
-// (breadth-first).
+// The frontier: a cut of the top tree into at most `target` subtrees, built
+// by one block with one entry per thread, kept in depth-first order (c1
+// before c2).  Children come from the records (c1, c2 at words 6 and 7).
+// Each entry carries its leaf range: the root of a tree built by
+// k_build_tree is internal node 0 (id n_tree) over [0, n_tree - 1], and an
+// internal node's c1 ends its range at the split index (c1's leaf id, or its
+// internal index = id - n_tree).  Subtrees are split largest-first by
+// weight = leaves, times `focus` for a subtree that holds some of the
+// launch's own particles [lo, hi): the near field of every group lies
+// there, and its walks are the longest of the launch.  Each round expands
+// the entries within a factor 2 of the heaviest one (heaviest first when
+// they exceed the budget), so the frontier takes a few dependent record
+// loads, not one per expansion.  Any cut gives the same visits and
+// interactions per particle; the weight only shapes the work per
+// wavefront.  With another root the ranges are unknown and the weight falls
+// back to 2^-depth (breadth-first).
 //
 // Expanded nodes go to an append-only history (node, stack slot, history
 // index of its parent, whether it is its parent's c1); an entry keeps the
 // history index of its parent and whether it is a c1.  Paths and leftmost
-// masks are read back from the history once, at the end.
-__global__ __launch_bounds__(64) void k_split_frontier(const void *__restrict__ tree, int n_tree,
-                                                       const int32_t *__restrict__ root_idx,
-                                                       int target, int64_t lo, int64_t hi,
-                                                       uint64_t focus,
-                                                       SplitFrontier *__restrict__ fr) {
+// masks are read back from the history once, at the end.  The block's
+// waves agree on the round's maximum weight and on each entry's new
+// position (entries before it that expand) through LDS.
+constexpr int kFrontierThreads = kSplitMaxFrontier;
+constexpr int kFrontierWaves = kFrontierThreads / kWave;
+
+__global__ __launch_bounds__(kFrontierThreads) void k_split_frontier(
+    const void *__restrict__ tree, int n_tree, const int32_t *__restrict__ root_idx, int target,
+    int64_t lo, int64_t hi, uint64_t focus, SplitFrontier *__restrict__ fr) {
+  constexpr int kT = kFrontierThreads;
   constexpr int kHist = kSplitMaxFrontier;  // at most target - 1 expansions
-  __shared__ int s_node[2][kSplitMaxFrontier], s_slot[2][kSplitMaxFrontier];
-  __shared__ int s_plen[2][kSplitMaxFrontier], s_hpar[2][kSplitMaxFrontier];
-  __shared__ int s_left[2][kSplitMaxFrontier];
-  __shared__ int s_rlo[2][kSplitMaxFrontier], s_rhi[2][kSplitMaxFrontier];
+  __shared__ int s_node[2][kT], s_slot[2][kT], s_plen[2][kT], s_hpar[2][kT], s_left[2][kT];
+  __shared__ int s_rlo[2][kT], s_rhi[2][kT];
   __shared__ int h_node[kHist], h_slot[kHist], h_par[kHist], h_left[kHist];
-  __shared__ uint64_t s_w[kSplitMaxFrontier];
+  __shared__ uint64_t s_w[kT];
+  __shared__ uint64_t s_wmax[kFrontierWaves];
+  __shared__ int s_nexp[kFrontierWaves];
   const int t = threadIdx.x;
+  const int wid = t / kWave;
+  const bool lead = (t & (kWave - 1)) == 0;
   const int32_t *words = reinterpret_cast<const int32_t *>(tree);
   const int root = root_idx[0];
   const bool ranged = root == n_tree;
   int cnt = root < 0 ? 0 : 1;
   int hcnt = 0;
   int cur = 0;
-  // this lane's entry
+  // this thread's entry
   int node = root, slot = 0, plen = 0, hpar = -1, left = 0, rlo = 0, rhi = n_tree - 1;
   for (int round = 0; round < 2 * kSplitMaxPath && cnt < target; ++round) {
     const bool have = t < cnt;
@@ -616,28 +622,50 @@ __global__ __launch_bounds__(64) void k_split_frontier(const void *__restrict__ 
         w = uint64_t(1) << (kSplitMaxPath - plen);
       }
     }
-    uint64_t wmax = w;
+    uint64_t wm = w;
 #pragma unroll
     for (int m = 1; m < kWave; m <<= 1) {
-      const uint64_t o = __shfl_xor(wmax, m);
-      wmax = o > wmax ? o : wmax;
+      const uint64_t o = __shfl_xor(wm, m);
+      wm = o > wm ? o : wm;
     }
-    if (wmax == 0) break;
+    if (lead) s_wmax[wid] = wm;
+    __syncthreads();
+    uint64_t wmax = 0;
+#pragma unroll
+    for (int u = 0; u < kFrontierWaves; ++u) wmax = s_wmax[u] > wmax ? s_wmax[u] : wmax;
+    if (wmax == 0) break;  // the same value in every thread
     bool expand = w != 0 && 2 * w > wmax;
     const int budget = target - cnt;
-    if (__popcll(__ballot(expand)) > budget) {  // heaviest first, ties by position
-      s_w[t] = expand ? w : 0;
-      __syncthreads();
+    {
+      const int pc = __popcll(__ballot(expand));  // ballot with every lane active
+      if (lead) s_nexp[wid] = pc;
+    }
+    s_w[t] = expand ? w : 0;
+    __syncthreads();
+    int total = 0;
+#pragma unroll
+    for (int u = 0; u < kFrontierWaves; ++u) total += s_nexp[u];
+    if (total > budget) {  // heaviest first, ties by position
       int rank = 0;
-      for (int u = 0; u < kWave; ++u) {
+      for (int u = 0; u < cnt; ++u) {
         const uint64_t wu = s_w[u];
         rank += (wu > w || (wu == w && u < t)) ? 1 : 0;
       }
-      __syncthreads();
       expand = expand && rank < budget;
+      const int pc = __popcll(__ballot(expand));
+      __syncthreads();
+      if (lead) s_nexp[wid] = pc;
+      __syncthreads();
     }
-    const uint64_t xm = __ballot(expand);
-    const int below = __popcll(xm & lanemask_lt());
+    // new position: this entry moves up by the number of entries before it
+    // that expand (each adds one)
+    int below = __popcll(__ballot(expand) & lanemask_lt());
+    int nexp = 0;
+#pragma unroll
+    for (int u = 0; u < kFrontierWaves; ++u) {
+      below += u < wid ? s_nexp[u] : 0;
+      nexp += s_nexp[u];
+    }
     const int pos = t + below;
     const int nxt = cur ^ 1;
     if (have) {
@@ -674,8 +702,8 @@ __global__ __launch_bounds__(64) void k_split_frontier(const void *__restrict__ 
         s_rhi[nxt][pos] = rhi;
       }
     }
-    hcnt += __popcll(xm);
-    cnt += __popcll(xm);
+    hcnt += nexp;
+    cnt += nexp;
     cur = nxt;
     __syncthreads();
     if (t < cnt) {
@@ -838,29 +866,45 @@ __global__ __launch_bounds__(256) void k_traverse_split(
   }
 }
 
-// force = G * (sum of the subtree partial sums in subtree order).
-__global__ __launch_bounds__(256) void k_split_reduce(const float *__restrict__ partial,
-                                                      const SplitFrontier *__restrict__ fr,
-                                                      int64_t lo, int64_t hi, float G,
-                                                      float *__restrict__ force) {
-  // one thread per force component (particle j, axis c at e = 3 j + c):
-  // consecutive threads read consecutive words of every subtree's block
-  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+// force = G * (sum of the subtree partial sums).  A block takes 32
+// consecutive force components; its 8 waves' halves each sum one contiguous
+// chunk of the subtrees in subtree order, and the chunk sums are added in
+// chunk order: a fixed association, independent of scheduling, with eight
+// times the loads in flight of one thread per component.
+constexpr int kReduceComps = 32;
+constexpr int kReduceChunks = 8;
+
+__global__ __launch_bounds__(kReduceComps * kReduceChunks) void k_split_reduce(
+    const float *__restrict__ partial, const SplitFrontier *__restrict__ fr, int64_t lo,
+    int64_t hi, float G, float *__restrict__ force) {
+  __shared__ float s_sum[kReduceChunks][kReduceComps];
+  const int c = threadIdx.x % kReduceComps, k = threadIdx.x / kReduceComps;
+  const int64_t e = int64_t(blockIdx.x) * kReduceComps + c;  // component 3 j + axis
   const int64_t m3 = (hi - lo) * 3;
-  if (e >= m3) return;
   const int nf = fr->count;
-  const float *o = partial + e;
-  float s = 0.0f;
-  int q = 0;
-  for (; q + 8 <= nf; q += 8) {  // eight loads in flight, added in subtree order
-    float v[8];
+  const int per = (nf + kReduceChunks - 1) / kReduceChunks;
+  const int q0 = k * per, q1 = min(nf, q0 + per);
+  float sum = 0.0f;
+  if (e < m3) {
+    const float *o = partial + e;
+    int q = q0;
+    for (; q + 4 <= q1; q += 4) {  // four loads in flight, added in subtree order
+      float v[4];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = o[int64_t(q + u) * m3];
+      for (int u = 0; u < 4; ++u) v[u] = o[int64_t(q + u) * m3];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) s += v[u];
+      for (int u = 0; u < 4; ++u) sum += v[u];
+    }
+    for (; q < q1; ++q) sum += o[int64_t(q) * m3];
   }
-  for (; q < nf; ++q) s += o[int64_t(q) * m3];
-  force[3 * lo + e] = G * s;
+  s_sum[k][c] = sum;
+  __syncthreads();
+  if (k == 0 && e < m3) {
+    float tot = s_sum[0][c];
+#pragma unroll
+    for (int u = 1; u < kReduceChunks; ++u) tot += s_sum[u][c];
+    force[3 * lo + e] = G * tot;
+  }
 }
 
 // FP64 direct sum: a block of 1024 threads takes 64 targets (one per lane);
@@ -1195,9 +1239,10 @@ int cosmo_traverse_split(cosmo_ctx *ctx, const float *px, const float *py, const
   const float sf = static_cast<float>(soft);
   p.eps2 = sf * sf;
   const int64_t groups = (hi - lo + kWave - 1) / kWave;
-  // about 8192 wavefronts in all (at most kSplitMaxFrontier per group)
-  const int F = static_cast<int>(std::max<int64_t>(
-      1, std::min<int64_t>(kSplitMaxFrontier, 8192 / groups)));
+  // F subtrees per group, at most kSplitWaveBudget wavefronts in all: the
+  // heaviest subtree walks (the near field of the groups at the batch's
+  // spatial boundary) bound the launch, and a finer cut shortens them
+  const int F = split_fanout(hi - lo);
   hipStream_t s = as_stream(stream);
   SplitFrontier *fr = reinterpret_cast<SplitFrontier *>(ctx->split_frontier);
   // weight of the subtrees that hold the launch's particles: half the
@@ -1205,8 +1250,8 @@ int cosmo_traverse_split(cosmo_ctx *ctx, const float *px, const float *py, const
   // (the reference harness's 5000-particle batches at 250k-2M particles
   // and theta 0.5-0.7 run fastest near it)
   const uint64_t focus = static_cast<uint64_t>(std::max<int64_t>(8, n_tree / (hi - lo) / 2));
-  k_split_frontier<<<1, kWave, 0, s>>>(nodes, static_cast<int>(n_tree), root_idx, F, lo, hi,
-                                        focus, fr);
+  k_split_frontier<<<1, kFrontierThreads, 0, s>>>(nodes, static_cast<int>(n_tree), root_idx, F,
+                                                   lo, hi, focus, fr);
   const unsigned grid = blocks_for(groups * F * kWave, 256);
   if (stats)
     k_traverse_split<true><<<grid, 256, 0, s>>>(px, py, pz, nodes, fr, F, ctx->split_partial, p,
@@ -1214,8 +1259,8 @@ int cosmo_traverse_split(cosmo_ctx *ctx, const float *px, const float *py, const
   else
     k_traverse_split<false><<<grid, 256, 0, s>>>(px, py, pz, nodes, fr, F, ctx->split_partial, p,
                                                  lo, hi, nullptr);
-  k_split_reduce<<<blocks_for(3 * (hi - lo), 256), 256, 0, s>>>(ctx->split_partial, fr, lo, hi, p.G,
-                                                          force);
+  k_split_reduce<<<blocks_for(3 * (hi - lo), kReduceComps), kReduceComps * kReduceChunks, 0, s>>>(
+      ctx->split_partial, fr, lo, hi, p.G, force);
   COSMO_CHECK_LAUNCH();
   return COSMO_OK;
 }
