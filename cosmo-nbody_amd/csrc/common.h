@@ -67,6 +67,10 @@ constexpr int64_t kSplitWaveBudget = 16384;
 static_assert(kSplitWaveBudget / (kSplitMaxCount / 64) >= 1, "one subtree per group at least");
 static_assert(kSplitWaveBudget >= kSplitMaxFrontier * 64, "partial buffer bound");
 
+// Frontiers planned once per tree for a batch loop (see forces.hip): at
+// most this many windows.
+constexpr int64_t kSplitMaxPlans = 1024;
+
 __host__ __device__ inline int split_fanout(int64_t count) {
   const int64_t groups = (count + 63) / 64;
   const int64_t f = kSplitWaveBudget / groups;
@@ -107,4 +111,11 @@ struct cosmo_ctx {
   int32_t *scratch_i32; // [4 n_max + 4] validator counters
   void *split_frontier; // cosmo::SplitFrontier of the split traversal
   float *split_partial; // [F][count][3] <= [kSplitWaveBudget * 64][3] (see split_fanout)
+  void *split_plans;    // cosmo::SplitFrontier[plan_cap]: one per window of a batch loop
+  int64_t plan_cap;     // min(kSplitMaxPlans, ceil(n_max / 64))
+  // what the planned frontiers were built for (plan_epoch 0: none)
+  uint64_t plan_epoch;
+  const int32_t *plan_root;
+  int64_t plan_n, plan_win, plan_span;
+  int plan_fanout;
 };

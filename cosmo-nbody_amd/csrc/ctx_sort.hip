@@ -292,6 +292,8 @@ int cosmo_ctx_create(int64_t n_max, cosmo_ctx **out) {
   if (e == hipSuccess) e = hipMalloc(&c->nodes, cosmo_tree_bytes(n_max, COSMO_MAC_AABB));
   if (e == hipSuccess) e = hipMalloc(&c->scratch_i32, sizeof(int32_t) * (4 * n_max + 4));
   if (e == hipSuccess) e = hipMalloc(&c->split_frontier, sizeof(SplitFrontier));
+  c->plan_cap = std::min<int64_t>(kSplitMaxPlans, (n_max + 63) / 64);
+  if (e == hipSuccess) e = hipMalloc(&c->split_plans, sizeof(SplitFrontier) * c->plan_cap);
   if (e == hipSuccess)
     e = hipMalloc(&c->split_partial,
                   sizeof(float) * 3 *
@@ -310,6 +312,7 @@ int cosmo_ctx_destroy(cosmo_ctx *c) {
   if (!c) return COSMO_OK;
   (void)hipFree(c->split_frontier);
   (void)hipFree(c->split_partial);
+  (void)hipFree(c->split_plans);
   (void)hipFree(c->keys_alt);
   (void)hipFree(c->vals_a);
   (void)hipFree(c->vals_b);

@@ -587,9 +587,17 @@ __global__ __launch_bounds__(256) void k_traverse(
 constexpr int kFrontierThreads = kSplitMaxFrontier;
 constexpr int kFrontierWaves = kFrontierThreads / kWave;
 
+//
+// Block b builds the frontier of the range [lo + b stride, min(n_limit,
+// lo + b stride + span)) into fr[b]: one block for a launch's own range, or
+// one per window of a batch loop's plan.
 __global__ __launch_bounds__(kFrontierThreads) void k_split_frontier(
     const void *__restrict__ tree, int n_tree, const int32_t *__restrict__ root_idx, int target,
-    int64_t lo, int64_t hi, uint64_t focus, SplitFrontier *__restrict__ fr) {
+    int64_t lo0, int64_t span, int64_t stride, int64_t n_limit, uint64_t focus,
+    SplitFrontier *__restrict__ frontiers) {
+  const int64_t lo = lo0 + int64_t(blockIdx.x) * stride;
+  const int64_t hi = lo + span < n_limit ? lo + span : n_limit;
+  SplitFrontier *__restrict__ fr = frontiers + blockIdx.x;
   constexpr int kT = kFrontierThreads;
   constexpr int kHist = kSplitMaxFrontier;  // at most target - 1 expansions
   __shared__ int s_node[2][kT], s_slot[2][kT], s_plen[2][kT], s_hpar[2][kT], s_left[2][kT];
@@ -1150,16 +1158,65 @@ int cosmo_order_groups(const float *theta_group, int64_t groups, int32_t *order,
 // The walk of compute_forces over the reference-test records in ctx->nodes:
 // a launch of at most 1024 groups cannot fill the device with one wavefront
 // per group, so it splits each group's walk over the top tree.
+//
+// Batch loops (engine.py:91-111: one call per 5000 particles on one tree)
+// with a nonzero epoch use planned frontiers: on the first call for an
+// (epoch, root, n, fan-out) one launch builds the frontiers of every window
+// [j W/2, j W/2 + W) of the tree (W = the power of two >= 2 * count, so every
+// range of `count` particles lies inside window floor(off / (W/2))), each
+// weighted towards its window's leaves; later calls of the loop take their
+// window's frontier and skip the per-launch build.  The cut only decides
+// which wavefront walks which subtree: visits and interactions per particle
+// are those of the walk either way.  The caller changes the epoch whenever
+// the records or the root change.
+static int split_walk(cosmo_ctx *ctx, const float *px, const float *py, const float *pz,
+                      const void *nodes, int64_t n_tree, const int32_t *root_idx, float *force,
+                      double theta, double G, double soft, int64_t lo, int64_t hi,
+                      const SplitFrontier *planned, int F_planned, uint64_t *stats,
+                      hipStream_t s);
+
 static int forces_from_ctx_records(cosmo_ctx *ctx, const float *px, const float *py,
                                    const float *pz, float *force, double theta, double G,
                                    double soft, double box, const int32_t *root_idx,
-                                   int64_t off, int64_t count, int64_t n_bodies, void *stream) {
-  const int64_t todo = std::min(off + count, n_bodies) - off;
-  if (todo > 0 && todo <= std::min<int64_t>(ctx->n_max, 1024 * kWave))
-    return cosmo_traverse_split(ctx, px, py, pz, ctx->nodes, n_bodies, root_idx, force, theta,
-                                G, soft, box, off, count, n_bodies, nullptr, stream);
-  return cosmo_traverse(px, py, pz, ctx->nodes, n_bodies, root_idx, force, theta, nullptr, G,
-                        soft, box, off, count, n_bodies, COSMO_MAC_REF, 0, nullptr, stream);
+                                   int64_t off, int64_t count, int64_t n_bodies, uint64_t epoch,
+                                   void *stream) {
+  const int64_t lo = off, hi = std::min(off + count, n_bodies);
+  if (hi <= lo) return COSMO_OK;
+  if (!px || !py || !pz || !root_idx || !force) return COSMO_E_INVALID;
+  if (hi - lo > std::min<int64_t>(ctx->n_max, 1024 * kWave))
+    return cosmo_traverse(px, py, pz, ctx->nodes, n_bodies, root_idx, force, theta, nullptr, G,
+                          soft, box, off, count, n_bodies, COSMO_MAC_REF, 0, nullptr, stream);
+  hipStream_t s = as_stream(stream);
+  const SplitFrontier *planned = nullptr;
+  int F = 0;
+  if (epoch != 0) {
+    int64_t W = 128;
+    while (W < 2 * (hi - lo)) W <<= 1;
+    const int64_t windows = (n_bodies + W / 2 - 1) / (W / 2);
+    if (windows <= ctx->plan_cap) {
+      // a plan built for launches of plan_span particles serves any shorter
+      // launch with the same window (the loop's last batch) at its fan-out
+      if (ctx->plan_epoch != epoch || ctx->plan_root != root_idx || ctx->plan_n != n_bodies ||
+          ctx->plan_win != W || hi - lo > ctx->plan_span) {
+        const int Fp = split_fanout(hi - lo);
+        const uint64_t focus = static_cast<uint64_t>(std::max<int64_t>(8, n_bodies / W));
+        k_split_frontier<<<static_cast<unsigned>(windows), kFrontierThreads, 0, s>>>(
+            ctx->nodes, static_cast<int>(n_bodies), root_idx, Fp, 0, W, W / 2, n_bodies, focus,
+            reinterpret_cast<SplitFrontier *>(ctx->split_plans));
+        COSMO_CHECK_LAUNCH();
+        ctx->plan_epoch = epoch;
+        ctx->plan_root = root_idx;
+        ctx->plan_n = n_bodies;
+        ctx->plan_win = W;
+        ctx->plan_span = hi - lo;
+        ctx->plan_fanout = Fp;
+      }
+      planned = reinterpret_cast<const SplitFrontier *>(ctx->split_plans) + lo / (W / 2);
+      F = ctx->plan_fanout;
+    }
+  }
+  return split_walk(ctx, px, py, pz, ctx->nodes, n_bodies, root_idx, force, theta, G, soft, lo,
+                    hi, planned, F, nullptr, s);
 }
 
 int cosmo_compute_forces(cosmo_ctx *ctx, const float *px, const float *py, const float *pz,
@@ -1176,8 +1233,9 @@ int cosmo_compute_forces(cosmo_ctx *ctx, const float *px, const float *py, const
   const int rc = cosmo_pack_nodes(children, n_mass, n_com, n_min, n_max, 2 * n_bodies - 1, box,
                                   COSMO_MAC_REF, ctx->nodes, stream);
   if (rc != COSMO_OK) return rc;
+  ctx->plan_epoch = 0;  // the records were just rewritten
   return forces_from_ctx_records(ctx, px, py, pz, force, theta, G, soft, box, root_idx, off,
-                                 count, n_bodies, stream);
+                                 count, n_bodies, 0, stream);
 }
 
 int cosmo_compute_forces_f32(cosmo_ctx *ctx, const float *px, const float *py, const float *pz,
@@ -1195,36 +1253,31 @@ int cosmo_compute_forces_f32(cosmo_ctx *ctx, const float *px, const float *py, c
   const int rc = cosmo_pack_nodes_f32(children, n_mass, n_com, n_min, n_max, 2 * n_bodies - 1,
                                       box, COSMO_MAC_REF, ctx->nodes, stream);
   if (rc != COSMO_OK) return rc;
+  ctx->plan_epoch = 0;  // the records were just rewritten
   return forces_from_ctx_records(ctx, px, py, pz, force, theta, G, soft, box, root_idx, off,
-                                 count, n_bodies, stream);
+                                 count, n_bodies, 0, stream);
 }
 
 int cosmo_compute_forces_packed(cosmo_ctx *ctx, const float *px, const float *py,
                                 const float *pz, float *force, double theta, double G,
                                 double soft, double box, const int32_t *root_idx, int64_t off,
-                                int64_t count, int64_t n_bodies, void *stream) {
+                                int64_t count, int64_t n_bodies, uint64_t epoch, void *stream) {
   if (!ctx || off < 0 || count < 0) return COSMO_E_INVALID;
   if (n_bodies > ctx->n_max) return COSMO_E_TOO_LARGE;
   if (n_bodies < 1) return COSMO_OK;
   return forces_from_ctx_records(ctx, px, py, pz, force, theta, G, soft, box, root_idx, off,
-                                 count, n_bodies, stream);
+                                 count, n_bodies, epoch, stream);
 }
 
-int cosmo_traverse_split(cosmo_ctx *ctx, const float *px, const float *py, const float *pz,
-                         const void *nodes, int64_t n_tree, const int32_t *root_idx,
-                         float *force, double theta, double G, double soft, double box,
-                         int64_t off, int64_t count, int64_t n_bodies, uint64_t *stats,
-                         void *stream) {
-  (void)box;
-  if (!ctx) return COSMO_E_INVALID;
-  if (n_bodies < 0 || n_bodies > kMaxBodies || off < 0 || count < 0 || n_tree < 1 ||
-      n_tree > kMaxBodies)
-    return COSMO_E_INVALID;
-  const int64_t lo = off;
-  const int64_t hi = (off + count < n_bodies) ? off + count : n_bodies;
-  if (hi <= lo) return COSMO_OK;
-  if (!px || !py || !pz || !nodes || !root_idx || !force) return COSMO_E_INVALID;
-  if (hi - lo > std::min<int64_t>(ctx->n_max, kSplitMaxCount)) return COSMO_E_TOO_LARGE;
+// The split walk of particles [lo, hi) over the reference-test records
+// `nodes` of an n_tree-leaf tree.  planned: a frontier built with target
+// F_planned for a range that contains [lo, hi), where F_planned * (hi - lo)
+// fits the partial buffer (nullptr: build the launch's own).
+static int split_walk(cosmo_ctx *ctx, const float *px, const float *py, const float *pz,
+                      const void *nodes, int64_t n_tree, const int32_t *root_idx, float *force,
+                      double theta, double G, double soft, int64_t lo, int64_t hi,
+                      const SplitFrontier *planned, int F_planned, uint64_t *stats,
+                      hipStream_t s) {
   TravParams p;
   memset(&p, 0, sizeof p);
   const float thf = static_cast<float>(theta);
@@ -1242,16 +1295,17 @@ int cosmo_traverse_split(cosmo_ctx *ctx, const float *px, const float *py, const
   // F subtrees per group, at most kSplitWaveBudget wavefronts in all: the
   // heaviest subtree walks (the near field of the groups at the batch's
   // spatial boundary) bound the launch, and a finer cut shortens them
-  const int F = split_fanout(hi - lo);
-  hipStream_t s = as_stream(stream);
-  SplitFrontier *fr = reinterpret_cast<SplitFrontier *>(ctx->split_frontier);
-  // weight of the subtrees that hold the launch's particles: half the
-  // number of launches of this size that would cover the tree, at least 8
-  // (the reference harness's 5000-particle batches at 250k-2M particles
-  // and theta 0.5-0.7 run fastest near it)
-  const uint64_t focus = static_cast<uint64_t>(std::max<int64_t>(8, n_tree / (hi - lo) / 2));
-  k_split_frontier<<<1, kFrontierThreads, 0, s>>>(nodes, static_cast<int>(n_tree), root_idx, F,
-                                                   lo, hi, focus, fr);
+  const int F = planned ? F_planned : split_fanout(hi - lo);
+  const SplitFrontier *fr = planned;
+  if (!fr) {
+    SplitFrontier *own = reinterpret_cast<SplitFrontier *>(ctx->split_frontier);
+    // weight of the subtrees that hold the launch's particles: half the
+    // number of launches of this size that would cover the tree, at least 8
+    const uint64_t focus = static_cast<uint64_t>(std::max<int64_t>(8, n_tree / (hi - lo) / 2));
+    k_split_frontier<<<1, kFrontierThreads, 0, s>>>(nodes, static_cast<int>(n_tree), root_idx,
+                                                     F, lo, hi - lo, 0, hi, focus, own);
+    fr = own;
+  }
   const unsigned grid = blocks_for(groups * F * kWave, 256);
   if (stats)
     k_traverse_split<true><<<grid, 256, 0, s>>>(px, py, pz, nodes, fr, F, ctx->split_partial, p,
@@ -1263,6 +1317,25 @@ int cosmo_traverse_split(cosmo_ctx *ctx, const float *px, const float *py, const
       ctx->split_partial, fr, lo, hi, p.G, force);
   COSMO_CHECK_LAUNCH();
   return COSMO_OK;
+}
+
+int cosmo_traverse_split(cosmo_ctx *ctx, const float *px, const float *py, const float *pz,
+                         const void *nodes, int64_t n_tree, const int32_t *root_idx,
+                         float *force, double theta, double G, double soft, double box,
+                         int64_t off, int64_t count, int64_t n_bodies, uint64_t *stats,
+                         void *stream) {
+  (void)box;
+  if (!ctx) return COSMO_E_INVALID;
+  if (n_bodies < 0 || n_bodies > kMaxBodies || off < 0 || count < 0 || n_tree < 1 ||
+      n_tree > kMaxBodies)
+    return COSMO_E_INVALID;
+  const int64_t lo = off;
+  const int64_t hi = (off + count < n_bodies) ? off + count : n_bodies;
+  if (hi <= lo) return COSMO_OK;
+  if (!px || !py || !pz || !nodes || !root_idx || !force) return COSMO_E_INVALID;
+  if (hi - lo > std::min<int64_t>(ctx->n_max, kSplitMaxCount)) return COSMO_E_TOO_LARGE;
+  return split_walk(ctx, px, py, pz, nodes, n_tree, root_idx, force, theta, G, soft, lo, hi,
+                    nullptr, 0, stats, as_stream(stream));
 }
 
 int cosmo_direct_sum(const float *px, const float *py, const float *pz, const float *mass,

@@ -59,7 +59,8 @@ SIGNATURES = {
     "cosmo_pack_nodes_f32": ([_p] * 5 + [_i64, _f64, _int, _p, _p], _int),
     "cosmo_compute_forces_f32": ([_p] * 10 + [_f64] * 4 + [_p, _p, _p, _i64, _i64, _i64, _p],
                                  _int),
-    "cosmo_compute_forces_packed": ([_p] * 5 + [_f64] * 4 + [_p, _i64, _i64, _i64, _p], _int),
+    "cosmo_compute_forces_packed": ([_p] * 5 + [_f64] * 4 + [_p, _i64, _i64, _i64, C.c_uint64,
+                                                              _p], _int),
     "cosmo_grid_add_fixed": ([_p, _p, _i64, _int, _p], _int),
     "cosmo_count_nonfinite": ([_p, _i64, _p, _p], _int),
     "cosmo_zeldovich_work_bytes": ([_i64], _i64),
@@ -166,6 +167,7 @@ class Context:
         self.handle = h
         self.device = torch.cuda.current_device()
         self.packed_key = None  # what the records in cosmo_ctx_nodes were packed from
+        self.packed_epoch = 0   # bumped at every pack: keys the library's planned frontiers
 
     @property
     def nodes_ptr(self):
@@ -186,8 +188,8 @@ class Context:
 _shared_ctx = {}
 
 # Tree generation: bumped by every drop-in that writes tree arrays
-# (reset_tree_buffers, build_tree, compute_multipoles, normalize_com and the
-# packing variants).  compute_forces repacks its node records only when the
+# (reset_tree_buffers, build_tree, compute_multipoles, normalize_com,
+# find_root and the packing variants).  compute_forces repacks its node records only when the
 # generation, the arrays or their torch versions changed since its last pack.
 _tree_generation = [0]
 

@@ -44,14 +44,16 @@ def compute_forces(pos_x, pos_y, pos_z, children, n_mass, n_com, n_min, n_max, f
     or this build's float64 mass and CoM.  The reference calls this once per
     5000-particle batch on one tree (engine.py:91-111): the FP32 records are
     packed into the shared context on the first call for a tree and reused
-    while the arrays are unchanged, so a batch only runs the walk."""
+    while the arrays (and the root) are unchanged, so a batch only runs the
+    walk; each pack starts a new epoch, under which the library plans the
+    split walk's frontiers for the whole batch loop once."""
     n = pos_x.shape[0]
     off = int(off)
     count = (n - off) if _launch is None else _launch[0] * _launch[1]
     m = 2 * n - 1
     ctx = shared_context(n)
     L = load()
-    key = _packed_key(ctx, (children, n_mass, n_com, n_min, n_max), n, box)
+    key = _packed_key(ctx, (children, n_mass, n_com, n_min, n_max, root), n, box)
     if ctx.packed_key != key:
         f = n_mass.dtype
         if f not in (torch.float32, torch.float64) or n_com.dtype != f:
@@ -63,10 +65,11 @@ def compute_forces(pos_x, pos_y, pos_z, children, n_mass, n_com, n_min, n_max, f
                    ptr(n_min, _F32, "n_min", numel=3 * m), ptr(n_max, _F32, "n_max", numel=3 * m),
                    m, float(box), MAC_REF, ctx.nodes_ptr, stream_handle()), "compute_forces")
         ctx.packed_key = key
+        ctx.packed_epoch += 1
     check(L.cosmo_compute_forces_packed(
         ctx.handle, *_xyz(pos_x, pos_y, pos_z, n), ptr(force, _F32, "force", numel=3 * n),
         float(theta), float(G), float(soft), float(box), ptr(root, torch.int32, "root"), off,
-        count, n, stream_handle()), "compute_forces")
+        count, n, ctx.packed_epoch, stream_handle()), "compute_forces")
 
 
 def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0, count=None,

@@ -127,6 +127,7 @@ def normalize_com(n_mass, n_com, n_nodes, _launch=None):
 @kernel
 def find_root(parents, root_idx, _launch=None):
     m = launch_extent(_launch, parents.shape[0])
+    bump_tree_generation()
     check(load().cosmo_find_root(ptr(parents, torch.int32, "parents", numel=m),
                                  ptr(root_idx, torch.int32, "root_idx"), m, stream_handle()),
           "find_root")

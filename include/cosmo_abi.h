@@ -355,13 +355,21 @@ int cosmo_compute_forces_f32(cosmo_ctx *ctx, const float *pos_x,
  * cosmo_pack_nodes_f32 with COSMO_MAC_REF for this n_bodies-leaf tree), for
  * particles [off, off+count).  engine.py:91-111 calls compute_forces once
  * per 5000-particle batch on one tree; packing once per tree and calling
- * this per batch gives the same forces. */
+ * this per batch gives the same forces.
+ * epoch: 0, or a value the caller changes whenever the records in the
+ * context or the root value change (e.g. a count of packs).  With a nonzero
+ * epoch the first call builds the split walk's frontiers for every window
+ * of the tree and later calls under the same epoch, root_idx pointer and
+ * n_bodies reuse them (one launch less per batch).  Forces are those of
+ * the walk in every case; only the association of the per-subtree partial
+ * sums follows the frontier. */
 int cosmo_compute_forces_packed(cosmo_ctx *ctx, const float *pos_x,
                                 const float *pos_y, const float *pos_z,
                                 float *force, double theta, double G,
                                 double soft, double box,
                                 const int32_t *root_idx, int64_t off,
-                                int64_t count, int64_t n_bodies, void *stream);
+                                int64_t count, int64_t n_bodies, uint64_t epoch,
+                                void *stream);
 
 /* src/kernels/physics_kernels.py:155-203 integrate(pos_x, pos_y, pos_z,
  * vel_x, vel_y, vel_z, force, a, H, dt): c_drag = 1 - H*dt and

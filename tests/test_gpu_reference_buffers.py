@@ -153,3 +153,65 @@ def test_compute_forces_repacks_on_change(orc):
         assert rel_rms(host(force), ref) <= 1e-5
         out.append(host(force))
     assert not np.array_equal(out[0], out[1])
+
+
+def test_batch_loop_planned_frontiers(orc):
+    """The batch loop of engine.py:91-111 through the drop-in: the first call
+    for a tree plans the split walk's frontiers for every window and later
+    calls reuse them.  Every batch (the last one shorter) matches the FP32
+    oracle; the same loop through the C ABI with epoch 0 (a frontier per
+    launch) matches it as well; a new root value under the same arrays
+    (root moved to the root's first child with a torch write) is planned
+    again and the forces follow that root."""
+    import torch
+
+    from src.kernels import _hip
+    from src.kernels.physics_kernels import compute_forces
+    rng = np.random.default_rng(5)
+    n = 40000
+    pts = [rng.random(n).astype(np.float32) for _ in range(3)]
+    codes, _ = orc.compute_codes(*pts)
+    perm = orc.argsort_stable(codes)
+    p = [x[perm] for x in pts]
+    ch, par = orc.build_tree(codes[perm])
+    mass = np.full(n, 1.0 / n, np.float32)
+    nm, nc, lo, hi = orc.compute_multipoles(*p, mass, ch, par)
+    com = orc.normalize_com(nm, nc).astype(np.float32)
+    nm32 = nm.astype(np.float32)
+    P = [torch.from_numpy(x).cuda() for x in p]
+    arrays = [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (ch, nm32, com, lo, hi)]
+    root = torch.tensor([n], dtype=torch.int32, device="cuda")
+    theta, G, soft = 0.6, 1.0, 1e-3
+
+    def loop(epoch_zero=False):
+        force = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+        for off in range(0, n, 5000):
+            c = min(5000, n - off)
+            blocks = (c + 255) // 256
+            if not epoch_zero:
+                compute_forces[blocks, 256](*P, *arrays, force, theta, G, soft, 1.0, root, None,
+                                            None, off)
+            else:
+                ctx = _hip.shared_context(n)
+                f32 = torch.float32
+                _hip.check(_hip.load().cosmo_compute_forces_packed(
+                    ctx.handle, *[_hip.ptr(t, f32, "pos") for t in P],
+                    _hip.ptr(force, f32, "force"), theta, G, soft, 1.0,
+                    _hip.ptr(root, torch.int32, "root"), off, blocks * 256, n, 0,
+                    _hip.stream_handle()), "packed")
+        torch.cuda.synchronize()
+        return host(force)
+
+    ref = orc.compute_forces(*p, ch, nm32, com, lo, hi, theta, G, soft, 1.0, n, precision=32)
+    planned = loop()
+    assert rel_rms(planned, ref) <= 5e-6
+    assert rel_rms(loop(epoch_zero=True), ref) <= 5e-6
+    # a second pass under the same epoch reuses the plan: the same bits
+    assert np.array_equal(loop(), planned)
+    # another root value: planned again, forces of the walk from that node
+    sub = int(ch[n][0])
+    root.fill_(sub)
+    ref_sub = orc.compute_forces(*p, ch, nm32, com, lo, hi, theta, G, soft, 1.0, sub,
+                                 precision=32)
+    assert rel_rms(loop(), ref_sub) <= 5e-6
+    assert not np.allclose(ref_sub, ref)
