@@ -18,14 +18,16 @@ Deliberate differences, each printed in the CSV's companion columns:
   cleared inside the interval as the reference clears its float grid.
 - compute_forces keeps the reference's 5000-particle batch loop (:281-296);
   the drop-in packs the node records from the reference-layout arrays on
-  the first call for a tree and reuses them while the arrays are unchanged.
-  The extra columns ``traverse_theta_*`` time the step path: one pack and
-  one launch over all particles.
+  the first call for a tree and reuses them while the arrays are unchanged,
+  and plans the split walk's frontiers once per tree.  Every timed loop
+  starts a new tree generation, so it pays the pack and the plan once, as a
+  simulation step does.  The extra columns ``traverse_theta_*`` time the
+  step path: one pack and one launch over all particles.
 - positions are drawn from a seeded generator (the reference's are
   unseeded).
 
 Run from the package directory: ``python -m benchmark.Performance_Benchmarking
-[--n-values 250000,2000000] [--out benchmark_results.csv]``.
+[--n-values 250000,2000000] [--thetas 0.5] [--out benchmark_results.csv]``.
 """
 from __future__ import annotations
 
@@ -140,8 +142,9 @@ class BenchState:
         torch.cuda.synchronize()
 
 
-def run_kernels(s: BenchState) -> dict:
+def run_kernels(s: BenchState, thetas=None) -> dict:
     from src.kernels.morton import compute_codes
+    from src.kernels._hip import bump_tree_generation
     from src.kernels.physics_kernels import compute_forces, integrate, traverse
     from src.kernels.render_kernels import render_density
     from src.kernels.tree_kernels import (MAC_REF, build_top_tree_cache, build_tree,
@@ -176,8 +179,12 @@ def run_kernels(s: BenchState) -> dict:
         s.pos_x, s.pos_y, s.pos_z, s.grid, _bench_mvp(), RES)))
     # integrate moved the particles; rebuild so the traversals see a valid tree
     s._build_tree()
-    for theta in THETA_LIST:
+    thetas = THETA_LIST if thetas is None else thetas
+    for theta in thetas:
         def batched(theta=theta):
+            # a new tree generation: the loop packs the records and plans the
+            # split walk's frontiers again, as each simulation step does
+            bump_tree_generation()
             s.force.zero_()
             for off in range(0, n, BATCH):
                 c = min(BATCH, n - off)
@@ -192,7 +199,8 @@ def run_kernels(s: BenchState) -> dict:
                      SOFTENING, BOX_SIZE)
         out[f"compute_forces_theta_{theta}"] = timed_gpu(batched)
         out[f"traverse_theta_{theta}"] = timed_gpu(single)
-    out["compute_forces"] = out[f"compute_forces_theta_{THETA_PIE}"]
+    pie = THETA_PIE if THETA_PIE in thetas else thetas[0]
+    out["compute_forces"] = out[f"compute_forces_theta_{pie}"]
     return out
 
 
@@ -237,17 +245,21 @@ def main(argv=None):
     ap.add_argument("--n-values", default=None,
                     help="comma-separated N list (default: the reference's 16 sizes)")
     ap.add_argument("--out", default="benchmark_results.csv")
+    ap.add_argument("--thetas", default=None,
+                    help="comma-separated theta list for the force columns (default: the "
+                         "reference's 0.3,0.5,0.7,0.9,1.1)")
     args = ap.parse_args(argv)
     import torch
     torch.cuda.set_device(0)
     ns = N_VALUES if args.n_values is None else [int(x) for x in args.n_values.split(",")]
+    thetas = THETA_LIST if args.thetas is None else [float(x) for x in args.thetas.split(",")]
     header = (["N"] + KERNEL_COLUMNS + ["compute_forces"]
-              + [f"compute_forces_theta_{t}" for t in THETA_LIST]
-              + [f"traverse_theta_{t}" for t in THETA_LIST])
+              + [f"compute_forces_theta_{t}" for t in thetas]
+              + [f"traverse_theta_{t}" for t in thetas])
     rows = []
     for n in ns:
         s = BenchState(n)
-        r = run_kernels(s)
+        r = run_kernels(s, thetas)
         rows.append([n] + [r[k] for k in header[1:]])
         print(f"N={n:,}: " + ", ".join(f"{k} {r[k]:.3f}" for k in header[1:]), flush=True)
         del s
