@@ -328,8 +328,8 @@ void *cosmo_ctx_nodes(cosmo_ctx *c) { return c ? c->nodes : nullptr; }
 
 int64_t cosmo_tree_bytes(int64_t n, int mac) {
   if (n < 1) return 0;
-  // REF: one record per node id
-  if (mac != COSMO_MAC_AABB) return (2 * n - 1) * int64_t(sizeof(cosmo_inode_ref));
+  // REF: one record per node id and the sentinel record 2n - 1
+  if (mac != COSMO_MAC_AABB) return 2 * n * int64_t(sizeof(cosmo_inode_ref));
   // AABB: the traversal reads a whole internal-record size at every node,
   // leaves included, so the buffer carries one internal record of tail room
   return n * int64_t(sizeof(cosmo_leaf)) + n * int64_t(sizeof(cosmo_inode_aabb));

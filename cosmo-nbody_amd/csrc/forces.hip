@@ -350,12 +350,13 @@ __global__ __launch_bounds__(256) void k_traverse(
     bool open[PPL], hit[PPL], rej[PPL];
     uint64_t om = 0;
     if constexpr (MAC == COSMO_MAC_REF) {
-      // lanes may open the node only while the stack has room (:143), and
-      // only an internal node has children to push (c1 != -1, :144): a leaf
-      // whose test fails on a non-finite distance is used as the reference
-      // uses it (no interaction), never opened
+      // lanes may open the node only while the stack has room (:143).  A
+      // leaf is always used (is_leaf = c1 == -1, :127); its test fails only
+      // on a non-finite distance, and then the lanes that open it push the
+      // two massless sentinel records that are a leaf's children, which
+      // are popped and skipped: no interaction, as in the reference, and
+      // no leaf check in the loop.
       const uint64_t m_room = room ? ~0ull : 0ull;
-      const uint64_t m_kids = __float_as_int(B.z) >= 0 ? ~0ull : 0ull;
       // The reference test with the lane sets kept as scalar masks: each
       // compare is one ballot, the set logic runs on the scalar unit, and
       // the divergent interaction runs under inverse_ballot(mask) (no lane
@@ -394,7 +395,7 @@ __global__ __launch_bounds__(256) void k_traverse(
           m_near = __builtin_amdgcn_ballot_w64(near_ok);
         }
         const uint64_t m_hit = m_in[h] & m_acc & m_near;
-        const uint64_t m_open = m_in[h] & ~m_acc & m_room & m_kids;
+        const uint64_t m_open = m_in[h] & ~m_acc & m_room;
         om |= m_open;
         open[h] = __builtin_amdgcn_inverse_ballot_w64(m_open);
         if (__builtin_amdgcn_inverse_ballot_w64(m_hit)) {
@@ -404,8 +405,8 @@ __global__ __launch_bounds__(256) void k_traverse(
         if (STATS) {
           w_vis += __builtin_popcountll(m_in[h]);
           w_int += __builtin_popcountll(m_hit);
-          // lanes whose children the full stack drops (:143)
-          w_drop += __builtin_popcountll(m_in[h] & ~m_acc & ~m_room & m_kids);
+          // lanes whose children the full stack drops (:143; leaves have none)
+          if (__float_as_int(B.x) >= 0) w_drop += __builtin_popcountll(m_in[h] & ~m_acc & ~m_room);
         }
       }
     } else {
@@ -817,7 +818,7 @@ __global__ __launch_bounds__(256) void k_traverse_split(
       const bool in = depth > top;
       depth = min(depth, top);
       const int below = top > floor_slot ? __builtin_amdgcn_readlane(stk, top - 1) : node;
-      const int c1 = __float_as_int(B.x) >= 0 ? __float_as_int(B.z) : node;
+      const int c1 = __float_as_int(B.z);  // a leaf's: the sentinel record
       const ConstF *rb = (const ConstF *)(tb + int64_t(below) * int64_t(sizeof(cosmo_inode_ref)));
       const ConstF *rc = (const ConstF *)(tb + int64_t(c1) * int64_t(sizeof(cosmo_inode_ref)));
       const float4 Ab = ld4(rb), Bb = ld4(rb + 4), Ac = ld4(rc), Bc = ld4(rc + 4);
@@ -838,9 +839,8 @@ __global__ __launch_bounds__(256) void k_traverse_split(
           w_vis += __builtin_popcountll(__ballot(in));
           w_int += __builtin_popcountll(__ballot(hit));
         }
-        // only an internal node has children to push (c1 != -1,
-        // physics_kernels.py:144)
-        if (__ballot(open) != 0 && __float_as_int(B.z) >= 0) {
+        // (a leaf opened on a non-finite distance pushes two sentinels)
+        if (__ballot(open) != 0) {
           stk = write_lane(stk, __float_as_int(B.w), top);
           stk = write_lane(stk, c1, top + 1);
           depth = open ? top + 2 : depth;
@@ -1067,7 +1067,7 @@ static int traverse_impl(const float *px, const float *py, const float *pz, cons
     return COSMO_E_INVALID;
   if (mac != COSMO_MAC_REF && mac != COSMO_MAC_AABB) return COSMO_E_INVALID;
   // the reference-test walk addresses its records with 32-bit byte offsets
-  if (mac == COSMO_MAC_REF && (2 * n_tree - 1) * int64_t(sizeof(cosmo_inode_ref)) > (int64_t(1) << 32))
+  if (mac == COSMO_MAC_REF && cosmo_tree_bytes(n_tree, COSMO_MAC_REF) > (int64_t(1) << 32))
     return COSMO_E_TOO_LARGE;
   const bool stats_on = (flags & (COSMO_TRAV_STATS | COSMO_TRAV_GROUP_STATS)) != 0;
   if (stats_on && !stats) return COSMO_E_INVALID;

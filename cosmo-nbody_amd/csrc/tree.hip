@@ -400,15 +400,27 @@ __device__ __forceinline__ void write_record(void *__restrict__ out, int64_t n, 
   const float fm = static_cast<float>(nm);
   if (MAC == COSMO_MAC_REF) {
     // One 32-byte record per node id.  physics_kernels.py:129-130 for an
-    // internal node; a leaf has sx2 = -1 (always accepted) and no children.
+    // internal node; a leaf has sx2 = -1 (always accepted).  A leaf's
+    // children are the massless sentinel record 2n - 1 that follows the
+    // nodes: a leaf whose test fails (only on a non-finite distance) and is
+    // opened pushes two sentinels, which the walk pops and skips, so the
+    // loop needs no leaf check before a push.  A node lacking a child
+    // (c1 == -1 is the reference's leaf, physics_kernels.py:127; a missing
+    // c2 only occurs in a malformed tree) is written as a leaf.
+    const int sentinel = static_cast<int>(2 * n - 1);
     float4 *dst = reinterpret_cast<float4 *>(out) + 2 * k;
     dst[0] = make_float4(fx, fy, fz, fm);
-    if (k < n) {
-      dst[1] = make_float4(-1.0f, 0.0f, __int_as_float(-1), __int_as_float(-1));
+    if (k < n || ch.x < 0 || ch.y < 0) {
+      dst[1] = make_float4(-1.0f, 0.0f, __int_as_float(sentinel), __int_as_float(sentinel));
     } else {
       float sx = hi0 - lo0;
       sx = (sx > boxf * 0.5f) ? boxf : sx;
-      dst[1] = make_float4(sx * sx, sx, __int_as_float(ch.x), __int_as_float(ch.y));
+      // |sx^2|: the sign bit marks leaves, also for a NaN extent
+      dst[1] = make_float4(fabsf(sx * sx), sx, __int_as_float(ch.x), __int_as_float(ch.y));
+    }
+    if (k == 2 * n - 2) {  // the sentinel: massless, itself as children
+      dst[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      dst[3] = make_float4(-1.0f, 0.0f, __int_as_float(sentinel), __int_as_float(sentinel));
     }
     return;
   }

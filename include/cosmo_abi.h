@@ -68,11 +68,14 @@ extern "C" {
 
 /* Packed FP32 tree records read by the traversal.  One buffer of
  * cosmo_tree_bytes(n, mac) bytes holds, for the test `mac`:
- *   REF  2n-1 records cosmo_inode_ref (32 B), record k for node id k:
+ *   REF  2n records cosmo_inode_ref (32 B), record k for node id k:
  *          internal nodes (ids n..2n-2): sx = x-extent, replaced by box
  *          when > box/2 (physics_kernels.py:129-130), and sx2 = sx*sx
- *          rounded to FP32; leaves (ids 0..n-1): CoM and mass with
- *          sx2 = -1, sx = 0 and c1 = c2 = -1;
+ *          rounded to FP32; leaves (ids 0..n-1, and any node lacking a
+ *          child): CoM and mass with sx2 = -1, sx = 0 and c1 = c2 = 2n-1,
+ *          the massless sentinel record 2n-1 (its own children), which a
+ *          walk that opens a leaf (only on a non-finite distance) pops and
+ *          skips;
  *   AABB n leaf records (cosmo_leaf, 16 B; node ids 0..n-1): CoM and mass,
  *        then n-1 internal records for node ids n..2n-2:
  *        cosmo_inode_aabb (64 B): s = largest extent of the node's box,

@@ -63,7 +63,7 @@ def test_invalid_arguments_rejected_without_gpu():
     assert L.cosmo_sort_by_code(None, None, None, None, 10, 63, None) == 1
     assert L.cosmo_traverse(None, None, None, None, 10, None, None, 0.5, None, 1.0, 0.0, 1.0, 0,
                             10, 10, 7, 0, None, None) == 1
-    assert L.cosmo_tree_bytes(10, 0) == 19 * 32  # one record per node id
+    assert L.cosmo_tree_bytes(10, 0) == 20 * 32  # one record per node id and the sentinel
     assert L.cosmo_tree_bytes(10, 1) == 10 * 16 + 10 * 64
     assert L.cosmo_render_density(None, None, None, None, None, 0, 10, 20, None) == 1
 
