@@ -279,7 +279,7 @@ class LearnedTheta:
     duration bounds the whole traversal."""
 
     def __init__(self, model, thetas, cost, lam, fallback, theta_min=0.0, mu_factors=None,
-                 keep=None):
+                 keep=None, fallback_mu_factor=1.0):
         factors = [1.0] * len(thetas) if mu_factors is None else list(mu_factors)
         keep = ([k for k, t in enumerate(thetas) if t >= theta_min] if keep is None
                 else list(keep))
@@ -288,7 +288,10 @@ class LearnedTheta:
         self.mu_factors = [float(factors[k]) for k in keep]
         self.cost_np = np.asarray(cost, np.float32)[keep]
         self.lam = float(lam)
+        # a group whose predictions are not finite gets the fixed pair: theta
+        # `fallback` with fallback_mu_factor times the engine's straddle mass
         self.fallback = float(fallback)
+        self.fallback_mu_factor = float(fallback_mu_factor)
         # straddle masses are chosen per group only when the candidates differ
         self.joint = len(set(self.mu_factors)) > 1
         self.mu_group = None
@@ -333,7 +336,9 @@ class LearnedTheta:
         self.model.predict(X, Y)
         if self.joint:
             K.select_theta(Y, G, th, cost, self.lam if lam is None else lam, self.fallback, tg,
-                           nf, mus=mus, fallback_mu=float(engine.straddle_mass), mu_out=mg)
+                           nf, mus=mus,
+                           fallback_mu=float(engine.straddle_mass) * self.fallback_mu_factor,
+                           mu_out=mg)
             self.mu_group = mg
         else:
             K.select_theta(Y, G, th, cost, self.lam if lam is None else lam, self.fallback, tg,
@@ -353,7 +358,7 @@ class LearnedTheta:
     def save(self, path):
         np.savez(path, params=self.model.params.cpu().numpy(), thetas=np.array(self.thetas_list),
                  mu_factors=np.array(self.mu_factors), cost=self.cost_np, lam=self.lam,
-                 fallback=self.fallback)
+                 fallback=self.fallback, fallback_mu_factor=self.fallback_mu_factor)
 
     @classmethod
     def load(cls, path, device="cuda"):
@@ -364,8 +369,9 @@ class LearnedTheta:
         else:
             model = ThetaModel(p, device=device)
         factors = d["mu_factors"].tolist() if "mu_factors" in d.files else None
+        fmu = float(d["fallback_mu_factor"]) if "fallback_mu_factor" in d.files else 1.0
         return cls(model, d["thetas"].tolist(), d["cost"], float(d["lam"]), float(d["fallback"]),
-                   mu_factors=factors)
+                   mu_factors=factors, fallback_mu_factor=fmu)
 
 
 def _time_traversal(engine, theta, theta_group, reps=5, group_order=None, straddle=None,
@@ -503,19 +509,21 @@ def measure_costs(c, state, thetas, log=print, mus=None):
 
 def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0,
              target_theta=None, lambdas=None, log=print, graph=False, mus=None):
-    """Fixed theta against the learned per-group selection on the held-out
-    state `val` at equal relative RMS error (FP64 direct sum on sampled
-    groups).  The candidates are (thetas[k], mus[k]) (mus None: the
-    configuration's straddle mass for all).  The fixed method keeps the
-    configuration's straddle mass; its theta is the largest one within the
-    target on the candidate thetas, refined by bisection.  lambda is swept on
-    a log grid, the cheapest settings meeting the target by the cost model
-    are timed and the fastest is kept.  Reports traversal and full-step
-    times, interactions and the hindsight bound (per group the candidate of
-    least cost + lambda * measured error).  theta* and lambda are tuned on
-    one sample of groups; both errors are then measured again on a disjoint
-    sample of as many groups (err_*_confirm).  Returns (result, LearnedTheta),
-    or (result, None) when the target cannot be met."""
+    """Fixed (theta, straddle mass) against the learned per-group selection
+    on the held-out state `val` at equal relative RMS error (FP64 direct sum
+    on sampled groups).  The candidates are (thetas[k], mus[k]) (mus None:
+    the configuration's straddle mass for all).  The fixed method is the best
+    fixed pair: for every straddle mass of the candidates, the largest theta
+    within the target on the candidate thetas, refined by bisection, is
+    timed, and the fastest pair is the baseline.  lambda is swept on a log
+    grid, the cheapest settings meeting the target by the cost model are
+    timed and the fastest is kept.  Reports traversal and full-step times,
+    interactions and the hindsight bound (per group the candidate of least
+    cost + lambda * measured error).  theta*, the straddle mass and lambda
+    are tuned on one sample of groups; both errors are then measured again
+    on a disjoint sample of as many groups (err_*_confirm).  Returns
+    (result, LearnedTheta), or (result, None) when the target cannot be
+    met."""
     eng = _engine_at(c, *val)
     G = (eng.state.n + 63) // 64
     mu0 = eng.straddle_mass
@@ -533,32 +541,50 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
         e2, _ = probe.errors(t, straddle=m)
         err2_k.append(e2.cpu().numpy())
     th_fixed = sorted(set(float(t) for t in thetas))
-    fixed_curve = []
-    for t in th_fixed:
-        e, inter = probe.rel_rms(t, straddle=mu0)
-        fixed_curve.append((t, e, inter))
-    log("fixed theta curve (theta, rel_rms, interactions): " +
-        ", ".join(f"({t:.2f}, {e:.2e}, {i:.0f})" for t, e, i in fixed_curve))
     if target_theta is not None:
         target = probe.rel_rms(target_theta)[0]
         log(f"target = error of fixed theta {target_theta}: {target:.4e}")
-    # the fixed theta meeting the target: the largest (cheapest) one whose
-    # error is within it, found on the candidate curve and refined by
-    # bisection up to the next candidate (the curve need not be monotone)
-    ok_c = [k for k, (_, e, _) in enumerate(fixed_curve) if e <= target]
-    if not ok_c:
+
+    def fixed_at(mu):
+        # the largest (cheapest) theta whose error is within the target, found
+        # on the candidate curve and refined by bisection up to the next
+        # candidate (the curve need not be monotone), and its traversal time
+        curve = []
+        for t in th_fixed:
+            e, inter = probe.rel_rms(t, straddle=mu)
+            curve.append((t, e, inter))
+        log(f"fixed theta curve at straddle {mu:.3g} (theta, rel_rms, interactions): " +
+            ", ".join(f"({t:.2f}, {e:.2e}, {i:.0f})" for t, e, i in curve))
+        ok_c = [k for k, (_, e, _) in enumerate(curve) if e <= target]
+        if not ok_c:
+            return curve, None
+        k_ok = max(ok_c)
+        th = float(th_fixed[k_ok])
+        if k_ok + 1 < len(th_fixed):
+            t_b = _bisect(lambda t: probe.rel_rms(t, straddle=mu)[0], th_fixed[k_ok],
+                          th_fixed[k_ok + 1], target, iters=14)
+            th = t_b if t_b is not None else th
+        e, inter = probe.rel_rms(th, straddle=mu)
+        t_ms = _time_traversal(eng, th, None, straddle=mu)
+        log(f"  fixed pair meeting {target:.1e}: theta {th:.4f}, straddle {mu:.3g} "
+            f"(error {e:.4e}): traversal {t_ms:.1f} ms")
+        return curve, (th, e, inter, t_ms)
+
+    fixed_by_mu = {}
+    fixed_curve = None
+    for mu in sorted(set(float(m) for m in mus)):
+        curve, best_mu = fixed_at(mu)
+        if mu == float(mu0):
+            fixed_curve = curve
+        if best_mu is not None:
+            fixed_by_mu[mu] = best_mu
+    if not fixed_by_mu:
         return {"reachable": False, "val_state": val, "target": target,
                 "fixed_curve": fixed_curve}, None
-    k_ok = max(ok_c)
-    theta_star = float(th_fixed[k_ok])
-    if k_ok + 1 < len(th_fixed):
-        t_b = _bisect(lambda t: probe.rel_rms(t)[0], th_fixed[k_ok], th_fixed[k_ok + 1], target,
-                      iters=14)
-        theta_star = t_b if t_b is not None else theta_star
-    e_fixed, i_fixed = probe.rel_rms(theta_star)
-    log(f"fixed theta meeting {target:.1e}: {theta_star:.4f} (error {e_fixed:.4e})")
-    t_fixed = _time_traversal(eng, theta_star, None)
-    log(f"fixed-theta traversal: {t_fixed:.1f} ms")
+    mu_star = min(fixed_by_mu, key=lambda m: fixed_by_mu[m][3])
+    theta_star, e_fixed, i_fixed, t_fixed = fixed_by_mu[mu_star]
+    log(f"best fixed pair: theta {theta_star:.4f}, straddle {mu_star:.3g} "
+        f"(error {e_fixed:.4e}), traversal {t_fixed:.1f} ms")
     E = np.stack(err2_k, 1)
     cost_arr = np.asarray(cost, np.float64)
     bound = None
@@ -589,7 +615,8 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
         if e <= target and (bound_kept is None or mc < bound_kept[1]):
             bound_kept = (float(lam), mc)
     lt = LearnedTheta(model, thetas, cost, 1.0, theta_star, keep=keep,
-                      mu_factors=[m / mu0 for m in mus] if mu0 > 0 else None)
+                      mu_factors=[m / mu0 for m in mus] if mu0 > 0 else None,
+                      fallback_mu_factor=(mu_star / mu0) if mu0 > 0 else 1.0)
     vg = torch.from_numpy(vgroups).to(eng.state.pos[0].device)
     cost_sel = np.asarray(lt.cost_np, np.float64)
     pairs_sel = list(zip(lt.thetas_list, [mu0 * f for f in lt.mu_factors]))
@@ -616,7 +643,8 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
     ok = sorted((sw for sw in sweep if sw[1] <= target), key=lambda sw: sw[3])
     if not ok:
         return {"reachable": False, "val_state": val, "target": target,
-                "theta_fixed": theta_star, "theta_min": theta_min, "sweep": sweep}, None
+                "theta_fixed": theta_star, "straddle_fixed": mu_star, "theta_min": theta_min,
+                "sweep": sweep}, None
     best = None
     for lam, _, _, _ in ok[:5]:  # the cheapest settings by the cost model, timed
         tg = lt.theta_groups(eng, lam).clone()
@@ -632,7 +660,7 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
     e_learned, i_learned = probe.rel_rms(0.5, sub(tg), straddle=mu0, straddle_group=sub(mg))
     cprobe = GroupProbe(eng, cgroups)
     cg = torch.from_numpy(cgroups).to(eng.state.pos[0].device)
-    e_fixed_c, _ = cprobe.rel_rms(theta_star)
+    e_fixed_c, _ = cprobe.rel_rms(theta_star, straddle=mu_star)
     e_learned_c, _ = cprobe.rel_rms(
         0.5, tg.index_select(0, cg).contiguous(), straddle=mu0,
         straddle_group=None if mg is None else mg.index_select(0, cg).contiguous())
@@ -649,7 +677,8 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
     e1.record()
     torch.cuda.synchronize()
     t_select = e0.elapsed_time(e1) / 5
-    step_fixed = _time_steps(c.replace(THETA=theta_star), val, None, graph=graph)
+    step_fixed = _time_steps(c.replace(THETA=theta_star, STRADDLE_MASS=mu_star), val, None,
+                             graph=graph)
     log(f"fixed-theta step: {step_fixed:.1f} ms")
     step_learned = _time_steps(c, val, lt, graph=graph)
     log(f"learned-theta step: {step_learned:.1f} ms")
@@ -667,7 +696,10 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
            "joint_straddle": bool(lt.joint), "candidates": len(lt.thetas_list),
            "n": eng.state.n, "val_state": val, "val_groups": int(len(vgroups)),
            "hip_graph_steps": bool(graph),
-           "theta_fixed": theta_star, "err_fixed": e_fixed,
+           "theta_fixed": theta_star, "straddle_fixed": mu_star,
+           "fixed_by_straddle": {f"{m:.4g}": {"theta": v[0], "err": v[1], "traversal_ms": v[3]}
+                                 for m, v in sorted(fixed_by_mu.items())},
+           "err_fixed": e_fixed,
            "inter_fixed": i_fixed, "lambda": lt.lam, "err_learned": e_learned,
            "inter_learned": i_learned, "confirm_groups": int(len(cgroups)),
            "err_fixed_confirm": e_fixed_c, "err_learned_confirm": e_learned_c,
