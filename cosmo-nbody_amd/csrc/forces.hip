@@ -307,9 +307,10 @@ __global__ __launch_bounds__(256) void k_traverse(
     for (int h = 0; h < PPL; ++h) {
       in[h] = depth[h] > top;
       m_in[h] = __builtin_amdgcn_ballot_w64(in[h]);
-      // (keeps the compare ahead of the update, so the old depth needs no copy)
-      __builtin_amdgcn_sched_barrier(0);
-      depth[h] = min(depth[h], top);
+      // min(depth, top): a lane of the popped entry has depth exactly top + 1
+      // (no lane's depth exceeds the stack's size), so the update subtracts
+      // the compare's bit (one borrow-subtract, no copy of the old depth)
+      depth[h] -= in[h] ? 1 : 0;
     }
     // Records are wave-uniform loads read through the constant address space
     // (the records are not written while a traversal runs), so every field
