@@ -57,6 +57,9 @@ def parse():
                          "network on three training states, evaluates on a held-out one")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-theta vs fixed-theta comparison of the default line")
+    ap.add_argument("--error-targets", type=int, default=0,
+                    help="also report the relative RMS force error of the final state "
+                         "against the FP64 direct sum on this many sampled particles")
     ap.add_argument("--shards", type=int, default=8,
                     help="also time the traversal of each of this many contiguous Morton "
                          "slices of the targets against the whole tree (0: skip)")
@@ -327,6 +330,22 @@ def main():
     }
     if shards is not None:
         out["shard_slices"] = shards
+    if eng.theta_group is not None:
+        g = eng.theta_group
+        out["config"]["learned_theta"] = {
+            "network": c.THETA_NETWORK, "theta_group_mean": float(g.mean().item()),
+            "theta_group_min": float(g.min().item()), "theta_group_max": float(g.max().item()),
+            "fallbacks": int(eng.theta_selector.n_fallback)}
+    if args.error_targets > 0:
+        from src.kernels.physics_kernels import direct_sum
+        k = min(args.error_targets, n)
+        tg = torch.from_numpy(np.sort(np.random.default_rng(1234).choice(n, k, replace=False)))
+        tg = tg.to(st.force.device)
+        ref = direct_sum(*st.pos, st.mass, tg, c.G, c.SOFTENING)
+        err = float(((st.force.index_select(0, tg).double() - ref) ** 2).sum().sqrt() /
+                    (ref ** 2).sum().sqrt())
+        out["force_error"] = {"rel_rms_vs_fp64_direct_sum": err, "targets": k,
+                              "sample": "PCG64 seed 1234, final state"}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pos, vel, c, args.cpu_threads)
     if not args.no_learned and c.NAME == "repo_128":
