@@ -70,13 +70,16 @@ __device__ __forceinline__ int write_lane(int v, int value, int slot) {
   return v;
 }
 
+// Neumaier's compensated step: s + x rounded, and its exact rounding error
+// (big - t) + small (Fast2Sum on the larger-magnitude operand) added to c.
+// Written with the operands picked before the sum is overwritten, so the new
+// sum can take the old one's register.
 __device__ __forceinline__ void neumaier_add(float &s, float &c, float x) {
-  const float t = s + x;
-  if (fabsf(s) >= fabsf(x))
-    c += (s - t) + x;
-  else
-    c += (x - t) + s;
-  s = t;
+  const bool sb = fabsf(s) >= fabsf(x);
+  const float big = sb ? s : x;
+  const float small = sb ? x : s;
+  s = s + x;
+  c += (big - s) + small;
 }
 
 // float in the constant address space: uniform loads through it are scalar
@@ -294,6 +297,17 @@ __global__ __launch_bounds__(256) void k_traverse(
   if (MAC == COSMO_MAC_AABB && p.straddle_group)
     straddle_mu =
         __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(p.straddle_group[wave])));
+  // m < straddle_mu as an integer compare of the bits on the scalar unit: a
+  // node reaching the test has m > 0 (massless nodes are skipped first), and
+  // for a non-negative straddle mass the bit patterns order as the values
+  // (+inf included; a NaN m compares above every finite limit and keeps the
+  // guard, as the float compare does).  A negative or NaN limit keeps the
+  // guard for every node, as the float compare does, through bits 0.
+  const int straddle_bits = __builtin_amdgcn_readfirstlane(
+      straddle_mu >= 0.0f ? __float_as_int(straddle_mu) : 0);
+  // internal record k (k >= n_tree) starts at 64 k - 48 n_tree bytes
+  const uint32_t inode_bias =
+      uint32_t(n_tree) * uint32_t(sizeof(Inode) - sizeof(cosmo_leaf));
   int stk = (lane == 0) ? root : 0;  // lane k holds stack slot k
   int top = 1;
 
@@ -310,7 +324,10 @@ __global__ __launch_bounds__(256) void k_traverse(
       // min(depth, top): a lane of the popped entry has depth exactly top + 1
       // (no lane's depth exceeds the stack's size), so the update subtracts
       // the compare's bit (one borrow-subtract, no copy of the old depth)
-      depth[h] -= in[h] ? 1 : 0;
+      if constexpr (MAC == COSMO_MAC_REF)
+        depth[h] -= in[h] ? 1 : 0;
+      else  // (here the compiler made the subtraction a select and a subtract)
+        depth[h] = min(depth[h], top);
     }
     // Records are wave-uniform loads read through the constant address space
     // (the records are not written while a traversal runs), so every field
@@ -332,23 +349,34 @@ __global__ __launch_bounds__(256) void k_traverse(
       C = D = A;
       leaf = __float_as_int(B.x) < 0;  // only read on the rare FP64 recheck
     } else {
+      // 32-bit byte offsets (traverse_impl refuses trees whose records
+      // pass 4 GiB): a leaf's record at 16 node, an internal node's at
+      // 16 n + 64 (node - n) = 64 node - 48 n, one select between two
+      // shifts on the scalar unit.  A leaf's B, C and D hold whatever
+      // follows its 16 bytes (the buffer has tail room); the walk reads
+      // them only for internal nodes: a leaf is accepted by every lane, so
+      // nothing is pushed and its fields go nowhere.
       leaf = node < n_tree;
-      const int64_t ofs = leaf ? int64_t(node) * int64_t(sizeof(cosmo_leaf))
-                               : int64_t(n_tree) * int64_t(sizeof(cosmo_leaf)) +
-                                     int64_t(node - n_tree) * int64_t(sizeof(Inode));
+      const uint32_t ofs = leaf ? uint32_t(node) * uint32_t(sizeof(cosmo_leaf))
+                                : uint32_t(node) * uint32_t(sizeof(Inode)) - inode_bias;
       const ConstF *rec = (const ConstF *)(reinterpret_cast<const char *>(tree) + ofs);
       A = ld4(rec);
       B = ld4(rec + 4);
       C = ld4(rec + 8);
       D = ld4(rec + 12);
-      if (leaf) B = C = D = A;
+      // keep the 64 bytes one load, issued before the massless-node exit
+      // (the compiler would otherwise sink the B, C, D loads below it and
+      // pay a second load latency on every pop); emits no instruction
+      asm volatile("" ::"s"(B.x), "s"(B.y), "s"(B.z), "s"(B.w), "s"(C.x), "s"(C.y), "s"(C.z),
+                   "s"(C.w), "s"(D.x), "s"(D.y), "s"(D.z), "s"(D.w));
     }
     // massless nodes are skipped (integer test on the uniform bits: m <= 0)
     if (__float_as_int(A.w) <= 0) continue;
     // every lane of a popped entry has depth == top, so the reference's
     // `stack_top + 1 < 64` (physics_kernels.py:143) is uniform here
     const bool room = top + 1 < 64;
-    bool open[PPL], hit[PPL], rej[PPL];
+    // lanes that open the node, and (AABB) lanes that reject it, as masks
+    uint64_t openm[PPL], rejm[PPL];
     uint64_t om = 0;
     if constexpr (MAC == COSMO_MAC_REF) {
       // lanes may open the node only while the stack has room (:143).  A
@@ -398,7 +426,8 @@ __global__ __launch_bounds__(256) void k_traverse(
         const uint64_t m_hit = m_in[h] & m_acc & m_near;
         const uint64_t m_open = m_in[h] & ~m_acc & m_room;
         om |= m_open;
-        open[h] = __builtin_amdgcn_inverse_ballot_w64(m_open);
+        openm[h] = m_open;
+        rejm[h] = 0;
         if (__builtin_amdgcn_inverse_ballot_w64(m_hit)) {
           fix_ties(d, A.x, A.y, A.z, x[h], y[h], z[h]);
           accumulate<COMP>(d, A.w, p.eps2, ax[h], ay[h], az[h], cx[h], cy[h], cz[h]);
@@ -411,45 +440,49 @@ __global__ __launch_bounds__(256) void k_traverse(
         }
       }
     } else {
+      const uint64_t m_room = room ? ~0ull : 0ull;
 #pragma unroll
-    for (int h = 0; h < PPL; ++h) {
-      // every lane evaluates the node; only the entry's lanes (in) use it
-      Disp d = displace(A.x, A.y, A.z, x[h], y[h], z[h]);
-      bool accept, near_ok;
-      {
+      for (int h = 0; h < PPL; ++h) {
+        // every lane evaluates the node; only the entry's lanes (m_in) use it
+        Disp d = displace(A.x, A.y, A.z, x[h], y[h], z[h]);
         // (s/theta + dl)^2 < d^2, and every particle of the node sits on the
         // same side of the minimum-image boundary as its CoM in each axis,
         // unless the node is lighter than straddle_mu (a node across the
-        // boundary puts at most its mass on the wrong image)
-        near_ok = d.d2 > 1e-7f;
-        const float r = __builtin_fmaf(B.x, inv_theta[h], B.y);
-        const float hb = p.half_box;
-        // only internal nodes at or above the straddle mass must pass the
-        // guard (a wave-uniform condition); the others skip it
-        bool inside = true;
-        if (!leaf && !(A.w < straddle_mu)) {
-          asm volatile("");
-          inside = (d.dx + C.w < hb) && (d.dx - C.x > -hb) && (d.dy + D.x < hb) &&
-                   (d.dy - C.y > -hb) && (d.dz + D.y < hb) && (d.dz - C.z > -hb);
+        // boundary puts at most its mass on the wrong image).  A leaf is
+        // always accepted.  The lane sets are scalar masks (one ballot per
+        // compare, set logic on the scalar unit).
+        uint64_t m_acc = ~0ull;
+        if (!leaf) {
+          const float r = __builtin_fmaf(B.x, inv_theta[h], B.y);
+          m_acc = __builtin_amdgcn_ballot_w64(r * r < d.d2);
+          // only internal nodes at or above the straddle mass must pass the
+          // guard (a wave-uniform condition); the others skip it
+          if (!(__float_as_int(A.w) < straddle_bits)) {
+            asm volatile("");
+            // the six one-sided tests as one max and one min: the same
+            // decisions (a NaN operand, which fmaxf/fminf would drop, only
+            // arises with a NaN or infinite CoM, whose d^2 is NaN and fails
+            // the distance test anyway)
+            const float hb = p.half_box;
+            const float up = fmaxf(fmaxf(d.dx + C.w, d.dy + D.x), d.dz + D.y);
+            const float dn = fminf(fminf(d.dx - C.x, d.dy - C.y), d.dz - C.z);
+            m_acc &= __builtin_amdgcn_ballot_w64(up < hb) & __builtin_amdgcn_ballot_w64(dn > -hb);
+          }
         }
-        accept = leaf || ((r * r < d.d2) && inside);
+        const uint64_t m_near = __builtin_amdgcn_ballot_w64(d.d2 > 1e-7f);
+        const uint64_t m_hit = m_in[h] & m_acc & m_near;
+        rejm[h] = m_in[h] & ~m_acc;
+        openm[h] = rejm[h] & m_room;
+        om |= openm[h];
+        if (__builtin_amdgcn_inverse_ballot_w64(m_hit)) {
+          fix_ties<true>(d, A.x, A.y, A.z, x[h], y[h], z[h]);
+          accumulate<COMP>(d, A.w, p.eps2, ax[h], ay[h], az[h], cx[h], cy[h], cz[h]);
+        }
+        if (STATS) {
+          w_vis += __builtin_popcountll(m_in[h]);
+          w_int += __builtin_popcountll(m_hit);
+        }
       }
-      hit[h] = in[h] & accept & near_ok;
-      rej[h] = in[h] & !accept;
-      open[h] = rej[h] & room;
-      if (hit[h]) {
-        fix_ties<true>(d, A.x, A.y, A.z, x[h], y[h], z[h]);
-        accumulate<COMP>(d, A.w, p.eps2, ax[h], ay[h], az[h], cx[h], cy[h], cz[h]);
-      }
-    }
-#pragma unroll
-    for (int h = 0; h < PPL; ++h) {
-      om |= __ballot(open[h]);
-      if (STATS) {
-        w_vis += __builtin_popcountll(__ballot(in[h]));
-        w_int += __builtin_popcountll(__ballot(hit[h]));
-      }
-    }
     }
     // AABB records of small nodes carry their leaf range (count > 0): the
     // lanes that reject such a node sum its leaves directly, in leaf order,
@@ -459,12 +492,36 @@ __global__ __launch_bounds__(256) void k_traverse(
     if constexpr (MAC == COSMO_MAC_AABB) bcount = leaf ? 0 : __float_as_int(D.w);
     if (STATS && MAC == COSMO_MAC_AABB && bcount == 0 && !room) {
 #pragma unroll
-      for (int h = 0; h < PPL; ++h) w_drop += __builtin_popcountll(__ballot(rej[h]));
+      for (int h = 0; h < PPL; ++h) w_drop += __builtin_popcountll(rejm[h]);
+    }
+    if constexpr (MAC == COSMO_MAC_AABB) {
+      // AABB: the push with its writes made unconditional.  When no lane
+      // opens the node (or it is a bucket, whose rejecting lanes sum its
+      // leaves below instead) the two writes land in slots at or above the
+      // top, which hold nothing live, the depth select changes no lane and
+      // the top stays.  With one definition of the stack on every path the
+      // compiler keeps the stack and the depths in place (no register
+      // copies per pop).
+      // The second slot is capped at 63: with a full stack (top = 63, no
+      // room, so nothing is pushed) slot 64 would name lane 0 of the
+      // register, the bottom entry.
+      const bool push = bcount == 0 && om != 0;
+      stk = write_lane(stk, __float_as_int(B.w), top);
+      stk = write_lane(stk, __float_as_int(B.z), min(top + 1, kWave - 1));
+#pragma unroll
+      for (int h = 0; h < PPL; ++h)
+        depth[h] = __builtin_amdgcn_inverse_ballot_w64(bcount == 0 ? openm[h] : 0ull) ? top + 2
+                                                                                  : depth[h];
+      top += push ? 2 : 0;
     }
     if (bcount > 0) {
       uint64_t bm = 0;  // lanes that reject the node (stack room is not needed)
+      bool rej[PPL];
 #pragma unroll
-      for (int h = 0; h < PPL; ++h) bm |= __ballot(rej[h]);
+      for (int h = 0; h < PPL; ++h) {
+        bm |= rejm[h];
+        rej[h] = __builtin_amdgcn_inverse_ballot_w64(rejm[h]);
+      }
       if (bm != 0) {
         const int first = __float_as_int(D.z);
         // the bucket's leaf records, read as wave-uniform scalar loads
@@ -474,6 +531,75 @@ __global__ __launch_bounds__(256) void k_traverse(
         float bx[PPL], by[PPL], bz[PPL];
 #pragma unroll
         for (int h = 0; h < PPL; ++h) bx[h] = by[h] = bz[h] = 0.0f;
+        // Near buckets.  Let r = fl(c - p) per axis (c the node's CoM) and
+        // e = max(|c - lo|, |hi - c|) the largest offset of the node's box
+        // from its CoM along that axis.  Every leaf q of the node lies in
+        // [lo, hi], so |q - p| <= |c - p| + e.  When |r| + e < 1/4 for
+        // every axis of every rejecting lane, |q - p| < 1/2 with room to
+        // spare, so fl(q - p) lies strictly inside (-1/2, 1/2): rint gives
+        // 0 and no component ties at 1/2.  The minimum-image displacement is
+        // then the plain difference, bit for bit what the general loop
+        // computes (r - rint(r) = r - 0 = r), and the tie fix never fires.
+        uint64_t far = 0;
+        {
+          const float ex = fmaxf(fabsf(C.x), fabsf(C.w));
+          const float ey = fmaxf(fabsf(C.y), fabsf(D.x));
+          const float ez = fmaxf(fabsf(C.z), fabsf(D.y));
+#pragma unroll
+          for (int h = 0; h < PPL; ++h) {
+            const float mx = fabsf(A.x - x[h]) + ex;
+            const float my = fabsf(A.y - y[h]) + ey;
+            const float mz = fabsf(A.z - z[h]) + ez;
+            // NaN or inf anywhere fails the compare and takes the general loop
+            far |= __ballot(rej[h] & !(fmaxf(fmaxf(mx, my), mz) < 0.25f));
+          }
+        }
+        auto near_leaf = [&](const float4 &L) {
+          // massless leaves are skipped, as in the walk (a wave-uniform
+          // condition folded into the lane mask rather than a branch)
+          const bool live = __float_as_int(L.w) > 0;
+#pragma unroll
+          for (int h = 0; h < PPL; ++h) {
+            Disp d;
+            d.dx = d.rx = L.x - x[h];
+            d.dy = d.ry = L.y - y[h];
+            d.dz = d.rz = L.z - z[h];
+            d.d2 = __builtin_fmaf(d.dz, d.dz, __builtin_fmaf(d.dy, d.dy, d.dx * d.dx));
+            const bool hb = rej[h] & live & (d.d2 > 1e-7f);
+            if (hb) {
+              if (COMP)
+                accumulate<false>(d, L.w, p.eps2, bx[h], by[h], bz[h], cx[h], cy[h], cz[h]);
+              else
+                accumulate<false>(d, L.w, p.eps2, ax[h], ay[h], az[h], cx[h], cy[h], cz[h]);
+            }
+            if (STATS && live) {
+              w_vis += __builtin_popcountll(__ballot(rej[h]));
+              w_int += __builtin_popcountll(__ballot(hb));
+            }
+          }
+        };
+        if (far == 0) {
+          // four leaf records per scalar load, then two, then one
+          int k = 0;
+          for (; k + 4 <= bcount; k += 4) {
+            const float4 L0 = ld4(lrec + 4 * k);
+            const float4 L1 = ld4(lrec + 4 * k + 4);
+            const float4 L2 = ld4(lrec + 4 * k + 8);
+            const float4 L3 = ld4(lrec + 4 * k + 12);
+            near_leaf(L0);
+            near_leaf(L1);
+            near_leaf(L2);
+            near_leaf(L3);
+          }
+          if (k + 2 <= bcount) {
+            const float4 L0 = ld4(lrec + 4 * k);
+            const float4 L1 = ld4(lrec + 4 * k + 4);
+            near_leaf(L0);
+            near_leaf(L1);
+            k += 2;
+          }
+          if (k < bcount) near_leaf(ld4(lrec + 4 * k));
+        } else {
 #pragma unroll 4
         for (int k = 0; k < bcount; ++k) {
           const float4 L = ld4(lrec + 4 * k);
@@ -499,6 +625,7 @@ __global__ __launch_bounds__(256) void k_traverse(
             }
           }
         }
+        }
         if (COMP) {
 #pragma unroll
           for (int h = 0; h < PPL; ++h) {
@@ -510,13 +637,14 @@ __global__ __launch_bounds__(256) void k_traverse(
           }
         }
       }
-    } else if (om != 0) {
+    } else if (MAC == COSMO_MAC_REF && om != 0) {
       // internal nodes have two children (build_tree): push c2, then c1
       // (physics_kernels.py:143-149); without room the reference drops them
       stk = write_lane(stk, __float_as_int(B.w), top);
       stk = write_lane(stk, __float_as_int(B.z), top + 1);
 #pragma unroll
-      for (int h = 0; h < PPL; ++h) depth[h] = open[h] ? top + 2 : depth[h];
+      for (int h = 0; h < PPL; ++h)
+        depth[h] = __builtin_amdgcn_inverse_ballot_w64(openm[h]) ? top + 2 : depth[h];
       top += 2;
     }
   }
@@ -1067,9 +1195,8 @@ static int traverse_impl(const float *px, const float *py, const float *pz, cons
       n_tree > kMaxBodies)
     return COSMO_E_INVALID;
   if (mac != COSMO_MAC_REF && mac != COSMO_MAC_AABB) return COSMO_E_INVALID;
-  // the reference-test walk addresses its records with 32-bit byte offsets
-  if (mac == COSMO_MAC_REF && cosmo_tree_bytes(n_tree, COSMO_MAC_REF) > (int64_t(1) << 32))
-    return COSMO_E_TOO_LARGE;
+  // the walk addresses its records with 32-bit byte offsets
+  if (cosmo_tree_bytes(n_tree, mac) > (int64_t(1) << 32)) return COSMO_E_TOO_LARGE;
   const bool stats_on = (flags & (COSMO_TRAV_STATS | COSMO_TRAV_GROUP_STATS)) != 0;
   if (stats_on && !stats) return COSMO_E_INVALID;
   const int64_t lo = off;
