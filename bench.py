@@ -165,6 +165,30 @@ def learned_line(args):
         "detail": res}, default=float))
 
 
+def timed_step(eng, ev=None):
+    """One engine step (the same stages as SimulationEngine.step plus the
+    density splat) with HIP events between the stages when ev (six events)
+    is given: sort | tree (build, upward pass, packing, and the learned
+    per-group selection of a learned preset) | forces | update | splat."""
+    from src.kernels.physics_kernels import integrate
+    st, c = eng.state, eng.config
+    H, dt = eng.timestep()
+    if ev: ev[0].record()
+    st.sort_buffer_by_morton(c)
+    if ev: ev[1].record()
+    eng.tree_manager.build_packed(st.pos, st.mass, st.codes, c.BOX_SIZE, eng.mac, eng.bucket)
+    eng.select_theta()
+    if ev: ev[2].record()
+    eng.compute_forces()
+    if ev: ev[3].record()
+    integrate(*st.pos, *st.vel, st.force, eng.a, H, dt)
+    if ev: ev[4].record()
+    eng.camera.capture(eng.steps_done, *st.pos)
+    if ev: ev[5].record()
+    eng.a += eng.da
+    eng.steps_done += 1
+
+
 def cpu_baseline(pos, vel, c, threads):
     """One full step of the CPU oracle (C/OpenMP restatement of the reference
     kernels, FP64 force arithmetic) on the host cores."""
@@ -206,7 +230,6 @@ def main():
 
     from src.core.engine import SimulationEngine
     from src.core.state import SimulationState
-    from src.kernels.physics_kernels import integrate
     from src.utils import config as cfg
     from src.utils.cosmology import generate_ics
 
@@ -225,20 +248,7 @@ def main():
     eng = SimulationEngine(c, st)
 
     def one_step(ev=None):
-        H, dt = eng.timestep()
-        if ev: ev[0].record()
-        st.sort_buffer_by_morton(c)
-        if ev: ev[1].record()
-        eng.tree_manager.build_packed(st.pos, st.mass, st.codes, c.BOX_SIZE, eng.mac, eng.bucket)
-        if ev: ev[2].record()
-        eng.compute_forces()
-        if ev: ev[3].record()
-        integrate(*st.pos, *st.vel, st.force, eng.a, H, dt)
-        if ev: ev[4].record()
-        eng.camera.capture(eng.steps_done, *st.pos)
-        if ev: ev[5].record()
-        eng.a += eng.da
-        eng.steps_done += 1
+        timed_step(eng, ev)
 
     for _ in range(args.warmup):
         one_step()

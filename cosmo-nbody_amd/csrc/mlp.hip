@@ -459,15 +459,35 @@ __global__ __launch_bounds__(256) void k_profile_nodes(
   }
 }
 
-// Prefix sums of the six difference rows (one thread per row).
-__global__ void k_profile_scan(const unsigned long long *__restrict__ diff, int nbins,
-                               unsigned long long *__restrict__ prof) {
-  const int row = threadIdx.x;
-  if (row >= 6) return;
-  unsigned long long run = 0ull;
-  for (int b = 0; b < nbins; ++b) {
-    run += diff[row * nbins + b];
-    prof[row * nbins + b] = run;
+// Prefix sums of the six difference rows: one workgroup per row, each
+// thread a contiguous chunk of bins; the chunk totals are scanned in LDS
+// (Hillis-Steele).  Integer sums, so the result does not depend on the
+// association.
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void k_profile_scan(
+    const unsigned long long *__restrict__ diff, int nbins,
+    unsigned long long *__restrict__ prof) {
+  __shared__ unsigned long long part[kScanThreads];
+  const int t = threadIdx.x;
+  const unsigned long long *d = diff + int64_t(blockIdx.x) * nbins;
+  unsigned long long *o = prof + int64_t(blockIdx.x) * nbins;
+  const int per = (nbins + kScanThreads - 1) / kScanThreads;
+  const int b0 = t * per;
+  const int b1 = min(b0 + per, nbins);
+  unsigned long long sum = 0ull;
+  for (int b = b0; b < b1; ++b) sum += d[b];
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    const unsigned long long v = t >= off ? part[t - off] : 0ull;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  unsigned long long run = part[t] - sum;  // exclusive prefix of this chunk
+  for (int b = b0; b < b1; ++b) {
+    run += d[b];
+    o[b] = run;
   }
 }
 
@@ -677,8 +697,8 @@ int cosmo_straddle_profile(const int32_t *parents, const double *n_mass, const f
   if (n > 1)
     k_profile_nodes<<<blocks_for(n - 1, 256), 256, 0, s>>>(parents, n_mass, n_min, n_max, n,
                                                            straddle_mass, nbins, diff);
-  k_profile_scan<<<1, 64, 0, s>>>(diff, nbins,
-                                  reinterpret_cast<unsigned long long *>(profile));
+  k_profile_scan<<<6, kScanThreads, 0, s>>>(diff, nbins,
+                                             reinterpret_cast<unsigned long long *>(profile));
   COSMO_CHECK_LAUNCH();
   return COSMO_OK;
 }

@@ -109,6 +109,12 @@ class SimulationEngine:
         s.sort_buffer_by_morton(self.config)
         self.tree_manager.build_packed(s.pos, s.mass, s.codes, self.config.BOX_SIZE, self.mac,
                                        self.bucket)
+        self.select_theta()
+
+    def select_theta(self):
+        """Per-group (theta, straddle mass) of the learned selection for the
+        current tree, and the traversal's launch order (no-op without a
+        selector)."""
         if self.theta_selector is not None:
             self.theta_group = self.theta_selector.theta_groups(self, a=self._a_dev)
             self.straddle_group = getattr(self.theta_selector, "mu_group", None)

@@ -645,6 +645,28 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
         return {"reachable": False, "val_state": val, "target": target,
                 "theta_fixed": theta_star, "straddle_fixed": mu_star, "theta_min": theta_min,
                 "sweep": sweep}, None
+    # The grid steps lambda by 1.33x, so its cheapest setting can sit well
+    # below the target; as theta* is bisected to the target, lambda is
+    # bisected (in log lambda) between that setting and the next smaller
+    # grid lambda that misses the target, and the result joins the timed
+    # candidates.
+    lam_ok = ok[0][0]
+    below = [sw for sw in sweep if sw[0] < lam_ok and sw[1] > target]
+    if below:
+        lo_l, hi_l = math.log(max(sw[0] for sw in below)), math.log(lam_ok)
+        for _ in range(10):
+            mid = 0.5 * (lo_l + hi_l)
+            tg = lt.theta_groups(eng, math.exp(mid))
+            mg = lt.mu_group
+            e, inter = probe.rel_rms(0.5, sub(tg), straddle=mu0, straddle_group=sub(mg))
+            if e <= target:
+                hi_l = mid
+                refined = (math.exp(mid), e, inter, model_cost(tg, mg))
+            else:
+                lo_l = mid
+        if hi_l < math.log(lam_ok):
+            ok = [refined] + ok
+            log(f"lambda refined to {refined[0]:.4g}: error {refined[1]:.4e}")
     best = None
     for lam, _, _, _ in ok[:5]:  # the cheapest settings by the cost model, timed
         tg = lt.theta_groups(eng, lam).clone()

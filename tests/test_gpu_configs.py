@@ -197,3 +197,36 @@ def test_learned_preset_installs_named_network():
         eng.step()
     assert bool(torch.isfinite(eng.state.force).all())
     assert eng.theta_selector.n_fallback == 0
+
+
+def test_bench_step_runs_learned_selection():
+    """bench.py's timed step on a LEARNED_THETA preset selects the per-group
+    (theta, straddle mass) after the tree build, as SimulationEngine does:
+    from the same state its selection and its accelerations equal those of
+    build_tree() + compute_forces() bit for bit (a step without the
+    selection would run the preset's fixed theta)."""
+    import os
+    import sys
+
+    import torch
+
+    from src.core.engine import SimulationEngine
+    from src.core.state import SimulationState
+    from src.utils import config as cfg
+    from src.utils.cosmology import generate_ics
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, root)
+    import bench
+    c = cfg.preset("learned_128", N_GRID=32, SOFTENING=0.02 / 32, STRADDLE_MASS=5e-5 * 64,
+                   IC_KIND="halos", N_HALOS=8)
+    pos, vel = generate_ics(c, 42)
+    a = SimulationEngine(c, SimulationState(len(pos[0]), pos, vel))
+    b = SimulationEngine(c, SimulationState(len(pos[0]), pos, vel))
+    assert a.theta_group is None
+    bench.timed_step(a)
+    b.build_tree()
+    b.compute_forces()
+    assert a.theta_group is not None and torch.equal(a.theta_group, b.theta_group)
+    assert torch.equal(a.straddle_group, b.straddle_group)
+    assert torch.equal(a.state.force, b.state.force)
+    assert a.steps_done == 1
