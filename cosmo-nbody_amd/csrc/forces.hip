@@ -203,7 +203,7 @@ __device__ __forceinline__ void ref_decide(const float4 &A, const float4 &B, boo
     // otherwise evaluate it for every node and select)
     asm volatile("");
     near_ok = d.d2 > 1e-7f;
-    unsure = in & ((fabsf(d.d2 - 1e-7f) <= 2e-10f) | (fabsf(sx2 - t) <= bnd));
+    unsure = in & (int(fabsf(d.d2 - 1e-7f) <= 2e-10f) | int(fabsf(sx2 - t) <= bnd));
   }
   if (__builtin_expect(unsure, 0)) {
     const double e2 = exact_d2(A.x, A.y, A.z, x, y, z);
