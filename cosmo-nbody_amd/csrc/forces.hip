@@ -600,7 +600,6 @@ __global__ __launch_bounds__(256) void k_traverse(
           }
           if (k < bcount) near_leaf(ld4(lrec + 4 * k));
         } else {
-#pragma unroll 4
         for (int k = 0; k < bcount; ++k) {
           const float4 L = ld4(lrec + 4 * k);
           const float qx = L.x, qy = L.y, qz = L.z;
