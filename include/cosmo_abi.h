@@ -260,7 +260,10 @@ int cosmo_normalize_pack_buckets(const int32_t *children, const int32_t *far,
  * flags has COSMO_TRAV_STATS.  With COSMO_TRAV_GROUP_STATS stats holds
  * 4 + 3*W entries for W = ceil(count / (64 * particles per lane))
  * wavefront groups: the totals accumulate in stats[0..3] and group w's own
- * interactions, visits and pops are written to stats[4+3w .. 4+3w+2]. */
+ * interactions, visits and pops are written to stats[4+3w .. 4+3w+2].
+ * The walk addresses records with 32-bit byte offsets: a tree whose
+ * records (cosmo_tree_bytes) pass 4 GiB returns COSMO_E_TOO_LARGE (about
+ * 2^26 leaves for the reference test, 5.4e7 for the AABB test). */
 int cosmo_traverse(const float *pos_x, const float *pos_y, const float *pos_z,
                    const void *nodes, int64_t n_tree, const int32_t *root_idx,
                    float *force, double theta, const float *theta_group,
