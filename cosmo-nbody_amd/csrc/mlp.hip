@@ -491,6 +491,147 @@ __global__ __launch_bounds__(kScanThreads) void k_profile_scan(
   }
 }
 
+// The straddle-level nodes of up to kMaxProfiles limits in one pass over the
+// nodes (each node read once): profile j gets the node's weights when the
+// node is lighter than limits[j] and its parent is not.  Same sums as
+// k_profile_nodes per limit (integer atomics, order-independent).
+constexpr int kMaxProfiles = 8;
+struct ProfileLimits {
+  double limit[kMaxProfiles];
+  int count;
+};
+
+__global__ __launch_bounds__(256) void k_profile_nodes_multi(
+    const int32_t *__restrict__ parents, const double *__restrict__ n_mass,
+    const float *__restrict__ n_min, const float *__restrict__ n_max, int64_t n,
+    ProfileLimits lim, int nbins, unsigned long long *__restrict__ diff) {
+  const int64_t k = n + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= 2 * n - 1) return;
+  const double m = n_mass[k];
+  const int p = parents[k];
+  const double mp = p >= 0 ? n_mass[p] : 0.0;
+  int b0[3], b1[3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    int lo = static_cast<int>(n_min[3 * k + ax] * static_cast<float>(nbins));
+    int hi = static_cast<int>(n_max[3 * k + ax] * static_cast<float>(nbins));
+    b0[ax] = lo < 0 ? 0 : (lo > nbins - 1 ? nbins - 1 : lo);
+    b1[ax] = hi < 0 ? 0 : (hi > nbins - 1 ? nbins - 1 : hi);
+  }
+  for (int j = 0; j < lim.count; ++j) {
+    const double limit = lim.limit[j];
+    if (!(m < limit)) continue;
+    if (p >= 0 && !(mp >= limit)) continue;
+    const double r = m / limit;
+    const unsigned long long w[2] = {static_cast<unsigned long long>(r * r * kProfScale),
+                                     static_cast<unsigned long long>(r * kProfScale)};
+    unsigned long long *dj = diff + int64_t(j) * 6 * nbins;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        unsigned long long *row = dj + (q * 3 + ax) * nbins;
+        atomicAdd(&row[b0[ax]], w[q]);
+        if (b1[ax] + 1 < nbins) atomicAdd(&row[b1[ax] + 1], 0ull - w[q]);
+      }
+    }
+  }
+}
+
+// One wavefront per group: the group's box from coalesced loads (one
+// particle per lane, exact min/max reductions), the ancestor features by
+// lane 0, and the straddle-profile features of each of the J profiles by
+// lane j.  feat: J blocks of [groups][9]; block j equals what
+// k_group_features computes with profile j (the same operations).
+__global__ __launch_bounds__(256) void k_group_features_multi(
+    const float *__restrict__ px, const float *__restrict__ py, const float *__restrict__ pz,
+    const int32_t *__restrict__ children, const int32_t *__restrict__ parents,
+    const double *__restrict__ n_mass, const double *__restrict__ n_com,
+    const float *__restrict__ n_min, const float *__restrict__ n_max, int64_t n, float a,
+    const float *__restrict__ a_dev, float n_grid, const unsigned long long *__restrict__ prof,
+    int nprof, int nbins, int64_t groups, float *__restrict__ feat) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t g = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+  if (g >= groups) return;  // wave-uniform
+  const int64_t b0 = g * 64;
+  const int64_t b1 = (b0 + 64 < n) ? b0 + 64 : n;
+  const int cnt = static_cast<int>(b1 - b0);
+  // lanes past the group's end repeat its first particle (min/max unchanged)
+  const int64_t i = (b0 + lane < b1) ? b0 + lane : b0;
+  float lo[3] = {px[i], py[i], pz[i]}, hi[3] = {lo[0], lo[1], lo[2]};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      const float l = __shfl_xor(lo[k], off), h = __shfl_xor(hi[k], off);
+      lo[k] = l < lo[k] ? l : lo[k];
+      hi[k] = h > hi[k] ? h : hi[k];
+    }
+  }
+  const float floor_len = 1.0f / (64.0f * n_grid);
+  float ext = 0.f, vol = 1.f, cen[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float e = hi[k] - lo[k];
+    ext = e > ext ? e : ext;
+    vol *= (e > floor_len ? e : floor_len);
+    cen[k] = 0.5f * (hi[k] + lo[k]);
+  }
+  float f[9] = {};
+  f[0] = log2f(fmaxf(ext * n_grid, 1e-6f));
+  f[1] = log2f(static_cast<float>(cnt) / (vol * static_cast<float>(n)));
+  f[2] = a_dev ? a_dev[0] : a;
+  if (lane == 0) {
+    // smallest ancestor of the group's first leaf holding >= cnt particles' mass
+    const double need = (static_cast<double>(cnt) - 0.5) / static_cast<double>(n);
+    int64_t node = b0;
+    while (n_mass[node] < need) {
+      const int p = parents[node];
+      if (p < 0) break;
+      node = p;
+    }
+    for (int lvl = 0; lvl < 2; ++lvl) {
+      const int p = parents[node];
+      float r = 0.f, mlog = 0.f;
+      if (p >= 0) {
+        const int2 ch = reinterpret_cast<const int2 *>(children)[p];
+        const int64_t s = (ch.x == node) ? ch.y : ch.x;
+        const float dx = min_image(static_cast<float>(n_com[3 * s]) - cen[0]);
+        const float dy = min_image(static_cast<float>(n_com[3 * s + 1]) - cen[1]);
+        const float dz = min_image(static_cast<float>(n_com[3 * s + 2]) - cen[2]);
+        const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+        r = node_extent(n_min, n_max, s) / fmaxf(d, 1e-6f);
+        mlog = log2f(fmaxf(static_cast<float>(n_mass[s]) * static_cast<float>(n) /
+                               static_cast<float>(cnt), 1e-6f));
+        node = p;
+      }
+      f[3 + 2 * lvl] = r;
+      f[4 + 2 * lvl] = mlog;
+    }
+  }
+#pragma unroll
+  for (int k = 3; k < 7; ++k) f[k] = __shfl(f[k], 0);
+  // lane j < nprof: the features of profile j (bins at the group centre +
+  // 1/2 in each axis)
+  if (lane < nprof) {
+    const unsigned long long *pj = prof + int64_t(lane) * 6 * nbins;
+    float s2 = 0.f, s1 = 0.f;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      float x = cen[ax] + 0.5f;
+      x -= floorf(x);
+      int b = static_cast<int>(x * static_cast<float>(nbins));
+      b = b < 0 ? 0 : (b > nbins - 1 ? nbins - 1 : b);
+      s2 += static_cast<float>(pj[ax * nbins + b]);
+      s1 += static_cast<float>(pj[(3 + ax) * nbins + b]);
+    }
+    f[7] = log2f(1.0f + s2 * 2.3283064365386963e-10f);
+    f[8] = log2f(1.0f + s1 * 2.3283064365386963e-10f);
+    float *o = feat + (int64_t(lane) * groups + g) * COSMO_MLP_GROUP_FEATURES;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) o[k] = f[k];
+  }
+}
+
 // One thread per group of 64 Morton-consecutive particles: the nine
 // per-group features (the tenth input is the candidate theta).
 __global__ __launch_bounds__(256) void k_group_features(
@@ -740,6 +881,56 @@ int cosmo_group_features_dev(const float *px, const float *py, const float *pz,
   if (!a) return COSMO_E_INVALID;
   return group_features_impl(px, py, pz, children, parents, n_mass, n_com, n_min, n_max, n,
                              0.0, a, n_grid, profile, nbins, feat, stream);
+}
+
+int cosmo_straddle_profiles(const int32_t *parents, const double *n_mass, const float *n_min,
+                            const float *n_max, int64_t n, const double *straddle_masses,
+                            int count, uint64_t *profiles, uint64_t *scratch, int nbins,
+                            void *stream) {
+  if (n < 1 || n > kMaxBodies || nbins < 1 || nbins > COSMO_MLP_PROFILE_MAX_BINS ||
+      count < 1 || count > COSMO_MLP_MAX_PROFILES || !straddle_masses)
+    return COSMO_E_INVALID;
+  if (!parents || !n_mass || !n_min || !n_max || !profiles || !scratch) return COSMO_E_INVALID;
+  ProfileLimits lim;
+  lim.count = count;
+  for (int j = 0; j < kMaxProfiles; ++j) {
+    lim.limit[j] = j < count ? straddle_masses[j] : 0.0;
+    if (j < count && !(straddle_masses[j] > 0.0)) return COSMO_E_INVALID;
+  }
+  hipStream_t s = as_stream(stream);
+  unsigned long long *diff = reinterpret_cast<unsigned long long *>(scratch);
+  const int len = 6 * nbins * count;
+  k_profile_clear<<<blocks_for(len, 256), 256, 0, s>>>(diff, len);
+  if (n > 1)
+    k_profile_nodes_multi<<<blocks_for(n - 1, 256), 256, 0, s>>>(parents, n_mass, n_min, n_max,
+                                                                 n, lim, nbins, diff);
+  // the prefix sums of every row of every profile: one workgroup per row
+  k_profile_scan<<<6 * count, kScanThreads, 0, s>>>(
+      diff, nbins, reinterpret_cast<unsigned long long *>(profiles));
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
+}
+
+int cosmo_group_features_multi(const float *px, const float *py, const float *pz,
+                               const int32_t *children, const int32_t *parents,
+                               const double *n_mass, const double *n_com, const float *n_min,
+                               const float *n_max, int64_t n, double a, const float *a_dev,
+                               double n_grid, const uint64_t *profiles, int count, int nbins,
+                               float *feat, void *stream) {
+  if (n < 1 || n > kMaxBodies || count < 1 || count > COSMO_MLP_MAX_PROFILES ||
+      nbins < 1 || nbins > COSMO_MLP_PROFILE_MAX_BINS)
+    return COSMO_E_INVALID;
+  if (!px || !py || !pz || !children || !parents || !n_mass || !n_com || !n_min || !n_max ||
+      !profiles || !feat)
+    return COSMO_E_INVALID;
+  const int64_t groups = (n + 63) / 64;
+  k_group_features_multi<<<blocks_for(groups * kWave, 256), 256, 0, as_stream(stream)>>>(
+      px, py, pz, children, parents, n_mass, n_com, n_min, n_max, n, static_cast<float>(a),
+      a_dev, static_cast<float>(n_grid), reinterpret_cast<const unsigned long long *>(profiles),
+      count,
+      nbins, groups, feat);
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
 }
 
 int cosmo_assemble_rows(const float *feat, int64_t groups, const float *thetas,

@@ -58,22 +58,28 @@ def candidates(c, thetas=None, pairs=True):
 def feature_blocks(engine, mus_l, feat=None, prof=None, a=None):
     """Group features computed with the straddle profile of each distinct
     straddle mass among the candidates' (mus_l): (feat float[J, G, 9], fidx
-    list of the block of each candidate).  feat and prof (int64 [J, 2,
-    6 * PROFILE_BINS]) are reused when given."""
+    list of the block of each candidate).  feat and prof (int64 [2, J,
+    6 * PROFILE_BINS]: the profiles, then their scratch) are reused when
+    given.  With every distinct straddle mass positive, the J profiles come
+    from one pass over the nodes and the J feature blocks from one pass over
+    the groups."""
     st, tm, c = engine.state, engine.tree_manager, engine.config
     G = (st.n + 63) // 64
     distinct = sorted(set(float(m) for m in mus_l))
+    J = len(distinct)
     dev = st.pos[0].device
     if feat is None:
-        feat = torch.empty((len(distinct), G, K.GROUP_FEATURES), dtype=torch.float32,
-                           device=dev)
+        feat = torch.empty((J, G, K.GROUP_FEATURES), dtype=torch.float32, device=dev)
     if prof is None:
-        prof = torch.empty((len(distinct), 2, 6 * K.PROFILE_BINS), dtype=torch.int64,
-                           device=dev)
-    for j, m in enumerate(distinct):
-        profile = K.straddle_profile(tm, m, prof[j, 0], prof[j, 1]) if m > 0 else None
-        K.group_features(st.pos, tm, engine.a if a is None else a, c.N_GRID, feat[j],
-                         profile=profile)
+        prof = torch.empty((2, J, 6 * K.PROFILE_BINS), dtype=torch.int64, device=dev)
+    a_val = engine.a if a is None else a
+    if all(m > 0 for m in distinct) and J <= K.MAX_PROFILES:
+        K.straddle_profiles(tm, distinct, prof[0], prof[1])
+        K.group_features_multi(st.pos, tm, a_val, c.N_GRID, prof[0], J, out=feat)
+    else:
+        for j, m in enumerate(distinct):
+            profile = K.straddle_profile(tm, m, prof[0, j], prof[1, j]) if m > 0 else None
+            K.group_features(st.pos, tm, a_val, c.N_GRID, feat[j], profile=profile)
     return feat, [distinct.index(float(m)) for m in mus_l]
 
 
@@ -314,7 +320,7 @@ class LearnedTheta:
             J = len(set(mus_l))
             feat = torch.empty((J, G, K.GROUP_FEATURES), dtype=torch.float32, device=device)
             order = torch.empty(G, dtype=torch.int32, device=device)
-            prof = torch.empty((J, 2, 6 * K.PROFILE_BINS), dtype=torch.int64, device=device)
+            prof = torch.empty((2, J, 6 * K.PROFILE_BINS), dtype=torch.int64, device=device)
             distinct = sorted(set(mus_l))
             fidx = torch.tensor([distinct.index(m) for m in mus_l], dtype=torch.int32,
                                 device=device)

@@ -92,6 +92,9 @@ SIGNATURES = {
     "cosmo_straddle_profile": ([_p] * 4 + [_i64, _f64, _p, _p, _int, _p], _int),
     "cosmo_group_features": ([_p] * 9 + [_i64, _f64, _f64, _p, _int, _p, _p], _int),
     "cosmo_group_features_dev": ([_p] * 9 + [_i64, _p, _f64, _p, _int, _p, _p], _int),
+    "cosmo_straddle_profiles": ([_p] * 4 + [_i64, _p, _int, _p, _p, _int, _p], _int),
+    "cosmo_group_features_multi": ([_p] * 9 + [_i64, _f64, _p, _f64, _p, _int, _int, _p, _p],
+                                   _int),
     "cosmo_assemble_rows": ([_p, _i64, _p, _p, _p, _int, _p, _p], _int),
     "cosmo_mlp_forward": ([_p, _i64, _p, _p, _p, _p], _int),
     "cosmo_mlp_grad": ([_p, _p, _i64, _p, _p, _p, _int, _p, _p], _int),

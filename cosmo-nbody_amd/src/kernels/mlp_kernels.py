@@ -19,6 +19,7 @@ _F32, _I16 = torch.float32, torch.int16
 
 
 PROFILE_BINS = 1024
+MAX_PROFILES = 8
 
 
 def straddle_profile(tree, straddle_mass, profile=None, scratch=None, nbins=PROFILE_BINS):
@@ -66,6 +67,56 @@ def group_features(pos, tree, a, n_grid, out=None, profile=None, nbins=PROFILE_B
         ptr(tree.aabb_max, _F32, "n_max", numel=3 * M), n, a_arg, float(n_grid), prof,
         int(nbins), ptr(out, _F32, "feat", numel=G * GROUP_FEATURES), stream_handle()),
         "group_features")
+    return out
+
+
+def straddle_profiles(tree, straddle_masses, profiles=None, scratch=None, nbins=PROFILE_BINS):
+    """Straddle profiles (uint64 [J, 2, 3, nbins]) of the tree for J straddle
+    limits at once (one pass over the nodes); block j equals
+    straddle_profile(tree, straddle_masses[j]).  See cosmo_straddle_profiles."""
+    import ctypes
+    n = tree.n
+    M = 2 * n - 1
+    J = len(straddle_masses)
+    dev = tree.child.device
+    profiles = (torch.empty(J * 6 * nbins, dtype=torch.int64, device=dev) if profiles is None
+                else profiles)
+    scratch = (torch.empty(J * 6 * nbins, dtype=torch.int64, device=dev) if scratch is None
+               else scratch)
+    lim = (ctypes.c_double * J)(*[float(m) for m in straddle_masses])
+    check(load().cosmo_straddle_profiles(
+        ptr(tree.parent, torch.int32, "parents", numel=M),
+        ptr(tree.mass, torch.float64, "n_mass", numel=M),
+        ptr(tree.aabb_min, _F32, "n_min", numel=3 * M),
+        ptr(tree.aabb_max, _F32, "n_max", numel=3 * M), n, ctypes.addressof(lim), J,
+        ptr(profiles, (torch.int64, torch.uint64), "profiles", numel=J * 6 * nbins),
+        ptr(scratch, (torch.int64, torch.uint64), "scratch", numel=J * 6 * nbins), int(nbins),
+        stream_handle()), "straddle_profiles")
+    return profiles
+
+
+def group_features_multi(pos, tree, a, n_grid, profiles, count, out=None, nbins=PROFILE_BINS):
+    """group_features for `count` profiles (straddle_profiles) at once:
+    float32 [count, groups, 9], block j as group_features with profile j.
+    a: the scale factor, a float or a device float tensor."""
+    n = pos[0].shape[0]
+    G = (n + 63) // 64
+    out = (torch.empty((count, G, GROUP_FEATURES), dtype=_F32, device=pos[0].device)
+           if out is None else out)
+    M = 2 * n - 1
+    on_dev = isinstance(a, torch.Tensor)
+    check(load().cosmo_group_features_multi(
+        *[ptr(t, _F32, "pos", numel=n) for t in pos],
+        ptr(tree.child, torch.int32, "children", numel=2 * M),
+        ptr(tree.parent, torch.int32, "parents", numel=M),
+        ptr(tree.mass, torch.float64, "n_mass", numel=M),
+        ptr(tree.com, torch.float64, "n_com", numel=3 * M),
+        ptr(tree.aabb_min, _F32, "n_min", numel=3 * M),
+        ptr(tree.aabb_max, _F32, "n_max", numel=3 * M), n, 0.0 if on_dev else float(a),
+        ptr(a, _F32, "a", numel=1) if on_dev else None, float(n_grid),
+        ptr(profiles, (torch.int64, torch.uint64), "profiles", numel=count * 6 * nbins),
+        int(count), int(nbins), ptr(out, _F32, "feat", numel=count * G * GROUP_FEATURES),
+        stream_handle()), "group_features_multi")
     return out
 
 

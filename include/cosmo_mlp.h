@@ -35,6 +35,7 @@ extern "C" {
 #define COSMO_MLP_BF16_WEIGHTS (16 * 64 + 64 * 64 + 64)
 
 #define COSMO_MLP_PROFILE_MAX_BINS 65536
+#define COSMO_MLP_MAX_PROFILES 8
 
 /* Straddle profile for the AABB test's straddle limit: the straddle-level
  * nodes are the internal nodes lighter than straddle_mass whose parent is
@@ -73,6 +74,30 @@ int cosmo_group_features_dev(const float *pos_x, const float *pos_y,
                              const float *n_max, int64_t n, const float *a,
                              double n_grid, const uint64_t *profile, int nbins,
                              float *feat, void *stream);
+
+/* cosmo_straddle_profile for count (1..COSMO_MLP_MAX_PROFILES) straddle
+ * masses at once (host array straddle_masses[count]), reading each node
+ * once: profiles and scratch are device uint64[count][2 * 3 * nbins], block
+ * j equal to cosmo_straddle_profile with straddle_masses[j]. */
+int cosmo_straddle_profiles(const int32_t *parents, const double *n_mass,
+                            const float *n_min, const float *n_max, int64_t n,
+                            const double *straddle_masses, int count,
+                            uint64_t *profiles, uint64_t *scratch, int nbins,
+                            void *stream);
+
+/* cosmo_group_features for count profiles (cosmo_straddle_profiles) at
+ * once: feat is float[count][ceil(n/64)][9], block j equal to
+ * cosmo_group_features with profile j (features 0-6 are the same in every
+ * block).  a_dev (nullable): the scale factor read from device memory in
+ * place of a, as cosmo_group_features_dev.  One wavefront per group. */
+int cosmo_group_features_multi(const float *pos_x, const float *pos_y,
+                               const float *pos_z, const int32_t *children,
+                               const int32_t *parents, const double *n_mass,
+                               const double *n_com, const float *n_min,
+                               const float *n_max, int64_t n, double a,
+                               const float *a_dev, double n_grid,
+                               const uint64_t *profiles, int count, int nbins,
+                               float *feat, void *stream);
 
 /* X[g*K + k][0..15] = BF16 [thetas[k], F[g][0..8], cand2[k], 0...] with
  * F = feat (float[groups][9]), or, when fidx is given, block fidx[k] of feat

@@ -150,6 +150,39 @@ def test_group_features_match_mirror():
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not gpu_available(), reason="needs a HIP device")
+def test_multi_profile_features_equal_single():
+    """The one-pass profiles of several straddle masses and the
+    one-wavefront-per-group features of all of them equal, bit for bit, the
+    per-mass profiles and features (a group count that is not a multiple of
+    64 particles; the scale factor given on the host and on the device)."""
+    import torch
+    from conftest import zeldovich
+    from src.core.octree import OctreeManager
+    from src.core.state import SimulationState
+    from src.kernels import mlp_kernels as K
+    from src.utils import config as cfg
+    pos, vel = zeldovich(16)
+    n = 4000
+    pos = [p[:n].copy() for p in pos]
+    vel = [v[:n].copy() for v in vel]
+    st = SimulationState(n, pos, vel)
+    st.sort_buffer_by_morton(cfg.preset("small_32"))
+    tree = OctreeManager(n)
+    tree.build(st.pos, st.mass, st.codes)
+    mus = [1e-3, 3e-3, 2e-2, 5e-2, 0.2]
+    profs = K.straddle_profiles(tree, mus).reshape(len(mus), -1)
+    a_dev = torch.tensor([0.02], dtype=torch.float32, device="cuda")
+    for a in (0.02, a_dev):
+        f = K.group_features_multi(st.pos, tree, a, 16, profs, len(mus))
+        for j, mu in enumerate(mus):
+            one = K.straddle_profile(tree, mu)
+            assert torch.equal(profs[j], one)
+            fj = K.group_features(st.pos, tree, 0.02, 16, profile=one)
+            assert torch.equal(f[j], fj)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a HIP device")
 def test_select_theta_and_nan_fallback():
     import torch
     from src.kernels import mlp_kernels as K
