@@ -336,7 +336,7 @@ __global__ __launch_bounds__(256) void k_traverse(
     // selection in the loop.  AABB: leaf and internal records start with
     // (x, y, z, m); one wave-uniform byte offset is selected for either kind
     // and the whole internal-record size is loaded at once (the buffer has
-    // tail room for the last leaf), then a leaf keeps only its 16 bytes.
+    // tail room for the last leaf); a leaf uses only its first 16 bytes.
     bool leaf;
     float4 A, B, C, D;
     if constexpr (MAC == COSMO_MAC_REF) {
