@@ -50,6 +50,11 @@ def compute_forces(pos_x, pos_y, pos_z, children, n_mass, n_com, n_min, n_max, f
     n = pos_x.shape[0]
     off = int(off)
     count = (n - off) if _launch is None else _launch[0] * _launch[1]
+    # a launch that covers no particle writes nothing (every thread of the
+    # reference's leaves at i = tid + off >= n, physics_kernels.py:48-50,76-77);
+    # a negative offset is left to the library, which refuses it
+    if off >= 0 and (n == 0 or count <= 0 or off >= n):
+        return
     m = 2 * n - 1
     ctx = shared_context(n)
     L = load()

@@ -774,6 +774,130 @@ def test_engine_deterministic(torch_mod):
     assert np.array_equal(host(e1.camera.grid_d), host(e2.camera.grid_d))
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 65])
+def test_tiny_sizes_every_stage(orc, torch_mod, n):
+    """Every stage at the smallest sizes, against the oracle on the same
+    input: n = 1 (the root is the only leaf, id 0, so find_root returns 0
+    and the walk visits it without an interaction), 2 and 3 (roots over
+    leaves only, and over a leaf and an internal node), 65 (a full group
+    and a one-particle group).  Codes, permutation, tree, root, FP64
+    moments, kick-drift and density grid bit for bit; per-particle visits
+    and interactions of both acceptance tests exact; forces of the walk and
+    of the compute_forces drop-in (its split walk) within FP32 rounding of
+    the FP32 oracle, and exactly zero for a lone particle."""
+    torch = torch_mod
+    from src.kernels.morton import compute_codes
+    from src.kernels.physics_kernels import compute_forces, integrate, traverse
+    from src.kernels.render_kernels import render_density
+    from src.kernels.tree_kernels import find_root, normalize_com, pack_nodes
+    from src.utils import config as cfg
+    from src.utils.visualizer import get_mvp_matrix
+    rng = np.random.default_rng(1000 + n)
+    pos = [rng.uniform(0, 1, n).astype(np.float32) for _ in range(3)]
+    vel = [rng.standard_normal(n).astype(np.float32) * np.float32(1e-3) for _ in range(3)]
+    blocks = (n + 255) // 256
+    ref_codes, ref_idx = orc.compute_codes(*pos)
+    codes = torch.zeros(n, dtype=torch.int64, device="cuda")
+    ind = torch.zeros(n, dtype=torch.int32, device="cuda")
+    compute_codes[blocks, 256](*[dev(p, torch) for p in pos], codes, ind, n)
+    assert np.array_equal(host(codes, np.uint64), ref_codes)
+    assert np.array_equal(host(ind, np.uint32), ref_idx)
+    sc, perm = _gpu_sort(torch, ref_codes)
+    rperm = orc.argsort_stable(ref_codes)
+    assert np.array_equal(perm, rperm)
+    assert np.array_equal(sc, ref_codes[rperm])
+    pos = [p[rperm] for p in pos]
+    vel = [v[rperm] for v in vel]
+    ch, par, cnt, root = _gpu_tree(torch, sc)
+    ref_ch, ref_par = orc.build_tree(sc)
+    assert np.array_equal(host(ch), ref_ch)
+    assert np.array_equal(host(par), ref_par)
+    M = 2 * n - 1
+    find_root[(M + 255) // 256, 256](par, root)
+    r = orc.find_root(ref_par)
+    assert int(root.item()) == r == (0 if n == 1 else n)
+    mass = np.full(n, 1.0 / n, np.float32)
+    nm, nc, lo, hi = _gpu_moments(torch, pos, mass, ref_ch, ref_par)
+    r_nm, r_nc, r_lo, r_hi = orc.compute_multipoles(*pos, mass, ref_ch, ref_par)
+    for a, b in ((nm, r_nm), (nc, r_nc), (lo, r_lo), (hi, r_hi)):
+        assert np.array_equal(host(a), b)
+    normalize_com[(M + 255) // 256, 256](nm, nc, M)
+    com = orc.normalize_com(r_nm, r_nc)
+    assert np.array_equal(host(nc), com)
+    soft = 1e-3
+    P = [dev(p, torch) for p in pos]
+    for mac in (0, 1):
+        nodes = pack_nodes(ch, nm, nc, lo, hi, 1.0, mac)
+        force = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+        stats = torch.zeros(4, dtype=torch.int64, device="cuda")
+        traverse(*P, nodes, root, force, 0.5, 1.0, soft, 1.0, mac=mac, stats=stats)
+        f32, ni, nv = orc.compute_forces(*pos, ref_ch, r_nm, com, r_lo, r_hi, 0.5, 1.0, soft,
+                                         1.0, r, precision=32, mac=mac, counts=True)
+        s = host(stats)
+        assert int(s[0]) == int(ni.sum()) and int(s[1]) == int(nv.sum())
+        if n == 1:
+            assert int(ni[0]) == 0 and int(nv[0]) == 1
+            assert not host(force).any()
+        else:
+            assert rel_rms(host(force), f32) <= 2e-6
+    f32, _, _ = orc.compute_forces(*pos, ref_ch, r_nm, com, r_lo, r_hi, 0.5, 1.0, soft, 1.0, r,
+                                   precision=32, counts=True)
+    force = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+    compute_forces[blocks, 256](*P, ch, nm, nc, lo, hi, force, 0.5, 1.0, soft, 1.0, root, None,
+                                None, 0)
+    if n == 1:
+        assert not host(force).any()
+    else:
+        assert rel_rms(host(force), f32) <= 5e-6
+    a, H, dt = 0.02, 23970.3, 6.54e-7
+    V = [dev(v, torch) for v in vel]
+    integrate[blocks, 256](*P, *V, force, a, H, dt)
+    ref = orc.integrate(*pos, *vel, host(force), a, H, dt)
+    for t, q in zip(P + V, ref):
+        assert np.array_equal(host(t), q)
+    mvp = get_mvp_matrix(3, (0.5, 0.5, 0.5), 1.0, cfg.preset("repo_128"))
+    grid = torch.zeros((64, 64), dtype=torch.int64, device="cuda")
+    render_density[blocks, 256](*P, grid, mvp, 64)
+    assert np.array_equal(host(grid, np.uint64), orc.render_density(*ref[:3], mvp, 64))
+
+
+def test_empty_inputs_are_no_ops(torch_mod):
+    """n = 0 at every drop-in (with a one-block launch, as a caller with an
+    empty batch would issue it): no error, nothing written."""
+    torch = torch_mod
+    from src.kernels.morton import compute_codes
+    from src.kernels.physics_kernels import compute_forces, integrate
+    from src.kernels.render_kernels import render_density
+    from src.kernels.tree_kernels import build_tree, compute_multipoles
+    from src.utils import config as cfg
+    from src.utils.visualizer import get_mvp_matrix
+    f0 = [torch.zeros(0, dtype=torch.float32, device="cuda") for _ in range(3)]
+    codes = torch.zeros(0, dtype=torch.int64, device="cuda")
+    ind = torch.zeros(0, dtype=torch.int32, device="cuda")
+    compute_codes[1, 256](*f0, codes, ind, 0)
+    ch = torch.zeros((0, 2), dtype=torch.int32, device="cuda")
+    par = torch.zeros(0, dtype=torch.int32, device="cuda")
+    build_tree[1, 256](codes, ch, par)
+    nm = torch.zeros(0, dtype=torch.float64, device="cuda")
+    nc = torch.zeros((0, 3), dtype=torch.float64, device="cuda")
+    lo = torch.zeros((0, 3), dtype=torch.float32, device="cuda")
+    hi = torch.zeros((0, 3), dtype=torch.float32, device="cuda")
+    cnt = torch.zeros(0, dtype=torch.int32, device="cuda")
+    compute_multipoles[1, 256](*f0, f0[0], ch, par, nm, nc, lo, hi, cnt)
+    force = torch.zeros((0, 3), dtype=torch.float32, device="cuda")
+    root = torch.zeros(1, dtype=torch.int32, device="cuda")
+    compute_forces[1, 256](*f0, ch, nm, nc, lo, hi, force, 0.5, 1.0, 1e-3, 1.0, root, None, None,
+                           0)
+    V = [torch.zeros(0, dtype=torch.float32, device="cuda") for _ in range(3)]
+    integrate[1, 256](*f0, *V, force, 0.02, 23970.3, 6.54e-7)
+    grid = torch.zeros((64, 64), dtype=torch.int64, device="cuda")
+    mvp = get_mvp_matrix(0, (0.5, 0.5, 0.5), 1.0, cfg.preset("repo_128"))
+    render_density[1, 256](*f0, grid, mvp, 64)
+    torch.cuda.synchronize()
+    assert int(grid.abs().sum().item()) == 0
+    assert int(root.item()) == 0
+
+
 def test_full_size_properties(torch_mod):
     """repo_128 (N = 128^3): sorted codes, a permutation, a valid tree with
     root n and root mass 1, finite forces and a non-empty density grid."""
