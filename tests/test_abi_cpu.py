@@ -68,6 +68,39 @@ def test_invalid_arguments_rejected_without_gpu():
     assert L.cosmo_render_density(None, None, None, None, None, 0, 10, 20, None) == 1
 
 
+def test_size_limits_refused_without_gpu():
+    """The walk addresses its records with 32-bit byte offsets: a tree whose
+    records pass 4 GiB is refused with COSMO_E_TOO_LARGE (3) before any
+    launch, at exactly the boundary (2^26 leaves fill 4 GiB of reference-test
+    records; AABB records take 80 bytes per leaf); a tree at the boundary
+    passes the size check and stops at the null pointers (1).  Particle
+    counts above 2^29 are invalid at every stage."""
+    from src.kernels import _hip
+    if not os.path.exists(_hip.LIB_PATH):
+        pytest.skip("libcosmo_hip.so not built")
+    L = _hip.load()
+    ok, invalid, too_large = 0, 1, 3
+
+    def trav(n_tree, mac):
+        return L.cosmo_traverse(None, None, None, None, n_tree, None, None, 0.5, None, 1.0, 0.0,
+                                1.0, 0, 64, 64, mac, 0, None, None)
+
+    assert L.cosmo_tree_bytes(1 << 26, 0) == 1 << 32
+    assert trav(1 << 26, 0) == invalid
+    assert trav((1 << 26) + 1, 0) == too_large
+    aabb_max = (1 << 32) // 80
+    assert L.cosmo_tree_bytes(aabb_max, 1) <= 1 << 32 < L.cosmo_tree_bytes(aabb_max + 1, 1)
+    assert trav(aabb_max, 1) == invalid
+    assert trav(aabb_max + 1, 1) == too_large
+    big = (1 << 29) + 1
+    assert L.cosmo_compute_codes(None, None, None, None, None, big, 0, None) == invalid
+    assert L.cosmo_build_tree(None, None, None, big, None) == invalid
+    assert L.cosmo_build_tree(None, None, None, 1, None) == ok  # one leaf: nothing to build
+    assert L.cosmo_compute_multipoles(None, None, None, None, None, None, None, None, None,
+                                      None, None, big, None) == invalid
+    assert trav(big, 0) == invalid
+
+
 def test_zeldovich_ics_arguments_checked_on_host():
     """Sizes outside [1, 1024] and null pointers are refused before any HIP
     call (so this runs without a GPU)."""
