@@ -31,7 +31,7 @@ FLOP_PER_INTERACTION = 20     # standard particle-node interaction count
 # HBM bytes per traversal launch measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (scaled by a 256 MiB copy: x2.0 and x1.0), per workload; the
 # summary with its commands is profiles/step_kernels_hbm_traffic_128.md.
-TRAVERSAL_TRAFFIC_BYTES = {"repo_128": 901496 * 1024 * 2 + 24576 * 1024}
+TRAVERSAL_TRAFFIC_BYTES = {"repo_128": 332581 * 1024 * 2 + 24576 * 1024}
 
 
 def parse():

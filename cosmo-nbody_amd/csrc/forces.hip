@@ -82,6 +82,26 @@ __device__ __forceinline__ void neumaier_add(float &s, float &c, float x) {
   c += (big - s) + small;
 }
 
+// The hardware hands workgroup b to XCD b mod 8, each XCD with its own L2.
+// xcd_block(b, nb) maps the launch's blocks so that XCD x runs the logical
+// blocks of chunks x, x + 8, x + 16, ... of kXcdChunk consecutive blocks
+// each: Morton-adjacent groups then meet in one L2, and the fine interleave
+// of chunks keeps the XCDs' work balanced.  A bijection on [0, nb): the
+// blocks past the last whole round of 8 chunks keep their own index.
+// Measured on the walk (forces bit-identical): 32 blocks (128 groups, 8192
+// particles) per chunk took the reference-test walk from 4.95 to 4.72 ms at
+// 128^3 and 40.5 to 39.4 ms at 256^3, and the AABB walk on the halo state by
+// 2%; chunks of 16, 64, 128 and 256 blocks gained less.
+constexpr int kXcds = 8;
+constexpr int kXcdChunk = 32;
+
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int full = nb / (kXcds * kXcdChunk) * (kXcds * kXcdChunk);
+  if (b >= full) return b;
+  const int x = b % kXcds, j = b / kXcds;
+  return ((j / kXcdChunk) * kXcds + x) * kXcdChunk + j % kXcdChunk;
+}
+
 // float in the constant address space: uniform loads through it are scalar
 // loads (the packed records are read-only for the whole traversal).
 typedef const __attribute__((address_space(4))) float ConstF;
@@ -235,9 +255,16 @@ __global__ __launch_bounds__(256) void k_traverse(
   typedef typename InodeOf<MAC>::type Inode;
   const float4 *__restrict__ leaves = reinterpret_cast<const float4 *>(tree);
   const int lane = threadIdx.x & (kWave - 1);
-  // slot = launch position of this wavefront; wave = the group it takes
+  // slot = launch position of this wavefront; wave = the group it takes.
+  // Without a launch order, runs of kXcdChunk consecutive blocks (Morton-
+  // adjacent groups, whose walks share their near-field records) go to one
+  // XCD and its L2 (see xcd_block).  A launch order puts the groups of one
+  // cost bin together, and those lie scattered in space: it keeps the plain
+  // block order, which spreads each bin over the XCDs.
+  const int blk = p.order ? static_cast<int>(blockIdx.x)
+                          : xcd_block(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
   const int slot = __builtin_amdgcn_readfirstlane(
-      static_cast<int>((int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave));
+      static_cast<int>((int64_t(blk) * blockDim.x + threadIdx.x) / kWave));
   int64_t wave = slot;
   if (p.order) {
     if (slot >= p.n_waves) return;
