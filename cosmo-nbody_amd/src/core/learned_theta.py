@@ -513,6 +513,41 @@ def measure_costs(c, state, thetas, log=print, mus=None):
     return np.asarray(out)
 
 
+def hindsight_choice(E, cost, norm, target, iters=30):
+    """The oracle choice behind the hindsight bound: per group the candidate
+    of least cost + lam * E / norm (E: squared errors, groups x candidates;
+    cost: per candidate), with lam the setting of least mean cost whose
+    relative RMS error sqrt(sum E / norm) meets the target.  lam is swept on
+    a log grid, then bisected in log lam between the best passing grid
+    setting and the next smaller one when that one misses the target, as
+    theta* and the selection's lambda are: on the grid alone the mean cost
+    jumps by up to the grid step where the error crosses the target.
+    Returns (lam, mean cost per group) or None."""
+    def at(lam):
+        k = np.argmin(cost[None, :] + lam * E / norm, axis=1)
+        return math.sqrt(E[np.arange(len(k)), k].sum() / norm), float(cost[k].mean())
+
+    grid = np.logspace(-6, 16, 221)
+    best, i_best = None, -1
+    for i, lam in enumerate(grid):
+        e, mc = at(lam)
+        if e <= target and (best is None or mc < best[1]):
+            best, i_best = (float(lam), mc), i
+    if best is None or i_best == 0 or at(grid[i_best - 1])[0] <= target:
+        return best
+    lo, hi = math.log(grid[i_best - 1]), math.log(grid[i_best])
+    for _ in range(iters):
+        mid = 0.5 * (lo + hi)
+        e, mc = at(math.exp(mid))
+        if e <= target:
+            hi = mid
+            if mc < best[1]:
+                best = (math.exp(mid), mc)
+        else:
+            lo = mid
+    return best
+
+
 def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0,
              target_theta=None, lambdas=None, log=print, graph=False, mus=None):
     """Fixed (theta, straddle mass) against the learned per-group selection
@@ -593,13 +628,7 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
         f"(error {e_fixed:.4e}), traversal {t_fixed:.1f} ms")
     E = np.stack(err2_k, 1)
     cost_arr = np.asarray(cost, np.float64)
-    bound = None
-    for lam in np.logspace(-6, 16, 221):
-        k = np.argmin(cost_arr[None, :] + lam * E / probe.norm, axis=1)
-        e = math.sqrt(E[np.arange(len(k)), k].sum() / probe.norm)
-        mc = float(cost_arr[k].mean())
-        if e <= target and (bound is None or mc < bound[1]):
-            bound = (float(lam), mc)
+    bound = hindsight_choice(E, cost_arr, probe.norm, target)
     # Candidates whose average single-wavefront time would exceed half the
     # fixed-theta traversal are dropped: the traversal cannot end before its
     # slowest wavefront.  All groups at candidate k take cost_k * G ms with
@@ -612,14 +641,7 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
         keep = [int(np.argmin(wave_ms))]
     theta_min = float(min(thetas[k] for k in keep))
     # the same bound over the candidates the selection keeps
-    bound_kept = None
-    Ek, ck = E[:, keep], cost_arr[keep]
-    for lam in np.logspace(-6, 16, 221):
-        k = np.argmin(ck[None, :] + lam * Ek / probe.norm, axis=1)
-        e = math.sqrt(Ek[np.arange(len(k)), k].sum() / probe.norm)
-        mc = float(ck[k].mean())
-        if e <= target and (bound_kept is None or mc < bound_kept[1]):
-            bound_kept = (float(lam), mc)
+    bound_kept = hindsight_choice(E[:, keep], cost_arr[keep], probe.norm, target)
     lt = LearnedTheta(model, thetas, cost, 1.0, theta_star, keep=keep,
                       mu_factors=[m / mu0 for m in mus] if mu0 > 0 else None,
                       fallback_mu_factor=(mu_star / mu0) if mu0 > 0 else 1.0)
