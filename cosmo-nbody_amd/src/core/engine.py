@@ -9,7 +9,8 @@ one traversal launch instead of 5000-particle batches (engine.py:91-111), no
 top-of-tree cache (build_top_tree_cache only feeds an on-chip copy of node
 data, tree_kernels.py:264-314), no host synchronisation inside the step,
 and normalize_com fused with packing of the node records.  ``run()``
-steps and captures the density grid; video output and NVML polling of the
+steps and captures the density grid, and with a StepMetrics writes one JSON
+line per step (src/core/metrics.py); video output and NVML polling of the
 reference are not part of this package.  With LEARNED_THETA the engine
 installs the trained per-group theta network named by THETA_NETWORK.
 """
@@ -104,9 +105,11 @@ class SimulationEngine:
         H = get_hubble_param(self.a, self.config)
         return H, self.da / (self.a * H) * 0.8
 
-    def build_tree(self):
+    def build_tree(self, probe=None):
         s = self.state
         s.sort_buffer_by_morton(self.config)
+        if probe is not None:
+            probe.mark("sort")
         self.tree_manager.build_packed(s.pos, s.mass, s.codes, self.config.BOX_SIZE, self.mac,
                                        self.bucket)
         self.select_theta()
@@ -147,21 +150,39 @@ class SimulationEngine:
                 "theta_fallbacks": (self.theta_selector.n_fallback if self.theta_selector
                                     is not None else 0)}
 
-    def step(self):
+    def step(self, probe=None):
+        """One step.  probe: a src.core.metrics.StepMetrics that records HIP
+        events between the stages (and samples the forces), or None."""
         H, dt = self.timestep()
-        self.build_tree()
+        if probe is not None:
+            probe.begin(self)
+        self.build_tree(probe)
+        if probe is not None:
+            probe.mark("tree")
         self.compute_forces()
+        if probe is not None:
+            probe.mark("forces")
+            probe.after_forces(self)
         s = self.state
         integrate(*s.pos, *s.vel, s.force, self.a, H, dt)
+        if probe is not None:
+            probe.mark("update")
         self.a += self.da
         self.steps_done += 1
 
-    def run(self, n_steps=None, capture=True):
+    def run(self, n_steps=None, capture=True, metrics=None):
+        """n_steps steps (the configuration's RUN_STEPS by default), the
+        density captured after each; metrics: a StepMetrics that writes one
+        JSON line per step."""
         n_steps = self.config.RUN_STEPS if n_steps is None else n_steps
         for i in range(n_steps):
-            self.step()
+            self.step(metrics)
             if capture:
                 self.camera.capture(self.steps_done - 1, *self.state.pos)
+                if metrics is not None:
+                    metrics.mark("splat")
+            if metrics is not None:
+                metrics.end(self)
 
     # ------------------------------------------------------------ HIP graphs
     #
