@@ -444,6 +444,37 @@ def test_traverse_interaction_counts(orc, torch_mod, ics32, mac):
     assert np.array_equal(g[:4], s) and int(per[:, 2].sum()) == int(s[2])
 
 
+@pytest.mark.parametrize("mac", [0, 1])
+def test_traverse_xcd_block_mapping(orc, torch_mod, mac):
+    """A launch of 293 blocks of 4 wavefronts: the first 256 blocks go to
+    the XCDs in runs of 32 (the remapped part), the last 37 keep their own
+    index.  Every group is walked exactly once with the oracle's visits and
+    interactions (per group) and forces within FP32 rounding."""
+    torch = torch_mod
+    from src.kernels.physics_kernels import traverse
+    from src.kernels.tree_kernels import pack_nodes
+    n = 293 * 256 - 100
+    rng = np.random.default_rng(293)
+    pos = [rng.uniform(0, 1, n).astype(np.float32) ** 2 for _ in range(3)]
+    pos, mass, ch, nm, com, lo, hi = _case(orc, pos)
+    soft = 1e-3
+    nodes = pack_nodes(dev(ch, torch), dev(nm, torch), dev(com, torch), dev(lo, torch),
+                       dev(hi, torch), 1.0, mac)
+    W = (n + 63) // 64
+    force = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+    gs = torch.zeros(4 + 3 * W, dtype=torch.int64, device="cuda")
+    root = torch.tensor([n], dtype=torch.int32, device="cuda")
+    traverse(*[dev(p, torch) for p in pos], nodes, root, force, 0.5, 1.0, soft, 1.0, mac=mac,
+             stats=gs, group_stats=True)
+    f32, ni, nv = orc.compute_forces(*pos, ch, nm, com, lo, hi, 0.5, 1.0, soft, 1.0, n,
+                                     precision=32, mac=mac, counts=True)
+    per = host(gs)[4:].reshape(W, 3)
+    pad = np.zeros(W * 64 - n, dtype=np.int64)
+    assert np.array_equal(per[:, 0], np.concatenate([ni, pad]).reshape(W, 64).sum(1))
+    assert np.array_equal(per[:, 1], np.concatenate([nv, pad]).reshape(W, 64).sum(1))
+    assert rel_rms(host(force), f32) <= 2e-6
+
+
 def _leaf_ranges(ch, n):
     """(first, last) leaf of every internal node from the children array."""
     first = np.arange(2 * n - 1, dtype=np.int64)
