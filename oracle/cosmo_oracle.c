@@ -21,10 +21,15 @@
  *                     two-body closed form, FP64 vs FP32 agreement,
  *                     convergence to the direct sum (AABB test), stack-
  *                     overflow drop on a valid 64-deep chain (exactly the
- *                     interactions with the dropped pair)           pinned
+ *                     interactions with the dropped pair), hand-derived
+ *                     visits, interactions and forces on 3-particle trees
+ *                     for the reference test's x-extent and its box/2
+ *                     clamp (tests/kat_cases.py)                     pinned
  *   direct_sum        two-body / three-body closed form             pinned
  *   kick_drift        closed form per element                       pinned
- *   render_density    projection closed form, centre pixel, totals  pinned
+ *   render_density    projection closed form, centre pixel, totals,
+ *                     hand-placed points' row (from -y) and column
+ *                     (from x)                                      pinned
  */
 #include "cosmo_oracle.h"
 
