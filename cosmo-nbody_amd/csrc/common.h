@@ -46,10 +46,8 @@ constexpr int64_t kMaxBodies = int64_t(1) << 29;
 // one pass of the traversal's leaf loop).
 constexpr int kMaxBucket = 64;
 
-// Radix sort geometry: 256 threads, 16 keys each, 8-bit digits.
-constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortThreads * kSortItems;
+// Radix sort geometry: 4096-key tiles, 8-bit digits.
+constexpr int kSortTile = 4096;
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 

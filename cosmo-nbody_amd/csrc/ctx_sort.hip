@@ -110,44 +110,56 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *t
 
 // Digit counts of one tile: plain LDS atomics (the order of equal digits
 // does not matter for a count), one histogram per wave to spread conflicts.
-__global__ __launch_bounds__(kSortThreads) void k_radix_count(
+// A launch has one block per 4096-key tile (512 at 128^3, two per CU), so
+// the tile is spread over kCountThreads threads (kCountItems keys each) to
+// keep enough wavefronts in flight; the totals are formed by the first
+// kRadix threads.
+constexpr int kCountThreads = 1024;
+constexpr int kCountItems = kSortTile / kCountThreads;
+static_assert(kCountItems * kCountThreads == kSortTile, "tile split");
+
+__global__ __launch_bounds__(kCountThreads) void k_radix_count(
     const uint64_t *__restrict__ keys, int64_t n, int shift,
     uint32_t *__restrict__ hist, int ntiles) {
-  constexpr int kWaves = kSortThreads / kWave;
+  constexpr int kWaves = kCountThreads / kWave;
   __shared__ uint32_t cnt[kWaves][kRadix];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-#pragma unroll
-  for (int w = 0; w < kWaves; ++w) cnt[w][tid] = 0;
+  for (int i = tid; i < kWaves * kRadix; i += kCountThreads) (&cnt[0][0])[i] = 0;
   __syncthreads();
-  const int64_t base = int64_t(blockIdx.x) * kSortTile + wave * (kSortItems * kWave);
-  uint64_t k[kSortItems];
+  const int64_t base = int64_t(blockIdx.x) * kSortTile + wave * (kCountItems * kWave);
+  uint64_t k[kCountItems];
 #pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
+  for (int r = 0; r < kCountItems; ++r) {
     const int64_t idx = base + r * kWave + lane;
     k[r] = idx < n ? keys[idx] : 0ull;
   }
 #pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
+  for (int r = 0; r < kCountItems; ++r) {
     const int64_t idx = base + r * kWave + lane;
     if (idx < n) atomicAdd(&cnt[wave][uint32_t(k[r] >> shift) & (kRadix - 1)], 1u);
   }
   __syncthreads();
-  uint32_t total = 0;
+  if (tid < kRadix) {
+    uint32_t total = 0;
 #pragma unroll
-  for (int w = 0; w < kWaves; ++w) total += cnt[w][tid];
-  hist[int64_t(tid) * ntiles + blockIdx.x] = total;
+    for (int w = 0; w < kWaves; ++w) total += cnt[w][tid];
+    hist[int64_t(tid) * ntiles + blockIdx.x] = total;
+  }
 }
 
-// One block per digit: exclusive scan of that digit's per-tile counts.
-__global__ __launch_bounds__(kSortThreads) void k_radix_rowscan(
+// One block per digit: exclusive scan of that digit's per-tile counts, in
+// runs of kRowscanThreads tiles.
+constexpr int kRowscanThreads = 1024;
+
+__global__ __launch_bounds__(kRowscanThreads) void k_radix_rowscan(
     uint32_t *__restrict__ hist, int ntiles, uint32_t *__restrict__ digit_tot) {
   uint32_t *row = hist + int64_t(blockIdx.x) * ntiles;
   uint32_t running = 0;
-  for (int start = 0; start < ntiles; start += kSortThreads) {
+  for (int start = 0; start < ntiles; start += kRowscanThreads) {
     const int j = start + threadIdx.x;
     const uint32_t v = (j < ntiles) ? row[j] : 0u;
     uint32_t total;
-    const uint32_t ex = block_exclusive_scan<kSortThreads>(v, &total);
+    const uint32_t ex = block_exclusive_scan<kRowscanThreads>(v, &total);
     if (j < ntiles) row[j] = running + ex;
     running += total;
   }
@@ -389,8 +401,8 @@ static int sort_with(uint64_t *keys_alt, uint32_t *vals_alt, uint32_t *hist, uin
     uint64_t *kout = to_out ? codes_out : keys_alt;
     uint32_t *vout = to_out ? perm : vals_alt;
     const int shift = p * kRadixBits;
-    k_radix_count<<<ntiles, kSortThreads, 0, s>>>(kin, n, shift, hist, ntiles);
-    k_radix_rowscan<<<kRadix, kSortThreads, 0, s>>>(hist, ntiles, digit_tot);
+    k_radix_count<<<ntiles, kCountThreads, 0, s>>>(kin, n, shift, hist, ntiles);
+    k_radix_rowscan<<<kRadix, kRowscanThreads, 0, s>>>(hist, ntiles, digit_tot);
     k_radix_scatter<<<ntiles, kScatterThreads, 0, s>>>(kin, vin, kout, vout, n, shift, hist,
                                                        digit_tot, ntiles);
     COSMO_CHECK_LAUNCH();
