@@ -36,6 +36,7 @@ struct TravParams {
   int big_bits;     // REF: bits of 2e-7 theta^2 (fixed theta; see k_traverse)
   const int32_t *order;  // launch order of the groups (nullable)
   int64_t n_waves;       // number of groups when order is given
+  uint32_t *group_pops;  // nullable: each group's wavefront stack pops
 };
 
 // Unit roundoff of FP32.
@@ -340,7 +341,10 @@ __global__ __launch_bounds__(256) void k_traverse(
 
   while (top > 0) {
     --top;
-    if (STATS) w_pop++;
+    // (the AABB walk always counts its pops, for cosmo_order_groups_by_pops:
+    // one scalar add; the reference-test walk only in its counting variant,
+    // where the add measurably slows the loop)
+    if (STATS || MAC == COSMO_MAC_AABB) w_pop++;
     const int node = __builtin_amdgcn_readlane(stk, top);
     bool in[PPL];
     uint64_t m_in[PPL];  // the entry's lanes as a scalar mask
@@ -696,6 +700,8 @@ __global__ __launch_bounds__(256) void k_traverse(
       ws[2] = w_pop;
     }
   }
+  // the group's cost for a later launch order (cosmo_order_groups_by_pops)
+  if (p.group_pops && lane == 0) p.group_pops[wave] = w_pop;
 }
 
 // ------------------------------------------------------------ split walk
@@ -1198,6 +1204,37 @@ __global__ __launch_bounds__(kOrderThreads) void k_order_groups(const float *__r
     order[atomicAdd(&hist[order_bin(tg[g])], 1)] = static_cast<int32_t>(g);
 }
 
+// Launch order from measured costs: groups by decreasing pop count of an
+// earlier launch (a group of the same Morton range walks nearly the same
+// nodes one step later), so the longest walks start first.  64 bins of a
+// third of an octave each over log2(1 + pops); order within a bin is
+// arbitrary.
+__device__ __forceinline__ int pops_bin(uint32_t pops) {
+  const float b = log2f(1.0f + static_cast<float>(pops)) * 3.0f;
+  return kOrderBins - 1 - (b >= kOrderBins - 1 ? kOrderBins - 1 : static_cast<int>(b));
+}
+
+__global__ __launch_bounds__(kOrderThreads) void k_order_by_pops(const uint32_t *__restrict__ pops,
+                                                                 int64_t groups,
+                                                                 int32_t *__restrict__ order) {
+  __shared__ int hist[kOrderBins];
+  if (threadIdx.x < kOrderBins) hist[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t g = threadIdx.x; g < groups; g += kOrderThreads) atomicAdd(&hist[pops_bin(pops[g])], 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int b = 0; b < kOrderBins; ++b) {
+      const int c = hist[b];
+      hist[b] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  for (int64_t g = threadIdx.x; g < groups; g += kOrderThreads)
+    order[atomicAdd(&hist[pops_bin(pops[g])], 1)] = static_cast<int32_t>(g);
+}
+
 __host__ __device__ constexpr bool pairs_of(int flags) { return (flags & COSMO_TRAV_PAIRS) != 0; }
 
 }  // namespace cosmo
@@ -1211,8 +1248,11 @@ static int traverse_impl(const float *px, const float *py, const float *pz, cons
                          const float *theta_group, double G, double soft, double box,
                          int64_t off, int64_t count, int64_t n_bodies, int mac, int flags,
                          uint64_t *stats, const int32_t *order, double straddle_mass,
-                         const float *straddle_group, void *stream) {
+                         const float *straddle_group, uint32_t *group_pops, void *stream) {
   if (!(straddle_mass >= 0.0) || (straddle_mass > 0.0 && mac != COSMO_MAC_AABB))
+    return COSMO_E_INVALID;
+  // pop counts are kept per 64-particle group, by the AABB walk
+  if (group_pops && ((flags & COSMO_TRAV_PAIRS) || mac != COSMO_MAC_AABB))
     return COSMO_E_INVALID;
   // per-group straddle masses: AABB test, one 64-particle group per wavefront
   if (straddle_group && (mac != COSMO_MAC_AABB || (flags & COSMO_TRAV_PAIRS)))
@@ -1247,6 +1287,7 @@ static int traverse_impl(const float *px, const float *py, const float *pz, cons
   p.straddle_group = straddle_group;
   p.wave_stats = (flags & COSMO_TRAV_GROUP_STATS) != 0;
   p.order = order;
+  p.group_pops = group_pops;
   p.n_waves = (hi - lo + kWave * (pairs_of(flags) ? 2 : 1) - 1) / (kWave * (pairs_of(flags) ? 2 : 1));
   hipStream_t s = as_stream(stream);
   const bool comp = (flags & COSMO_TRAV_COMPENSATED) != 0;
@@ -1274,7 +1315,7 @@ int cosmo_traverse(const float *px, const float *py, const float *pz, const void
                    void *stream) {
   return traverse_impl(px, py, pz, nodes, n_tree, root_idx, force, theta, theta_group, G, soft,
                        box, off, count, n_bodies, mac, flags, stats, nullptr, 0.0, nullptr,
-                       stream);
+                       nullptr, stream);
 }
 
 int cosmo_traverse_ordered(const float *px, const float *py, const float *pz, const void *nodes,
@@ -1285,7 +1326,7 @@ int cosmo_traverse_ordered(const float *px, const float *py, const float *pz, co
   if (!group_order || (flags & COSMO_TRAV_PAIRS)) return COSMO_E_INVALID;
   return traverse_impl(px, py, pz, nodes, n_tree, root_idx, force, theta, theta_group, G, soft,
                        box, off, count, n_bodies, mac, flags, stats, group_order, 0.0, nullptr,
-                       stream);
+                       nullptr, stream);
 }
 
 int cosmo_traverse_ex(const float *px, const float *py, const float *pz, const void *nodes,
@@ -1293,11 +1334,21 @@ int cosmo_traverse_ex(const float *px, const float *py, const float *pz, const v
                       const float *theta_group, double G, double soft, double box, int64_t off,
                       int64_t count, int64_t n_bodies, int mac, int flags, uint64_t *stats,
                       const int32_t *group_order, double straddle_mass,
-                      const float *straddle_group, void *stream) {
+                      const float *straddle_group, uint32_t *group_pops, void *stream) {
   if (group_order && (flags & COSMO_TRAV_PAIRS)) return COSMO_E_INVALID;
   return traverse_impl(px, py, pz, nodes, n_tree, root_idx, force, theta, theta_group, G, soft,
                        box, off, count, n_bodies, mac, flags, stats, group_order, straddle_mass,
-                       straddle_group, stream);
+                       straddle_group, group_pops, stream);
+}
+
+int cosmo_order_groups_by_pops(const uint32_t *group_pops, int64_t groups, int32_t *order,
+                               void *stream) {
+  if (groups < 0 || groups > kMaxBodies) return COSMO_E_INVALID;
+  if (groups == 0) return COSMO_OK;
+  if (!group_pops || !order) return COSMO_E_INVALID;
+  k_order_by_pops<<<1, kOrderThreads, 0, as_stream(stream)>>>(group_pops, groups, order);
+  COSMO_CHECK_LAUNCH();
+  return COSMO_OK;
 }
 
 int cosmo_order_groups(const float *theta_group, int64_t groups, int32_t *order, void *stream) {

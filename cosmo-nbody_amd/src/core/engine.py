@@ -24,7 +24,8 @@ import torch
 from src.core.camera import Camera
 from src.core.octree import OctreeManager
 from src.kernels import _hip
-from src.kernels.physics_kernels import MAC_AABB, MAC_REF, count_nonfinite, integrate, traverse
+from src.kernels.physics_kernels import (MAC_AABB, MAC_REF, count_nonfinite, integrate,
+                                        order_groups_by_pops, traverse)
 from src.utils.cosmology import get_hubble_param
 from src.utils.visualizer import get_mvp_matrix
 
@@ -90,6 +91,14 @@ class SimulationEngine:
         self._a_dev = None          # device scale factor while a step is being captured
         self.stats = None           # int64[4] interaction / visit / pop / drop counters, or None
         self.pairs = bool(getattr(config, "TRAVERSAL_PAIRS", False))
+        # LONGEST_FIRST: each traversal stores its groups' stack pops, and the
+        # next one launches the groups by decreasing pops (zeros at first)
+        self.longest_first = (bool(getattr(config, "LONGEST_FIRST", False)) and not self.pairs
+                              and self.mac == MAC_AABB)
+        if self.longest_first:
+            groups = (state.n + 63) // 64
+            self.group_pops = torch.zeros(groups, dtype=torch.int32, device=device)
+            self._pops_order = torch.zeros(groups, dtype=torch.int32, device=device)
         self.steps_done = 0
         # failure detection (health=True): the traversal's counters (its
         # overflow drops among them) and the non-finite accelerations,
@@ -121,7 +130,17 @@ class SimulationEngine:
         if self.theta_selector is not None:
             self.theta_group = self.theta_selector.theta_groups(self, a=self._a_dev)
             self.straddle_group = getattr(self.theta_selector, "mu_group", None)
-            self.group_order = self.theta_selector.launch_order(self.theta_group)
+            if not self.longest_first:
+                self.group_order = self.theta_selector.launch_order(self.theta_group)
+
+    def launch_order(self):
+        """The traversal's launch order of the groups: by the previous
+        launch's pops (LONGEST_FIRST), else by the learned thetas, else none."""
+        if self.longest_first:
+            return order_groups_by_pops(self.group_pops, self._pops_order)
+        if self.theta_group is not None and not self.pairs:
+            return self.group_order
+        return None
 
     def compute_forces(self):
         s, c = self.state, self.config
@@ -131,8 +150,8 @@ class SimulationEngine:
                  compensated=bool(getattr(c, "COMPENSATED", False)), stats=stats,
                  pairs=self.pairs, straddle_mass=self.straddle_mass,
                  straddle_group=self.straddle_group if self.theta_group is not None else None,
-                 group_order=(self.group_order if self.theta_group is not None and not self.pairs
-                              else None))
+                 group_order=self.launch_order(),
+                 group_pops=self.group_pops if self.longest_first else None)
         if self.health:
             count_nonfinite(s.force, self._nonfinite)
             self._health_steps += 1

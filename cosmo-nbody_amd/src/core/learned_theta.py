@@ -382,19 +382,28 @@ class LearnedTheta:
 
 def _time_traversal(engine, theta, theta_group, reps=5, group_order=None, straddle=None,
                     straddle_group=None):
-    """Average HIP-event time (ms) of the full traversal (all particles)."""
+    """Average HIP-event time (ms) of the full traversal (all particles).
+    With the engine's LONGEST_FIRST schedule every launch (the untimed first
+    one included) stores its groups' pops and the next one is launched in
+    the order they give, the order kernel inside the timed region, whatever
+    group_order says; fixed and per-group theta are timed alike."""
     st, tm, c = engine.state, engine.tree_manager, engine.config
     comp = bool(getattr(c, "COMPENSATED", False))
     mu = engine.straddle_mass if straddle is None else straddle
-    traverse(*st.pos, tm.nodes, tm.root_idx, st.force, theta, c.G, c.SOFTENING, c.BOX_SIZE,
-             mac=engine.mac, theta_group=theta_group, group_order=group_order,
-             compensated=comp, straddle_mass=mu, straddle_group=straddle_group)
+    sched = engine.longest_first
+
+    def launch():
+        order = engine.launch_order() if sched else group_order
+        traverse(*st.pos, tm.nodes, tm.root_idx, st.force, theta, c.G, c.SOFTENING, c.BOX_SIZE,
+                 mac=engine.mac, theta_group=theta_group, group_order=order,
+                 compensated=comp, straddle_mass=mu, straddle_group=straddle_group,
+                 group_pops=engine.group_pops if sched else None)
+
+    launch()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(reps):
-        traverse(*st.pos, tm.nodes, tm.root_idx, st.force, theta, c.G, c.SOFTENING, c.BOX_SIZE,
-                 mac=engine.mac, theta_group=theta_group, group_order=group_order,
-                 compensated=comp, straddle_mass=mu, straddle_group=straddle_group)
+        launch()
     e1.record()
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps

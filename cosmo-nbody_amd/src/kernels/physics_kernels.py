@@ -17,9 +17,10 @@ from __future__ import annotations
 import torch
 
 from ._hip import (MAC_AABB, MAC_REF, TRAV_COMPENSATED, TRAV_GROUP_STATS, TRAV_PAIRS, TRAV_STATS,
-                   check, kernel, load, ptr, shared_context, stream_handle, tree_generation)
+                   U32, check, kernel, load, ptr, shared_context, stream_handle, tree_generation)
 
-__all__ = ["compute_forces", "integrate", "kick_drift", "traverse", "order_groups", "direct_sum",
+__all__ = ["compute_forces", "integrate", "kick_drift", "traverse", "order_groups",
+           "order_groups_by_pops", "direct_sum",
            "count_nonfinite", "MAC_REF", "MAC_AABB"]
 
 _F32 = torch.float32
@@ -80,7 +81,7 @@ def compute_forces(pos_x, pos_y, pos_z, children, n_mass, n_com, n_min, n_max, f
 def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0, count=None,
              mac=MAC_REF, theta_group=None, compensated=False, stats=None, n_nodes=None,
              pairs=False, group_stats=False, group_order=None, straddle_mass=0.0,
-             straddle_group=None):
+             straddle_group=None, group_pops=None):
     """Traversal over packed node records; stats: int64[4] tensor accumulating
     (interactions, node visits, wavefront stack pops, overflow drops) when
     given; with group_stats it has 4 + 3*W entries and entries 4+3w.. hold
@@ -90,7 +91,9 @@ def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0
     64-particle groups (a permutation, e.g. from order_groups).
     straddle_mass (AABB test): nodes lighter than straddle_mass pass the
     minimum-image guard (0 = exact guard); straddle_group (float32 per
-    64-particle group) gives each group its own."""
+    64-particle group) gives each group its own.  group_pops (int32/uint32
+    per 64-particle group) receives each group's stack pops, from which
+    order_groups_by_pops makes a longest-first order for a later launch."""
     n = pos_x.shape[0]
     n_nodes = 2 * n - 1 if n_nodes is None else int(n_nodes)
     n_tree = (n_nodes + 1) // 2
@@ -113,7 +116,9 @@ def traverse(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box, off=0
                                                  numel=max(groups, 0))
     sg = None if straddle_group is None else ptr(straddle_group, _F32, "straddle_group",
                                                  numel=max(groups, 0))
-    check(load().cosmo_traverse_ex(*args, order, float(straddle_mass), sg, stream_handle()),
+    gp = None if group_pops is None else ptr(group_pops, U32, "group_pops",
+                                             numel=max(groups, 0))
+    check(load().cosmo_traverse_ex(*args, order, float(straddle_mass), sg, gp, stream_handle()),
           "traverse")
 
 
@@ -133,6 +138,16 @@ def traverse_split(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box,
         ptr(root, torch.int32, "root"), ptr(force, _F32, "force", numel=3 * n), float(theta),
         float(G), float(soft), float(box), int(off), int(count), n, st, stream_handle()),
         "traverse_split")
+
+
+def order_groups_by_pops(group_pops, out=None):
+    """Launch order of the groups by decreasing pop count (longest first)."""
+    g = group_pops.shape[0]
+    out = torch.empty(g, dtype=torch.int32, device=group_pops.device) if out is None else out
+    check(load().cosmo_order_groups_by_pops(ptr(group_pops, U32, "group_pops", numel=g), g,
+                                            ptr(out, torch.int32, "order", numel=g),
+                                            stream_handle()), "order_groups_by_pops")
+    return out
 
 
 def order_groups(theta_group, out=None):

@@ -76,6 +76,8 @@ BUCKET = 0                 # AABB test: nodes of <= BUCKET leaves are summed lea
 STRADDLE_MASS = 0.0        # AABB test: nodes lighter than STRADDLE_MASS pass the
                            # minimum-image guard (0 = exact guard)
 TRAVERSAL_PAIRS = False    # two particles per lane in the traversal (same results, slower at 128^3)
+LONGEST_FIRST = False      # AABB test: launch the traversal's groups by decreasing stack
+                           # pops of the previous launch (pays on clustered states; same results)
 LEARNED_THETA = False
 THETA_NETWORK = ""         # trained per-group theta network (LearnedTheta.save file, a path
                            # or a name in cosmo-nbody_amd/networks/) for LEARNED_THETA
@@ -122,6 +124,7 @@ class Config:
     BUCKET: int = BUCKET
     STRADDLE_MASS: float = STRADDLE_MASS
     TRAVERSAL_PAIRS: bool = TRAVERSAL_PAIRS
+    LONGEST_FIRST: bool = LONGEST_FIRST
     LEARNED_THETA: bool = LEARNED_THETA
     THETA_NETWORK: str = THETA_NETWORK
     NAME: str = "reference_defaults"
@@ -156,17 +159,20 @@ PRESETS = {
     "repo_256": lambda: _plain(256, RUN_STEPS=50, RES=2048, NAME="repo_256"),
     "learned_128": lambda: _plain(128, RUN_STEPS=20, LEARNED_THETA=True, MAC="aabb", BUCKET=32,
                                   IC_KIND="halos", COMPENSATED=True, STRADDLE_MASS=5e-5,
-                                  THETA_NETWORK="halos_128.npz", NAME="learned_128"),
+                                  THETA_NETWORK="halos_128.npz", LONGEST_FIRST=True,
+                                  NAME="learned_128"),
     "large_256": lambda: _plain(256, RUN_STEPS=50, RES=2048, LEARNED_THETA=True, MAC="aabb",
                                 IC_KIND="halos", BUCKET=32, COMPENSATED=True,
                                 STRADDLE_MASS=6.25e-6, THETA_NETWORK="halos_128.npz",
+                                LONGEST_FIRST=True,
                                 NAME="large_256"),
     "halos_128": lambda: _plain(128, RUN_STEPS=20, MAC="aabb", IC_KIND="halos", BUCKET=32,
-                                COMPENSATED=True, STRADDLE_MASS=5e-5, NAME="halos_128"),
+                                COMPENSATED=True, STRADDLE_MASS=5e-5, LONGEST_FIRST=True,
+                                NAME="halos_128"),
     "halos_256": lambda: _plain(256, RUN_STEPS=50, RES=2048, MAC="aabb", IC_KIND="halos",
                                 BUCKET=32, COMPENSATED=True, STRADDLE_MASS=6.25e-6,
                                 LEARNED_THETA=True, THETA_NETWORK="halos_128.npz",
-                                NAME="halos_256"),
+                                LONGEST_FIRST=True, NAME="halos_256"),
 }
 
 

@@ -311,7 +311,10 @@ int cosmo_traverse_ordered(const float *pos_x, const float *pos_y,
  * puts at most its mass on the wrong image, so the limit bounds the error
  * it adds.  straddle_group (nullable, device float[ceil(count/64)], AABB
  * test, not with COSMO_TRAV_PAIRS) gives each group its own straddle mass
- * in place of straddle_mass. */
+ * in place of straddle_mass.  group_pops (nullable, device
+ * uint32[ceil(count/64)], AABB test, not with COSMO_TRAV_PAIRS) receives
+ * each group's number of stack pops, the cost measure of
+ * cosmo_order_groups_by_pops. */
 int cosmo_traverse_ex(const float *pos_x, const float *pos_y, const float *pos_z,
                       const void *nodes, int64_t n_tree, const int32_t *root_idx,
                       float *force, double theta, const float *theta_group,
@@ -319,13 +322,20 @@ int cosmo_traverse_ex(const float *pos_x, const float *pos_y, const float *pos_z
                       int64_t count, int64_t n_bodies, int mac, int flags,
                       uint64_t *stats, const int32_t *group_order,
                       double straddle_mass, const float *straddle_group,
-                      void *stream);
+                      uint32_t *group_pops, void *stream);
 
 /* order[0..groups) = the groups sorted by increasing theta_group (64
  * logarithmic bins; arbitrary order inside a bin): the launch order for
  * cosmo_traverse_ordered with per-group theta. */
 int cosmo_order_groups(const float *theta_group, int64_t groups, int32_t *order,
                        void *stream);
+
+/* order[0..groups) = the groups sorted by decreasing group_pops (the pop
+ * counts an earlier cosmo_traverse_ex stored; 64 bins of a third of an
+ * octave; arbitrary order inside a bin): a longest-first launch order for
+ * the next launch over the same groups (no reference counterpart). */
+int cosmo_order_groups_by_pops(const uint32_t *group_pops, int64_t groups, int32_t *order,
+                               void *stream);
 
 /* src/kernels/physics_kernels.py:14-153 compute_forces(pos_x, pos_y, pos_z,
  * children, n_mass, n_com, n_min, n_max, force, theta, G, soft, box, root,

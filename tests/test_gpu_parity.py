@@ -653,6 +653,64 @@ def test_traverse_launch_order(orc, torch_mod, ics32):
     assert np.array_equal(res[0][1], res[1][1])
 
 
+def test_traverse_group_pops_and_longest_first(orc, torch_mod, ics32):
+    """The AABB walk stores each group's stack pops (equal to the counting
+    walk's per-group pops); order_groups_by_pops gives a permutation with
+    non-increasing pop bins, and launching in it changes no result bit.
+    The reference-test walk refuses group_pops.  An engine with
+    LONGEST_FIRST ends three steps bit for bit where one without does."""
+    torch = torch_mod
+    from src.kernels.physics_kernels import order_groups_by_pops, traverse
+    from src.kernels.tree_kernels import pack_nodes
+    pos, mass, ch, nm, com, lo, hi = _case(orc, ics32)
+    n = len(mass)
+    W = (n + 63) // 64
+    nodes = pack_nodes(dev(ch, torch), dev(nm, torch), dev(com, torch), dev(lo, torch),
+                       dev(hi, torch), 1.0, 1)
+    root = torch.tensor([n], dtype=torch.int32, device="cuda")
+    P = [dev(p, torch) for p in pos]
+    pops = torch.zeros(W, dtype=torch.int32, device="cuda")
+    gs = torch.zeros(4 + 3 * W, dtype=torch.int64, device="cuda")
+    f0 = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+    traverse(*P, nodes, root, f0, 0.4, 1.0, 0.02 / 32, 1.0, mac=1, stats=gs, group_stats=True,
+             group_pops=pops, straddle_mass=1e-4)
+    per = host(gs)[4:].reshape(W, 3)
+    assert np.array_equal(host(pops).astype(np.int64), per[:, 2])
+    order = order_groups_by_pops(pops)
+    o = host(order)
+    assert np.array_equal(np.sort(o), np.arange(W))
+    # heaviest first, to within one bin (a third of an octave): no later
+    # group exceeds an earlier one by more than that
+    po = 1.0 + host(pops)[o].astype(np.float64)
+    later_max = np.maximum.accumulate(po[::-1])[::-1]
+    assert np.all(later_max <= po * 2.0 ** (1.0 / 3.0) * (1.0 + 1e-5))
+    f1 = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
+    pops2 = torch.zeros(W, dtype=torch.int32, device="cuda")
+    traverse(*P, nodes, root, f1, 0.4, 1.0, 0.02 / 32, 1.0, mac=1, group_order=order,
+             group_pops=pops2, straddle_mass=1e-4)
+    assert np.array_equal(host(f0), host(f1))
+    assert np.array_equal(host(pops), host(pops2))
+    ref_nodes = pack_nodes(dev(ch, torch), dev(nm, torch), dev(com, torch), dev(lo, torch),
+                           dev(hi, torch), 1.0, 0)
+    with pytest.raises(RuntimeError):
+        traverse(*P, ref_nodes, root, f1, 0.4, 1.0, 0.02 / 32, 1.0, mac=0, group_pops=pops)
+    # the engine
+    from src.core.engine import SimulationEngine
+    from src.core.state import SimulationState
+    from src.utils import config as cfg
+    runs = []
+    for lf in (False, True):
+        c = cfg.preset("small_32", MAC="aabb", BUCKET=16, STRADDLE_MASS=1e-4, THETA=0.4,
+                       LONGEST_FIRST=lf)
+        p0, v0 = zeldovich(32, c.SEED)
+        eng = SimulationEngine(c, SimulationState(len(p0[0]), p0, v0))
+        assert eng.longest_first == lf
+        eng.run(3)
+        runs.append([host(t) for t in eng.state.pos + eng.state.vel])
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+
+
 def test_traverse_group_theta_and_compensated(orc, torch_mod, ics32):
     torch = torch_mod
     from src.kernels.physics_kernels import traverse

@@ -10,21 +10,24 @@ pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not gpu_available(), reason="needs a HIP device")]
 
 
-def _engine(n_grid=32, mac="ref"):
+def _engine(n_grid=32, mac="ref", **kw):
     from src.core.engine import SimulationEngine
     from src.core.state import SimulationState
     from src.utils import config as cfg
-    c = cfg.preset("small_32", N_GRID=n_grid, SOFTENING=0.02 / n_grid, MAC=mac, RES=256)
+    c = cfg.preset("small_32", N_GRID=n_grid, SOFTENING=0.02 / n_grid, MAC=mac, RES=256, **kw)
     pos, vel = zeldovich(n_grid)
     st = SimulationState(len(pos[0]), pos, vel)
     return SimulationEngine(c, st)
 
 
-@pytest.mark.parametrize("mac", ["ref", "aabb"])
-def test_graph_replays_match_eager(mac):
+@pytest.mark.parametrize("mac,longest_first", [("ref", False), ("aabb", False), ("aabb", True)])
+def test_graph_replays_match_eager(mac, longest_first):
+    """With LONGEST_FIRST the captured step also orders the groups by the
+    previous launch's pops and stores the new ones."""
     import torch
-    eager = _engine(mac=mac)
-    graph = _engine(mac=mac)
+    eager = _engine(mac=mac, LONGEST_FIRST=longest_first)
+    graph = _engine(mac=mac, LONGEST_FIRST=longest_first)
+    assert graph.longest_first == longest_first
     steps = 5
     graph.capture_graphs(steps)
     for k in range(steps):
