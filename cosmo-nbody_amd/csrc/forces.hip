@@ -1206,21 +1206,39 @@ __global__ __launch_bounds__(kOrderThreads) void k_order_groups(const float *__r
 
 // Launch order from measured costs: groups by decreasing pop count of an
 // earlier launch (a group of the same Morton range walks nearly the same
-// nodes one step later), so the longest walks start first.  64 bins of a
-// third of an octave each over log2(1 + pops); order within a bin is
-// arbitrary.
-__device__ __forceinline__ int pops_bin(uint32_t pops) {
-  const float b = log2f(1.0f + static_cast<float>(pops)) * 3.0f;
+// nodes one step later), so the longest walks start first.  When the
+// groups' candidates of a per-group selection are given for both launches,
+// a group's pops are scaled by cost[k_now] / cost[k_prev]: the few groups
+// that switch candidates between steps switch between very different costs.
+// 64 bins of a third of an octave each over log2(1 + estimate); order
+// within a bin is arbitrary.
+__device__ __forceinline__ int pops_bin(float est) {
+  const float b = log2f(1.0f + est) * 3.0f;
   return kOrderBins - 1 - (b >= kOrderBins - 1 ? kOrderBins - 1 : static_cast<int>(b));
 }
 
-__global__ __launch_bounds__(kOrderThreads) void k_order_by_pops(const uint32_t *__restrict__ pops,
-                                                                 int64_t groups,
-                                                                 int32_t *__restrict__ order) {
+__device__ __forceinline__ float pops_estimate(const uint32_t *__restrict__ pops,
+                                               const int32_t *__restrict__ k_now,
+                                               const int32_t *__restrict__ k_prev,
+                                               const float *__restrict__ cost, int K,
+                                               int64_t g) {
+  float est = static_cast<float>(pops[g]);
+  if (k_now) {
+    const int a = k_now[g], b = k_prev[g];
+    if (a >= 0 && a < K && b >= 0 && b < K && cost[b] > 0.0f) est *= cost[a] / cost[b];
+  }
+  return est;
+}
+
+__global__ __launch_bounds__(kOrderThreads) void k_order_by_pops(
+    const uint32_t *__restrict__ pops, const int32_t *__restrict__ k_now,
+    const int32_t *__restrict__ k_prev, const float *__restrict__ cost, int K, int64_t groups,
+    int32_t *__restrict__ order) {
   __shared__ int hist[kOrderBins];
   if (threadIdx.x < kOrderBins) hist[threadIdx.x] = 0;
   __syncthreads();
-  for (int64_t g = threadIdx.x; g < groups; g += kOrderThreads) atomicAdd(&hist[pops_bin(pops[g])], 1);
+  for (int64_t g = threadIdx.x; g < groups; g += kOrderThreads)
+    atomicAdd(&hist[pops_bin(pops_estimate(pops, k_now, k_prev, cost, K, g))], 1);
   __syncthreads();
   if (threadIdx.x == 0) {
     int run = 0;
@@ -1232,7 +1250,8 @@ __global__ __launch_bounds__(kOrderThreads) void k_order_by_pops(const uint32_t 
   }
   __syncthreads();
   for (int64_t g = threadIdx.x; g < groups; g += kOrderThreads)
-    order[atomicAdd(&hist[pops_bin(pops[g])], 1)] = static_cast<int32_t>(g);
+    order[atomicAdd(&hist[pops_bin(pops_estimate(pops, k_now, k_prev, cost, K, g))], 1)] =
+        static_cast<int32_t>(g);
 }
 
 __host__ __device__ constexpr bool pairs_of(int flags) { return (flags & COSMO_TRAV_PAIRS) != 0; }
@@ -1341,12 +1360,16 @@ int cosmo_traverse_ex(const float *px, const float *py, const float *pz, const v
                        straddle_group, group_pops, stream);
 }
 
-int cosmo_order_groups_by_pops(const uint32_t *group_pops, int64_t groups, int32_t *order,
-                               void *stream) {
+int cosmo_order_groups_by_pops(const uint32_t *group_pops, int64_t groups,
+                               const int32_t *k_now, const int32_t *k_prev, const float *cost,
+                               int K, int32_t *order, void *stream) {
   if (groups < 0 || groups > kMaxBodies) return COSMO_E_INVALID;
+  if ((k_now != nullptr) != (k_prev != nullptr) || (k_now && (!cost || K < 1)))
+    return COSMO_E_INVALID;
   if (groups == 0) return COSMO_OK;
   if (!group_pops || !order) return COSMO_E_INVALID;
-  k_order_by_pops<<<1, kOrderThreads, 0, as_stream(stream)>>>(group_pops, groups, order);
+  k_order_by_pops<<<1, kOrderThreads, 0, as_stream(stream)>>>(group_pops, k_now, k_prev, cost,
+                                                               k_now ? K : 0, groups, order);
   COSMO_CHECK_LAUNCH();
   return COSMO_OK;
 }

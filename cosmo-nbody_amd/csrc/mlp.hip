@@ -737,8 +737,9 @@ __global__ __launch_bounds__(256) void k_assemble_rows(const float *__restrict__
 }
 
 // k* = argmin_k cost[k] + lambda * 2^pred[g*K+k]: theta_group[g] =
-// thetas[k*] and, when given, mu_group[g] = mus[k*]; any non-finite
-// prediction of the group selects the fixed fallbacks.
+// thetas[k*] and, when given, mu_group[g] = mus[k*] and k_group[g] = k*;
+// any non-finite prediction of the group selects the fixed fallbacks
+// (k_group -1).
 __global__ __launch_bounds__(256) void k_select_theta(const float *__restrict__ pred,
                                                       int64_t groups,
                                                       const float *__restrict__ thetas,
@@ -748,6 +749,7 @@ __global__ __launch_bounds__(256) void k_select_theta(const float *__restrict__ 
                                                       float fallback_mu,
                                                       float *__restrict__ theta_group,
                                                       float *__restrict__ mu_group,
+                                                      int32_t *__restrict__ k_group,
                                                       unsigned long long *__restrict__ n_fallback) {
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (g >= groups) return;
@@ -764,14 +766,17 @@ __global__ __launch_bounds__(256) void k_select_theta(const float *__restrict__ 
     }
   }
   float th = fallback, mu = fallback_mu;
+  int kk = -1;
   if (!ok || !isfinite(best) || kb < 0) {
     if (n_fallback) atomicAdd(n_fallback, 1ull);
   } else {
     th = thetas[kb];
     if (mus) mu = mus[kb];
+    kk = kb;
   }
   theta_group[g] = th;
   if (mu_group) mu_group[g] = mu;
+  if (k_group) k_group[g] = kk;
 }
 
 }  // namespace cosmo
@@ -947,7 +952,7 @@ int cosmo_assemble_rows(const float *feat, int64_t groups, const float *thetas,
 int cosmo_select_theta(const float *pred, int64_t groups, const float *thetas,
                        const float *mus, const float *cost, int K, double lambda,
                        double fallback, double fallback_mu, float *theta_group,
-                       float *mu_group, uint64_t *n_fallback, void *stream) {
+                       float *mu_group, int32_t *k_group, uint64_t *n_fallback, void *stream) {
   if (groups < 0 || K < 1 || (groups && (!pred || !thetas || !cost || !theta_group)) ||
       (mu_group && !mus))
     return COSMO_E_INVALID;
@@ -955,7 +960,7 @@ int cosmo_select_theta(const float *pred, int64_t groups, const float *thetas,
   k_select_theta<<<blocks_for(groups, 256), 256, 0, as_stream(stream)>>>(
       pred, groups, thetas, mus, cost, K, static_cast<float>(lambda),
       static_cast<float>(fallback), static_cast<float>(fallback_mu), theta_group, mu_group,
-      reinterpret_cast<unsigned long long *>(n_fallback));
+      k_group, reinterpret_cast<unsigned long long *>(n_fallback));
   COSMO_CHECK_LAUNCH();
   return COSMO_OK;
 }

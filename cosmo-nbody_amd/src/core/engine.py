@@ -133,10 +133,16 @@ class SimulationEngine:
             if not self.longest_first:
                 self.group_order = self.theta_selector.launch_order(self.theta_group)
 
-    def launch_order(self):
+    def launch_order(self, scale=True):
         """The traversal's launch order of the groups: by the previous
-        launch's pops (LONGEST_FIRST), else by the learned thetas, else none."""
+        launch's pops (LONGEST_FIRST; with a learned selection and scale, each
+        group's pops scaled by the change of its candidate's cost since that
+        launch), else by the learned thetas, else none."""
         if self.longest_first:
+            sel = self.theta_selector
+            if scale and sel is not None and self.theta_group is not None:
+                return order_groups_by_pops(self.group_pops, self._pops_order, sel.k_group,
+                                            sel.k_prev, sel.cost_dev)
             return order_groups_by_pops(self.group_pops, self._pops_order)
         if self.theta_group is not None and not self.pairs:
             return self.group_order

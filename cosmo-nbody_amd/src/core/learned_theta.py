@@ -326,7 +326,16 @@ class LearnedTheta:
                                 device=device)
             self._dev = (G, th, cost, X, Y, tg, nf, feat, order, prof, mus, codes, mg, mus_l,
                          fidx)
+            # the chosen candidates of this and the previous selection (-1:
+            # none), swapped at every selection
+            self.k_group = torch.full((G,), -1, dtype=torch.int32, device=device)
+            self.k_prev = torch.full((G,), -1, dtype=torch.int32, device=device)
         return self._dev
+
+    @property
+    def cost_dev(self):
+        """The candidates' cost per group (device float32[K])."""
+        return self._dev[2]
 
     def theta_groups(self, engine, lam=None, a=None):
         """Per-group theta for the engine's current tree (device float[G]);
@@ -340,15 +349,16 @@ class LearnedTheta:
         feature_blocks(engine, mus_l, feat, prof, a)
         K.assemble_rows(feat, th, X, cand2=codes, fidx=fidx)
         self.model.predict(X, Y)
+        self.k_group, self.k_prev = self.k_prev, self.k_group
         if self.joint:
             K.select_theta(Y, G, th, cost, self.lam if lam is None else lam, self.fallback, tg,
                            nf, mus=mus,
                            fallback_mu=float(engine.straddle_mass) * self.fallback_mu_factor,
-                           mu_out=mg)
+                           mu_out=mg, k_out=self.k_group)
             self.mu_group = mg
         else:
             K.select_theta(Y, G, th, cost, self.lam if lam is None else lam, self.fallback, tg,
-                           nf)
+                           nf, k_out=self.k_group)
             self.mu_group = None
         return tg
 
@@ -393,7 +403,8 @@ def _time_traversal(engine, theta, theta_group, reps=5, group_order=None, stradd
     sched = engine.longest_first
 
     def launch():
-        order = engine.launch_order() if sched else group_order
+        # (every launch here walks the same selection: plain pops)
+        order = engine.launch_order(scale=False) if sched else group_order
         traverse(*st.pos, tm.nodes, tm.root_idx, st.force, theta, c.G, c.SOFTENING, c.BOX_SIZE,
                  mac=engine.mac, theta_group=theta_group, group_order=order,
                  compensated=comp, straddle_mass=mu, straddle_group=straddle_group,

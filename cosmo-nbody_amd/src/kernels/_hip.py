@@ -79,7 +79,7 @@ SIGNATURES = {
     "cosmo_traverse_ex": ([_p] * 4 + [_i64, _p, _p, _f64, _p, _f64, _f64, _f64, _i64, _i64,
                                        _i64, _int, _int, _p, _p, _f64, _p, _p, _p], _int),
     "cosmo_order_groups": ([_p, _i64, _p, _p], _int),
-    "cosmo_order_groups_by_pops": ([_p, _i64, _p, _p], _int),
+    "cosmo_order_groups_by_pops": ([_p, _i64, _p, _p, _p, _int, _p, _p], _int),
     "cosmo_compute_forces": ([_p] * 10 + [_f64] * 4 + [_p, _p, _p, _i64, _i64, _i64, _p], _int),
     "cosmo_integrate": ([_p] * 7 + [_f64, _f64, _f64, _i64, _p], _int),
     "cosmo_kick_drift": ([_p] * 7 + [_f32, _f32, _f32, _i64, _p], _int),
@@ -101,7 +101,7 @@ SIGNATURES = {
     "cosmo_mlp_grad": ([_p, _p, _i64, _p, _p, _p, _int, _p, _p], _int),
     "cosmo_mlp_adam": ([_p, _p, _p, _p, _f32, _f32, _f32, _f32, _int, _p, _p], _int),
     "cosmo_mlp_to_bf16": ([_p, _p, _p], _int),
-    "cosmo_select_theta": ([_p, _i64, _p, _p, _p, _int, _f64, _f64, _f64, _p, _p, _p, _p],
+    "cosmo_select_theta": ([_p, _i64, _p, _p, _p, _int, _f64, _f64, _f64, _p, _p, _p, _p, _p],
                            _int),
 }
 

@@ -179,9 +179,10 @@ def to_bf16(params, wbf):
 
 
 def select_theta(pred, groups, thetas, cost, lam, fallback, out, n_fallback=None, mus=None,
-                 fallback_mu=0.0, mu_out=None):
+                 fallback_mu=0.0, mu_out=None, k_out=None):
     """Per-group theta (and, with mus and mu_out, straddle mass) of the
-    candidate minimising cost + lam * 2^pred."""
+    candidate minimising cost + lam * 2^pred; k_out (int32 per group) gets
+    the chosen candidate's index, -1 for a fallback."""
     K = thetas.shape[0]
     if (mus is None) != (mu_out is None):
         raise ValueError("select_theta: mus and mu_out go together")
@@ -193,6 +194,8 @@ def select_theta(pred, groups, thetas, cost, lam, fallback, out, n_fallback=None
                                     ptr(out, _F32, "theta_group", numel=groups),
                                     None if mu_out is None else
                                     ptr(mu_out, _F32, "mu_group", numel=groups),
+                                    None if k_out is None else
+                                    ptr(k_out, torch.int32, "k_group", numel=groups),
                                     None if n_fallback is None else
                                     ptr(n_fallback, (torch.int64, torch.uint64), "n_fallback"),
                                     stream_handle()), "select_theta")

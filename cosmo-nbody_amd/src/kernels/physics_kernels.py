@@ -140,13 +140,22 @@ def traverse_split(pos_x, pos_y, pos_z, nodes, root, force, theta, G, soft, box,
         "traverse_split")
 
 
-def order_groups_by_pops(group_pops, out=None):
-    """Launch order of the groups by decreasing pop count (longest first)."""
+def order_groups_by_pops(group_pops, out=None, k_now=None, k_prev=None, cost=None):
+    """Launch order of the groups by decreasing pop count (longest first).
+    With a per-group selection, k_now / k_prev (int32 per group: the chosen
+    candidate now and at the counting launch) and cost (float32 per
+    candidate) scale each group's pops by cost[k_now] / cost[k_prev]."""
     g = group_pops.shape[0]
     out = torch.empty(g, dtype=torch.int32, device=group_pops.device) if out is None else out
-    check(load().cosmo_order_groups_by_pops(ptr(group_pops, U32, "group_pops", numel=g), g,
-                                            ptr(out, torch.int32, "order", numel=g),
-                                            stream_handle()), "order_groups_by_pops")
+    if (k_now is None) != (k_prev is None) or (k_now is not None and cost is None):
+        raise ValueError("order_groups_by_pops: k_now, k_prev and cost go together")
+    K = 0 if cost is None else cost.shape[0]
+    check(load().cosmo_order_groups_by_pops(
+        ptr(group_pops, U32, "group_pops", numel=g), g,
+        None if k_now is None else ptr(k_now, torch.int32, "k_now", numel=g),
+        None if k_prev is None else ptr(k_prev, torch.int32, "k_prev", numel=g),
+        None if cost is None else ptr(cost, _F32, "cost", numel=K), K,
+        ptr(out, torch.int32, "order", numel=g), stream_handle()), "order_groups_by_pops")
     return out
 
 

@@ -333,9 +333,14 @@ int cosmo_order_groups(const float *theta_group, int64_t groups, int32_t *order,
 /* order[0..groups) = the groups sorted by decreasing group_pops (the pop
  * counts an earlier cosmo_traverse_ex stored; 64 bins of a third of an
  * octave; arbitrary order inside a bin): a longest-first launch order for
- * the next launch over the same groups (no reference counterpart). */
-int cosmo_order_groups_by_pops(const uint32_t *group_pops, int64_t groups, int32_t *order,
-                               void *stream);
+ * the next launch over the same groups (no reference counterpart).  With a
+ * per-group selection (cosmo_select_theta's k_group) k_now and k_prev
+ * (nullable together) give each group's candidate now and at the launch
+ * that counted the pops; a group's pops are then scaled by cost[k_now] /
+ * cost[k_prev] (cost: device float[K], both indices in [0, K)). */
+int cosmo_order_groups_by_pops(const uint32_t *group_pops, int64_t groups,
+                               const int32_t *k_now, const int32_t *k_prev, const float *cost,
+                               int K, int32_t *order, void *stream);
 
 /* src/kernels/physics_kernels.py:14-153 compute_forces(pos_x, pos_y, pos_z,
  * children, n_mass, n_com, n_min, n_max, force, theta, G, soft, box, root,

@@ -130,13 +130,14 @@ int cosmo_mlp_to_bf16(const float *params, uint16_t *wbf, void *stream);
 
 /* With k* = argmin_k cost[k] + lambda * 2^pred[g*K+k]: theta_group[g] =
  * thetas[k*] and, when mu_group is given, mu_group[g] = mus[k*] (the
- * candidates' straddle masses).  A group with any non-finite prediction
- * gets `fallback` and `fallback_mu` and increments *n_fallback (nullable,
- * device uint64). */
+ * candidates' straddle masses); when k_group is given, k_group[g] = k*.  A
+ * group with any non-finite prediction gets `fallback` and `fallback_mu`
+ * (k_group -1) and increments *n_fallback (nullable, device uint64). */
 int cosmo_select_theta(const float *pred, int64_t groups, const float *thetas,
                        const float *mus, const float *cost, int K, double lambda,
                        double fallback, double fallback_mu, float *theta_group,
-                       float *mu_group, uint64_t *n_fallback, void *stream);
+                       float *mu_group, int32_t *k_group, uint64_t *n_fallback,
+                       void *stream);
 
 #ifdef __cplusplus
 }

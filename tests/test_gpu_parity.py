@@ -684,6 +684,20 @@ def test_traverse_group_pops_and_longest_first(orc, torch_mod, ics32):
     po = 1.0 + host(pops)[o].astype(np.float64)
     later_max = np.maximum.accumulate(po[::-1])[::-1]
     assert np.all(later_max <= po * 2.0 ** (1.0 / 3.0) * (1.0 + 1e-5))
+    # scaled by the change of each group's candidate cost: the estimate
+    # pops * cost[k_now] / cost[k_prev] orders the groups (to within a bin);
+    # candidate -1 (none) keeps the plain pops
+    rng = np.random.default_rng(11)
+    cost = torch.tensor([8.0, 2.0, 1.0, 0.5], dtype=torch.float32, device="cuda")
+    kn = rng.integers(-1, 4, W).astype(np.int32)
+    kp = rng.integers(-1, 4, W).astype(np.int32)
+    so = host(order_groups_by_pops(pops, None, dev(kn, torch), dev(kp, torch), cost))
+    assert np.array_equal(np.sort(so), np.arange(W))
+    c = host(cost).astype(np.float64)
+    scale = np.where((kn >= 0) & (kp >= 0), c[np.maximum(kn, 0)] / c[np.maximum(kp, 0)], 1.0)
+    est = 1.0 + host(pops).astype(np.float64)[so] * scale[so]
+    later_max = np.maximum.accumulate(est[::-1])[::-1]
+    assert np.all(later_max <= est * 2.0 ** (1.0 / 3.0) * (1.0 + 1e-5))
     f1 = torch.zeros((n, 3), dtype=torch.float32, device="cuda")
     pops2 = torch.zeros(W, dtype=torch.int32, device="cuda")
     traverse(*P, nodes, root, f1, 0.4, 1.0, 0.02 / 32, 1.0, mac=1, group_order=order,

@@ -49,17 +49,21 @@ def test_graph_replays_match_eager(mac, longest_first):
         graph.step_graph()
 
 
-def test_graph_with_learned_theta_matches_eager():
+@pytest.mark.parametrize("longest_first", [False, True])
+def test_graph_with_learned_theta_matches_eager(longest_first):
     """The learned per-group theta (features from the step's tree and scale
     factor, MLP, selection) inside the captured step: replays equal eager
-    steps bit for bit, theta_group included."""
+    steps bit for bit, theta_group included.  With LONGEST_FIRST the
+    captured step also orders the groups by the previous launch's pops
+    scaled by the change of their candidates (the two parities' graphs
+    swap the candidate buffers as eager steps do)."""
     import torch
 
     from src.core.learned_theta import DEFAULT_THETAS, LearnedTheta, ThetaModel
     cost = np.linspace(4000.0, 500.0, len(DEFAULT_THETAS))
     engines = []
     for _ in range(2):
-        e = _engine(mac="aabb")
+        e = _engine(mac="aabb", LONGEST_FIRST=longest_first)
         e.theta_selector = LearnedTheta(ThetaModel(seed=3), DEFAULT_THETAS, cost, 50.0, 0.5)
         engines.append(e)
     eager, graph = engines

@@ -223,14 +223,18 @@ def test_select_pairs_and_rows():
     out = torch.empty(G, dtype=torch.float32, device="cuda")
     mout = torch.empty(G, dtype=torch.float32, device="cuda")
     nf = torch.zeros(1, dtype=torch.int64, device="cuda")
+    kout = torch.empty(G, dtype=torch.int32, device="cuda")
     K.select_theta(dv(pred.reshape(-1)), G, dv(th), dv(cost), 3.0, 0.45, out, nf, mus=dv(mus),
-                   fallback_mu=3e-5, mu_out=mout)
+                   fallback_mu=3e-5, mu_out=mout, k_out=kout)
     k = np.argmin(cost[None, :] + 3.0 * np.exp2(pred), 1)
     o, mo = out.cpu().numpy(), mout.cpu().numpy()
     ok = np.ones(G, bool)
     ok[11] = False
     assert np.array_equal(o[ok], th[k][ok]) and np.array_equal(mo[ok], mus[k][ok])
     assert o[11] == np.float32(0.45) and mo[11] == np.float32(3e-5) and int(nf.item()) == 1
+    # the chosen candidate's index, -1 for the fallback
+    ko = kout.cpu().numpy()
+    assert np.array_equal(ko[ok], k[ok]) and ko[11] == -1
     feat = dv(rng.standard_normal((7, K.GROUP_FEATURES)).astype(np.float32))
     codes = dv(np.log2(1.0 + mus * 2 ** 21).astype(np.float32))
     X = K.assemble_rows(feat, dv(th), cand2=codes).cpu().numpy().view(np.uint16)
