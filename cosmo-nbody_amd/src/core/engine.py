@@ -99,6 +99,9 @@ class SimulationEngine:
             groups = (state.n + 63) // 64
             self.group_pops = torch.zeros(groups, dtype=torch.int32, device=device)
             self._pops_order = torch.zeros(groups, dtype=torch.int32, device=device)
+        # a per-group selection since the last launch (its pops then belong to
+        # the previous candidates)
+        self._fresh_selection = False
         self.steps_done = 0
         # failure detection (health=True): the traversal's counters (its
         # overflow drops among them) and the non-finite accelerations,
@@ -130,14 +133,15 @@ class SimulationEngine:
         if self.theta_selector is not None:
             self.theta_group = self.theta_selector.theta_groups(self, a=self._a_dev)
             self.straddle_group = getattr(self.theta_selector, "mu_group", None)
+            self._fresh_selection = True
             if not self.longest_first:
                 self.group_order = self.theta_selector.launch_order(self.theta_group)
 
     def launch_order(self, scale=True):
         """The traversal's launch order of the groups: by the previous
-        launch's pops (LONGEST_FIRST; with a learned selection and scale, each
-        group's pops scaled by the change of its candidate's cost since that
-        launch), else by the learned thetas, else none."""
+        launch's pops (LONGEST_FIRST; with scale after a new learned
+        selection, each group's pops scaled by the change of its candidate's
+        cost since that launch), else by the learned thetas, else none."""
         if self.longest_first:
             sel = self.theta_selector
             if scale and sel is not None and self.theta_group is not None:
@@ -156,8 +160,9 @@ class SimulationEngine:
                  compensated=bool(getattr(c, "COMPENSATED", False)), stats=stats,
                  pairs=self.pairs, straddle_mass=self.straddle_mass,
                  straddle_group=self.straddle_group if self.theta_group is not None else None,
-                 group_order=self.launch_order(),
+                 group_order=self.launch_order(scale=self._fresh_selection),
                  group_pops=self.group_pops if self.longest_first else None)
+        self._fresh_selection = False
         if self.health:
             count_nonfinite(s.force, self._nonfinite)
             self._health_steps += 1
