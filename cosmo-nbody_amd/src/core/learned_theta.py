@@ -37,20 +37,23 @@ DEFAULT_THETAS = (0.04, 0.05, 0.06, 0.07, 0.08, 0.1, 0.12, 0.15, 0.2, 0.25, 0.3,
                   0.8, 1.0, 1.4, 2.0)
 LABEL_FLOOR = -40.0   # log2 of the smallest squared-error share a label keeps
 # Candidates of the joint selection: opening angles x straddle masses (the
-# configuration's straddle mass times these factors).
+# configuration's straddle mass times these factors: 1/4 to 4 in steps of
+# sqrt(2) above 1/2).
 PAIR_THETAS = (0.1, 0.12, 0.15, 0.2, 0.25, 0.3, 0.4, 0.5, 0.6, 0.8, 1.0, 1.4)
-MU_FACTORS = (0.25, 0.5, 1.0, 2.0, 4.0)
+MU_FACTORS = (0.25, 0.5, 0.7071, 1.0, 1.4142, 2.0, 2.8284, 4.0)
+FIT_STEPS = 80000      # Adam steps per network of the ensemble
 
 
-def candidates(c, thetas=None, pairs=True):
+def candidates(c, thetas=None, pairs=True, mu_factors=None):
     """(thetas, mus) of the candidates for configuration c: every theta with
-    every straddle mass c.STRADDLE_MASS * MU_FACTORS when pairs is set and the
-    AABB test has a straddle mass, else each theta with c.STRADDLE_MASS."""
+    every straddle mass c.STRADDLE_MASS * mu_factors (default MU_FACTORS)
+    when pairs is set and the AABB test has a straddle mass, else each theta
+    with c.STRADDLE_MASS."""
     mu0 = float(getattr(c, "STRADDLE_MASS", 0.0))
     if pairs and mu0 > 0:
         th = PAIR_THETAS if thetas is None else tuple(thetas)
-        return ([t for _ in MU_FACTORS for t in th],
-                [mu0 * f for f in MU_FACTORS for _ in th])
+        fs = MU_FACTORS if mu_factors is None else tuple(mu_factors)
+        return ([t for _ in fs for t in th], [mu0 * f for f in fs for _ in th])
     th = DEFAULT_THETAS if thetas is None else tuple(thetas)
     return list(th), [mu0] * len(th)
 
@@ -790,9 +793,9 @@ def evaluate(c, model, cost, thetas, val, target=1e-3, n_val_groups=1024, seed=0
 
 
 def experiment(preset="repo_128", target=1e-3, mac="aabb", states=None, n_groups=2048,
-               n_val_groups=1024, fit_steps=20000, seed=0, thetas=None,
+               n_val_groups=1024, fit_steps=FIT_STEPS, seed=0, thetas=None,
                target_theta=None, lambdas=None, log=print, overrides=None, val_states=None,
-               pairs=True):
+               pairs=True, mu_factors=None):
     """Trains the network on the GPU from samples the method generates on the
     training states, then evaluates it (see evaluate) on each held-out
     state.  The candidates are opening angles, and with pairs (and a
@@ -801,7 +804,7 @@ def experiment(preset="repo_128", target=1e-3, mac="aabb", states=None, n_groups
     training info."""
     from src.utils import config as cfg
     c = cfg.preset(preset, MAC=mac, **(overrides or {}))
-    th, mus = candidates(c, thetas, pairs)
+    th, mus = candidates(c, thetas, pairs, mu_factors)
     train_states, val = default_states(c) if states is None else states
     val_states = [val] if val_states is None else list(val_states)
     model, cost, info = train(c, train_states, n_groups, th, fit_steps, seed, log, mus=mus)
@@ -821,7 +824,10 @@ def main():
     ap.add_argument("--preset", default="repo_128")
     ap.add_argument("--mac", default="aabb")
     ap.add_argument("--target", type=float, default=1e-3)
-    ap.add_argument("--fit-steps", type=int, default=20000)
+    ap.add_argument("--fit-steps", type=int, default=FIT_STEPS)
+    ap.add_argument("--mu-factors", default="",
+                    help="comma-separated straddle-mass factors of the candidates (default "
+                         + ",".join(str(f) for f in MU_FACTORS) + ")")
     ap.add_argument("--groups", type=int, default=10240)
     ap.add_argument("--target-theta", type=float, default=None)
     ap.add_argument("--val-groups", type=int, default=1024)
@@ -859,10 +865,11 @@ def main():
     if args.train_seeds:
         train_states = [(int(x), 0) for x in args.train_seeds.split(",") if x]
     vals = [val] + [(int(x), 0) for x in args.val_seeds.split(",") if x]
+    mu_f = tuple(float(x) for x in args.mu_factors.split(",") if x) or None
     if args.eval_preset:
         log = lambda m: print(m, flush=True)  # noqa: E731
         pairs = not args.theta_only
-        th_t, mus_t = candidates(c, None, pairs)
+        th_t, mus_t = candidates(c, None, pairs, mu_f)
         model, _, info = train(c, train_states, args.groups, th_t, args.fit_steps, log=log,
                                mus=mus_t)
         over_e = dict(over)
@@ -871,7 +878,7 @@ def main():
         ce = cfg.preset(args.eval_preset, MAC=args.mac, **over_e)
         th = (tuple(float(x) for x in args.eval_thetas.split(",")) if args.eval_thetas
               else None)
-        th, mus = candidates(ce, th, pairs)
+        th, mus = candidates(ce, th, pairs, mu_f)
         cost = measure_costs(ce, vals[0], th, log, mus=mus)
         for v in vals:
             res, _ = evaluate(ce, model, cost, th, v, args.target, args.val_groups, log=log,
@@ -884,7 +891,7 @@ def main():
                      n_groups=args.groups, n_val_groups=args.val_groups,
                      target_theta=args.target_theta, log=lambda m: print(m, flush=True),
                      overrides=over, val_states=vals, states=(train_states, val),
-                     pairs=not args.theta_only)
+                     pairs=not args.theta_only, mu_factors=mu_f)
     for res, lt in out:
         print(json.dumps(res, default=float), flush=True)
     if args.save and out[0][1] is not None:
