@@ -41,7 +41,7 @@ LABEL_FLOOR = -40.0   # log2 of the smallest squared-error share a label keeps
 # sqrt(2) above 1/2).
 PAIR_THETAS = (0.1, 0.12, 0.15, 0.2, 0.25, 0.3, 0.4, 0.5, 0.6, 0.8, 1.0, 1.4)
 MU_FACTORS = (0.25, 0.5, 0.7071, 1.0, 1.4142, 2.0, 2.8284, 4.0)
-FIT_STEPS = 80000      # Adam steps per network of the ensemble
+FIT_STEPS = 160000     # Adam steps per network of the ensemble
 
 
 def candidates(c, thetas=None, pairs=True, mu_factors=None):
